@@ -1,0 +1,210 @@
+/*
+ * ns3d.h -- C ABI of the MI355X implementation of NS3D's SIMPLE-family outer
+ * iteration (libns3d_gpu.so).
+ *
+ * One context holds one image on one GPU: the fluid-index arrays, the four
+ * staggered linear systems (u, v, w momentum and the pressure / pressure-
+ * correction system) in device memory, and every field the outer-loop drivers
+ * use.  Each entry point below replaces one method of the reference's
+ * NavierStokesBase class (src/NSBase.h:63-594), with the same argument
+ * meaning; the outer-loop drivers (SIMPLEC/SIMPLER/SIMPLERR/IDEAL) are host
+ * code calling these entry points in the reference's order.
+ *
+ * Conventions
+ *  - Images are uint8, 0 = fluid, 1 = solid, C order with shape (nx, ny, nz):
+ *    byte (i*ny + j)*nz + k is voxel (x=i, y=j, z=k).  Rotation for flow along
+ *    y/z and the x border are applied by the caller (ns3d_prepare_image).
+ *  - Index grids are int32, -1 = no node; u's grid has shape (nx+1, ny, nz).
+ *  - All floating-point state is FP64.  Host buffers are caller-owned and are
+ *    only read or written during the call.
+ *  - Errors are returned as ns3d_status; ns3d_last_error() gives the message
+ *    of the last failure on the calling thread.  Nothing throws across the ABI.
+ *  - One host thread per context.  Different contexts may be driven from
+ *    different threads; each context owns its own HIP stream.
+ */
+#ifndef NS3D_H
+#define NS3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ns3d_ctx ns3d_ctx;
+
+typedef enum {
+  NS3D_OK = 0,
+  NS3D_ERR_INVALID = 1, /* bad argument or state */
+  NS3D_ERR_OOM = 2,     /* device allocation failed */
+  NS3D_ERR_HIP = 3,     /* HIP runtime error */
+  NS3D_ERR_NODEV = 4    /* no usable GPU */
+} ns3d_status;
+
+/* Linear systems. */
+enum { NS3D_SYS_U = 0, NS3D_SYS_V = 1, NS3D_SYS_W = 2, NS3D_SYS_P = 3 };
+
+/* Velocity triples held by a context: (u, v, w), (u_temp, v_temp, w_temp),
+ * (u_hat, v_hat, w_hat) -- the drivers' u/u_temp/u_hat vectors
+ * (SIMPLER.h:62-83, IDEAL.h:67-89). */
+enum { NS3D_SET_VEL = 0, NS3D_SET_TEMP = 1, NS3D_SET_HAT = 2 };
+
+/* Vectors for ns3d_get_vector / ns3d_set_vector. */
+enum {
+  NS3D_VEC_U = 0, NS3D_VEC_V, NS3D_VEC_W,
+  NS3D_VEC_UT, NS3D_VEC_VT, NS3D_VEC_WT,
+  NS3D_VEC_UH, NS3D_VEC_VH, NS3D_VEC_WH,
+  NS3D_VEC_P, NS3D_VEC_PC, NS3D_VEC_B
+};
+
+/* Pressure-type systems (ns3d_assemble_pressure). */
+enum { NS3D_PRESSURE_FULL = 0, NS3D_PC_SIMPLE = 1, NS3D_PC_SIMPLEC = 2 };
+
+/* Velocity corrections (ns3d_correct_velocity). */
+enum { NS3D_CORR_SIMPLE = 0, NS3D_CORR_SIMPLEC = 1, NS3D_CORR_SIMPLER = 2 };
+
+/* Result of one linear solve, with Eigen's BiCGSTAB meaning: iterations(),
+ * info() (0 Success, 1 NumericalIssue, 2 NoConvergence, 3 InvalidInput) and
+ * error() = |r|/|b|. */
+typedef struct {
+  int64_t iterations;
+  int32_t info;
+  int32_t restarts;
+  double error;
+} ns3d_solve_info;
+
+/* Physical and numerical constants of one solve.  The per-iteration
+ * relaxation factors alpha and beta are passed to the calls that use them,
+ * since the drivers change them (NSBase.cxx:196-301). */
+typedef struct {
+  double dx, dy, dz;   /* grid spacing (dy, dz already defaulted to dx) */
+  double mu, rho;      /* viscosity, density */
+  double p0, p1;       /* inlet / outlet pressure (after NSBase.cxx:155-161) */
+  int32_t border;      /* x border added by the caller */
+  int32_t reserved;
+} ns3d_params;
+
+/* Last error message on this thread ("" if none). */
+const char* ns3d_last_error(void);
+
+/* Number of GPUs visible; <0 on a runtime error. */
+int ns3d_device_count(void);
+
+/* Rotation (direction 2: y->x, 3: z->x) and fluid border along x, on the host.
+ * Replaces NavierStokesBase::Rotation (NSBase.cxx:2451-2492) and AddBorder
+ * (NSBase.cxx:2425-2448; pads with 0 = fluid).  With out == NULL only dims is
+ * filled; out must hold dims[0]*dims[1]*dims[2] bytes. */
+ns3d_status ns3d_prepare_image(const uint8_t* img, int nx, int ny, int nz, int direction,
+                               int border, uint8_t* out, int dims[3]);
+
+/* Uploads the image and builds the fluid-index arrays with a device scan,
+ * bit-exact with NavierStokesBase::InitIndexArrays (NSBase.cxx:2495-2573),
+ * then the structure of the four systems.  device < 0 keeps the current
+ * device. */
+ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, ns3d_ctx** out);
+void ns3d_destroy(ns3d_ctx* ctx);
+
+/* np, nu, nv, nw (NSBase.h:76). */
+ns3d_status ns3d_counts(ns3d_ctx* ctx, int64_t out[4]);
+
+/* Copies pindex/vindex/windex (nx*ny*nz each) and uindex ((nx+1)*ny*nz) to
+ * host buffers; any pointer may be NULL (NSBase.h:90-93). */
+ns3d_status ns3d_get_index(ns3d_ctx* ctx, int32_t* p, int32_t* u, int32_t* v, int32_t* w);
+
+/* Sets the constants of the solve (NavierStokesBase constructor,
+ * NSBase.cxx:50-194). */
+ns3d_status ns3d_set_params(ns3d_ctx* ctx, const ns3d_params* params);
+
+/* Zeroes every velocity triple, p, pc and b (driver initialisation, e.g.
+ * SIMPLEC.cxx:95-106). */
+ns3d_status ns3d_zero_fields(ns3d_ctx* ctx);
+
+/* Linear pressure profile p = (p1-p0)*(i+1)/nx + p0 (Initpvector,
+ * NSBase.cxx:2671-2681). */
+ns3d_status ns3d_init_pressure_linear(ns3d_ctx* ctx);
+
+/* Host <-> device copies of one vector (length from ns3d_counts). */
+ns3d_status ns3d_set_vector(ns3d_ctx* ctx, int which, const double* host, int64_t n);
+ns3d_status ns3d_get_vector(ns3d_ctx* ctx, int which, double* host, int64_t n);
+
+/* dst triple := src triple (the drivers' u_temp = u, SIMPLER.cxx:161-163). */
+ns3d_status ns3d_copy_velocity(ns3d_ctx* ctx, int dst_set, int src_set);
+
+/* AssembleAllMomentum (NSBase.cxx:853-884): u, v, w rows from the VEL triple
+ * (AssembleMomentum_x/y/z, NSBase.cxx:886-1442) plus the Jacobi setup of the
+ * three solvers (compute()). */
+ns3d_status ns3d_assemble_momentum(ns3d_ctx* ctx, double alpha);
+
+/* ComputeVelocityField (NSBase.cxx:1773-1884): RHS (Assemble_RHS_u/v/w,
+ * NSBase.cxx:1445-1546), BiCGSTAB from the current VEL triple, revert on
+ * 0 iterations or failure, optional clamp to +-delta_v_max.  info[3] gets u,
+ * v, w (w untouched when nw == 0). */
+ns3d_status ns3d_compute_velocity_field(ns3d_ctx* ctx, double alpha, int limit, double delta_v_max,
+                                        double tol, int64_t maxit, ns3d_solve_info info[3]);
+
+/* Compute_u_hat / v_hat / w_hat (NSBase.cxx:1548-1755) into the HAT triple,
+ * with the off-diagonal products taken on uc_set and relaxation on uold_set. */
+ns3d_status ns3d_compute_hat(ns3d_ctx* ctx, double alpha, int uc_set, int uold_set);
+
+/* AssemblePressure (kind FULL, NSBase.cxx:478-610), AssemblePressureCorrection
+ * SIMPLE (:734-851) or SIMPLEC (:612-732) on the velocities of vel_set, plus
+ * the Jacobi setup of the pressure solver.  relax/beta apply to FULL only. */
+ns3d_status ns3d_assemble_pressure(ns3d_ctx* ctx, int kind, int relax, double beta, int vel_set);
+
+/* ComputePressureCorrection (NSBase.cxx:2337-2365): pc from the guess pc. */
+ns3d_status ns3d_compute_pressure_correction(ns3d_ctx* ctx, double tol, int64_t maxit,
+                                             ns3d_solve_info* info);
+
+/* ComputePressureField (NSBase.cxx:2383-2423): solve from p, tighten the
+ * tolerance by 10x while a solve takes 0 iterations, keep the result only if
+ * the last solve iterated. */
+ns3d_status ns3d_compute_pressure_field(ns3d_ctx* ctx, double tol, int64_t maxit,
+                                        ns3d_solve_info* info);
+
+/* CorrectVelocityFieldSIMPLE / SIMPLEC (NSBase.cxx:2082-2263): target_set +=
+ * clamped -grad(pc)/a.  CorrectVelocityFieldSIMPLER (:2265-2335): target_set
+ * = hat - grad(p)/a with p_0/p_1 at the inlet/outlet. */
+ns3d_status ns3d_correct_velocity(ns3d_ctx* ctx, int kind, double delta_v_max, int target_set,
+                                  double p_0, double p_1);
+
+/* CorrectPressureField (NSBase.cxx:2367-2380): p += beta*pc; out = beta*max,
+ * beta*min, beta*norm of pc. */
+ns3d_status ns3d_correct_pressure(ns3d_ctx* ctx, double beta, double out[3]);
+
+/* ComputeDiv (NSBase.cxx:1958-2080) on vel_set: out = MaxDeltaV, normdiv,
+ * velocityNorm, maxdiv. */
+ns3d_status ns3d_compute_div(ns3d_ctx* ctx, int vel_set, double out[4]);
+
+/* ComputePermeability (NSBase.cxx:1886-1956): out = Q, v_moy, K, p_inlet,
+ * p_outlet. */
+ns3d_status ns3d_compute_permeability(ns3d_ctx* ctx, double out[5]);
+
+/* Matrix export for tests: diag[n], off[6*n] (E,W,N,S,T,B blocks of n),
+ * nbr[6*n] (neighbour rows, the row itself where absent), b[n] when sys is
+ * NS3D_SYS_P (else may be NULL), rowsum[n] for momentum systems (may be NULL). */
+ns3d_status ns3d_get_matrix(ns3d_ctx* ctx, int sys, double* diag, double* off, int32_t* nbr,
+                            double* rowsum);
+
+/* Stand-alone Jacobi-BiCGSTAB (Eigen semantics) on a 7-point matrix given as
+ * diag[n], off[6*n], nbr[6*n] (same layout as ns3d_get_matrix); x holds the
+ * guess on entry and the solution on exit.  Used to test the solver alone. */
+ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, const int32_t* nbr,
+                                  const double* b, double* x, double tol, int64_t maxit,
+                                  ns3d_solve_info* info);
+
+/* Device time of the fused BiCGSTAB kernels since the last reset, measured
+ * with HIP events on the context's stream: out[0] = ms in SpMV-carrying
+ * kernels, out[1] = their launch count, out[2] = ms in all solver kernels,
+ * out[3] = solver kernel launches, out[4] = algorithmic bytes moved by the
+ * SpMV-carrying kernels.  Timing is only recorded after ns3d_timing(ctx, 1). */
+ns3d_status ns3d_timing(ns3d_ctx* ctx, int enable);
+ns3d_status ns3d_timing_read(ns3d_ctx* ctx, double out[5]);
+
+/* Synchronises the context's stream. */
+ns3d_status ns3d_sync(ns3d_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NS3D_H */

@@ -1,0 +1,728 @@
+// capi.hip -- the C ABI of libns3d_gpu.so (include/ns3d.h): context
+// management, device memory, and the host side of each NavierStokesBase
+// method (the loop control of the linear solves, scalar post-processing).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ns3d.h"
+#include "kernels.h"
+
+using namespace ns3d;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HipFail {
+  hipError_t e;
+  const char* what;
+};
+
+#define HCHK(expr)                                       \
+  do {                                                   \
+    hipError_t e_ = (expr);                              \
+    if (e_ != hipSuccess) throw HipFail{e_, #expr};      \
+  } while (0)
+
+struct Invalid {
+  std::string msg;
+};
+
+struct SysMem {
+  int n = 0;
+  int d0 = 0;
+  int32_t* grid = nullptr;   // owned by the context (index grids)
+  int32_t* cell = nullptr;
+  int32_t* nbr = nullptr;
+  double* coef = nullptr;
+  double* invdiag = nullptr;
+  double* rowsum = nullptr;
+  int64_t last_iters = 16;   // chunk-size hint for the next solve
+  SysView view() const { return SysView{n, d0, grid, cell, nbr, coef, invdiag, rowsum}; }
+};
+
+struct Timing {
+  bool on = false;
+  double spmv_ms = 0, spmv_launches = 0, all_ms = 0, all_launches = 0, spmv_bytes = 0;
+};
+
+}  // namespace
+
+struct ns3d_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nx = 0, ny = 0, nz = 0;
+  int64_t ncell = 0;
+  uint8_t* img = nullptr;
+  int32_t *pg = nullptr, *ug = nullptr, *vg = nullptr, *wg = nullptr;
+  SysMem sys[4];  // u, v, w, p
+  // fields
+  double* vel[3][3] = {{nullptr}};  // [set][component]
+  double* p = nullptr;
+  double* pc = nullptr;
+  double* b = nullptr;
+  double* vb = nullptr;     // momentum right-hand side
+  double* temp = nullptr;   // saved field for revert/clamp
+  // solver workspace
+  BWork bw{};
+  double* partials = nullptr;
+  unsigned int* counters = nullptr;  // field reductions
+  double* result = nullptr;          // reduction results (device)
+  BState* hstate = nullptr;          // pinned copy of the solver state
+  double* hresult = nullptr;         // pinned copy of results
+  ns3d_params prm{};
+  bool have_params = false;
+  std::vector<void*> allocs;
+  Timing tm;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+
+  Geo geo() const { return Geo{nx, ny, nz, pg, ug, vg, wg}; }
+
+  template <typename T>
+  T* alloc(int64_t count) {
+    void* ptr = nullptr;
+    const size_t bytes = (size_t)(count > 0 ? count : 1) * sizeof(T);
+    hipError_t e = hipMalloc(&ptr, bytes);
+    if (e != hipSuccess) throw HipFail{e, "hipMalloc"};
+    allocs.push_back(ptr);
+    return (T*)ptr;
+  }
+};
+
+namespace {
+
+ns3d_status fail_hip(const HipFail& f) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s", (int)f.e, hipGetErrorString(f.e), f.what);
+  g_err = buf;
+  return f.e == hipErrorOutOfMemory ? NS3D_ERR_OOM : NS3D_ERR_HIP;
+}
+
+ns3d_status fail_invalid(const std::string& m) {
+  g_err = m;
+  return NS3D_ERR_INVALID;
+}
+
+template <typename F>
+ns3d_status guarded(F&& f) {
+  try {
+    f();
+    return NS3D_OK;
+  } catch (const HipFail& e) {
+    return fail_hip(e);
+  } catch (const Invalid& e) {
+    return fail_invalid(e.msg);
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return NS3D_ERR_OOM;
+  }
+}
+
+void need(bool c, const char* msg) {
+  if (!c) throw Invalid{msg};
+}
+
+Phys phys_of(const ns3d_ctx* c, double alpha, double beta) {
+  Phys ph;
+  ph.dx = c->prm.dx; ph.dy = c->prm.dy; ph.dz = c->prm.dz;
+  ph.mu = c->prm.mu; ph.rho = c->prm.rho;
+  ph.p0 = c->prm.p0; ph.p1 = c->prm.p1;
+  ph.alpha = alpha; ph.beta = beta;
+  ph.sf = 0.0;  // schemefactor (NSBase.cxx:171)
+  return ph;
+}
+
+double* vec_ptr(ns3d_ctx* c, int which, int64_t& n) {
+  switch (which) {
+    case NS3D_VEC_U: case NS3D_VEC_UT: case NS3D_VEC_UH: n = c->sys[0].n; break;
+    case NS3D_VEC_V: case NS3D_VEC_VT: case NS3D_VEC_VH: n = c->sys[1].n; break;
+    case NS3D_VEC_W: case NS3D_VEC_WT: case NS3D_VEC_WH: n = c->sys[2].n; break;
+    case NS3D_VEC_P: case NS3D_VEC_PC: case NS3D_VEC_B: n = c->sys[3].n; break;
+    default: throw Invalid{"unknown vector selector"};
+  }
+  if (which <= NS3D_VEC_WH) return c->vel[which / 3][which % 3];
+  if (which == NS3D_VEC_P) return c->p;
+  if (which == NS3D_VEC_PC) return c->pc;
+  return c->b;
+}
+
+Vel3 vel3(ns3d_ctx* c, int set) {
+  if (set < 0 || set > 2) throw Invalid{"unknown velocity set"};
+  return Vel3{c->vel[set][0], c->vel[set][1], c->vel[set][2]};
+}
+
+// Records an event pair around one launch when timing is enabled.
+struct KTimer {
+  ns3d_ctx* c;
+  bool on;
+  explicit KTimer(ns3d_ctx* cc) : c(cc), on(cc->tm.on) {
+    if (on) HCHK(hipEventRecord(c->ev[0], c->stream));
+  }
+  float stop() {
+    if (!on) return 0.f;
+    HCHK(hipEventRecord(c->ev[1], c->stream));
+    HCHK(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    return ms;
+  }
+};
+
+// Algorithmic bytes of one SpMV-carrying BiCGSTAB kernel (A or B): 7 stored
+// coefficients + 6 neighbour indices per row are counted as nnz*(8+4) with
+// nnz the rows' stored entries; A streams r, p, v, M^-1, r0 and writes p, v;
+// B streams r, v, M^-1 and writes t.
+double kernel_bytes(int64_t n, int64_t nnz, bool is_a) {
+  const double mat = (double)nnz * 12.0;
+  return mat + (is_a ? 7.0 : 4.0) * 8.0 * (double)n;
+}
+
+// One BiCGSTAB solve on system sid with x as guess (Eigen semantics).
+void bicgstab(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int64_t maxit,
+              ns3d_solve_info* info, int64_t nnz_hint) {
+  SysMem& S = c->sys[sid];
+  const SysView V = S.view();
+  hipStream_t s = c->stream;
+  if (S.n == 0) {
+    info->iterations = 0; info->info = 0; info->restarts = 0; info->error = 0;
+    return;
+  }
+  BWork w = c->bw;
+  HCHK(launch_bicg_init(V, x, b, w, tol, (long long)maxit, s));
+  HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
+  HCHK(hipStreamSynchronize(s));
+  int64_t chunk = S.last_iters > 4 ? (S.last_iters * 3) / 4 : 4;
+  int64_t launched = 0;
+  const int64_t cap = 4 * (maxit > 0 ? maxit : 1) + 64;
+  long long exec_before = 0;
+  while (c->hstate->status == BS_RUNNING || c->hstate->status == BS_RESTART) {
+    if (c->hstate->status == BS_RESTART) {
+      HCHK(launch_bicg_restart(V, x, b, w, s));
+    }
+    if (chunk > maxit + 1) chunk = maxit + 1;
+    if (c->tm.on) {
+      // time A and B launches individually (they carry the SpMV)
+      for (int64_t q = 0; q < chunk; ++q) {
+        KTimer ta(c);
+        HCHK(launch_bicg_A(V, w, s));
+        float ma = ta.stop();
+        KTimer tb(c);
+        HCHK(launch_bicg_B(V, w, s));
+        float mb = tb.stop();
+        KTimer tc(c);
+        HCHK(launch_bicg_C(V, x, w, s));
+        float mc = tc.stop();
+        HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
+        HCHK(hipStreamSynchronize(s));
+        if (c->hstate->nexec > exec_before) {
+          exec_before = c->hstate->nexec;
+          c->tm.spmv_ms += ma + mb;
+          c->tm.spmv_launches += 2;
+          c->tm.all_ms += ma + mb + mc;
+          c->tm.all_launches += 3;
+          c->tm.spmv_bytes += kernel_bytes(S.n, nnz_hint, true) + kernel_bytes(S.n, nnz_hint, false);
+        }
+        if (c->hstate->status != BS_RUNNING) break;
+      }
+    } else {
+      for (int64_t q = 0; q < chunk; ++q) {
+        HCHK(launch_bicg_A(V, w, s));
+        HCHK(launch_bicg_B(V, w, s));
+        HCHK(launch_bicg_C(V, x, w, s));
+      }
+      HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
+      HCHK(hipStreamSynchronize(s));
+    }
+    launched += chunk;
+    if (launched > cap) throw Invalid{"BiCGSTAB did not stop within its iteration cap"};
+    chunk = 16;
+  }
+  const BState& st = *c->hstate;
+  if (st.status == BS_ZERO_RHS) {
+    HCHK(launch_fill(x, S.n, 0.0, s));
+    info->iterations = maxit;
+    info->error = tol;
+    info->info = 0;
+    info->restarts = 0;
+  } else {
+    info->iterations = st.i;
+    info->error = std::sqrt(st.rr / st.rhs_sqnorm);
+    info->info = (info->error <= tol) ? 0 : 2;
+    info->restarts = (int32_t)st.restarts;
+    S.last_iters = st.nexec > 0 ? st.nexec : 4;
+  }
+}
+
+int64_t approx_nnz(ns3d_ctx* c, int sid) {
+  // 6.6 stored entries per row on sphere packs; used only for byte accounting.
+  return (int64_t)((double)c->sys[sid].n * 6.6);
+}
+
+void alloc_system(ns3d_ctx* c, int sid, int n, int d0, int32_t* grid, bool momentum) {
+  SysMem& S = c->sys[sid];
+  S.n = n;
+  S.d0 = d0;
+  S.grid = grid;
+  S.cell = c->alloc<int32_t>(n);
+  S.nbr = c->alloc<int32_t>((int64_t)6 * n);
+  S.coef = c->alloc<double>((int64_t)7 * n);
+  S.invdiag = c->alloc<double>(n);
+  S.rowsum = momentum ? c->alloc<double>(n) : nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ns3d_last_error(void) { return g_err.c_str(); }
+
+int ns3d_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return -1;
+  return n;
+}
+
+ns3d_status ns3d_prepare_image(const uint8_t* img, int nx, int ny, int nz, int direction, int border,
+                               uint8_t* out, int dims[3]) {
+  if (!img || !dims || nx <= 0 || ny <= 0 || nz <= 0 || border < 0 || direction < 1 || direction > 3)
+    return fail_invalid("ns3d_prepare_image: bad arguments");
+  int ox = nx, oy = ny, oz = nz;
+  if (direction == 2) { ox = ny; oy = nx; oz = nz; }
+  if (direction == 3) { ox = nz; oy = ny; oz = nx; }
+  dims[0] = ox + 2 * border;
+  dims[1] = oy;
+  dims[2] = oz;
+  if (!out) return NS3D_OK;
+  const size_t plane = (size_t)oy * oz;
+  std::memset(out, 0, (size_t)dims[0] * plane);
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        const uint8_t v = img[((size_t)i * ny + j) * nz + k];
+        int a, bq, cq;
+        if (direction == 2) { a = j; bq = nx - 1 - i; cq = k; }       // NSBase.cxx:2468
+        else if (direction == 3) { a = k; bq = j; cq = i; }           // NSBase.cxx:2482
+        else { a = i; bq = j; cq = k; }
+        out[(size_t)(a + border) * plane + (size_t)bq * oz + cq] = v;
+      }
+  return NS3D_OK;
+}
+
+ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, ns3d_ctx** out) {
+  if (!img || !out || nx <= 0 || ny <= 0 || nz <= 0) return fail_invalid("ns3d_create: bad arguments");
+  const int64_t ncell = (int64_t)nx * ny * nz;
+  if (ncell + (int64_t)ny * nz >= ((int64_t)1 << 31) - 1)
+    return fail_invalid("ns3d_create: image too large for 32-bit node indices");
+  *out = nullptr;
+  ns3d_ctx* c = new ns3d_ctx();
+  ns3d_status st = guarded([&] {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw Invalid{"no GPU visible"};
+    if (device >= 0) {
+      if (device >= ndev) throw Invalid{"ns3d_create: device index out of range"};
+      HCHK(hipSetDevice(device));
+    }
+    HCHK(hipGetDevice(&c->device));
+    HCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HCHK(hipEventCreate(&c->ev[0]));
+    HCHK(hipEventCreate(&c->ev[1]));
+    c->nx = nx; c->ny = ny; c->nz = nz; c->ncell = ncell;
+    hipStream_t s = c->stream;
+    c->img = c->alloc<uint8_t>(ncell);
+    HCHK(hipMemcpyAsync(c->img, img, (size_t)ncell, hipMemcpyHostToDevice, s));
+    const int64_t ugrid = (int64_t)(nx + 1) * ny * nz;
+    c->pg = c->alloc<int32_t>(ncell);
+    c->vg = c->alloc<int32_t>(ncell);
+    c->wg = c->alloc<int32_t>(ncell);
+    c->ug = c->alloc<int32_t>(ugrid);
+    ScanBuffers sb;
+    sb.ncell = ncell;
+    sb.nblocks = (int)((ncell + kScanCells - 1) / kScanCells);
+    sb.blockcounts = c->alloc<int32_t>((int64_t)4 * sb.nblocks);
+    sb.blockoffsets = c->alloc<int32_t>((int64_t)4 * sb.nblocks);
+    sb.totals = c->alloc<int32_t>(4);
+    HCHK(launch_scan_count(c->img, nx, ny, nz, sb, s));
+    int32_t tot[4];
+    HCHK(hipMemcpyAsync(tot, sb.totals, sizeof(tot), hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    const int np = tot[0], nu = tot[1], nv = tot[2], nw = tot[3];
+    alloc_system(c, 0, nu, nx + 1, c->ug, true);
+    alloc_system(c, 1, nv, nx, c->vg, true);
+    alloc_system(c, 2, nw, nx, c->wg, true);
+    alloc_system(c, 3, np, nx, c->pg, false);
+    HCHK(launch_scan_write(c->img, nx, ny, nz, sb, c->pg, c->ug, c->vg, c->wg, c->sys[3].cell,
+                           c->sys[0].cell, c->sys[1].cell, c->sys[2].cell, s));
+    const Geo g = c->geo();
+    for (int q = 0; q < 4; ++q) HCHK(launch_build_nbr(g, q, c->sys[q].cell, c->sys[q].n, c->sys[q].nbr, s));
+    for (int set = 0; set < 3; ++set)
+      for (int comp = 0; comp < 3; ++comp) c->vel[set][comp] = c->alloc<double>(c->sys[comp].n);
+    c->p = c->alloc<double>(np);
+    c->pc = c->alloc<double>(np);
+    c->b = c->alloc<double>(np);
+    int nmax = np;
+    for (int q = 0; q < 3; ++q) nmax = std::max(nmax, c->sys[q].n);
+    c->vb = c->alloc<double>(nmax);
+    c->temp = c->alloc<double>(nmax);
+    BWork& w = c->bw;
+    w.r = c->alloc<double>(nmax);
+    w.r0 = c->alloc<double>(nmax);
+    w.p[0] = c->alloc<double>(nmax);
+    w.p[1] = c->alloc<double>(nmax);
+    w.v[0] = c->alloc<double>(nmax);
+    w.v[1] = c->alloc<double>(nmax);
+    w.t = c->alloc<double>(nmax);
+    const int64_t pb = padded_blocks(std::max<int64_t>(nmax, (int64_t)ny * nz));
+    c->partials = c->alloc<double>(8 * pb);
+    w.partials = c->partials;
+    w.st = c->alloc<BState>(1);
+    HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), s));
+    c->counters = c->alloc<unsigned int>(32);
+    HCHK(hipMemsetAsync(c->counters, 0, 32 * sizeof(unsigned int), s));
+    c->result = c->alloc<double>(16);
+    HCHK(hipHostMalloc((void**)&c->hstate, sizeof(BState), hipHostMallocDefault));
+    HCHK(hipHostMalloc((void**)&c->hresult, 16 * sizeof(double), hipHostMallocDefault));
+    HCHK(hipStreamSynchronize(s));
+  });
+  if (st != NS3D_OK) {
+    ns3d_destroy(c);
+    return st;
+  }
+  const ns3d_status z = ns3d_zero_fields(c);
+  if (z != NS3D_OK) {
+    ns3d_destroy(c);
+    return z;
+  }
+  *out = c;
+  return NS3D_OK;
+}
+
+void ns3d_destroy(ns3d_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->hstate) (void)hipHostFree(c->hstate);
+  if (c->hresult) (void)hipHostFree(c->hresult);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+ns3d_status ns3d_counts(ns3d_ctx* c, int64_t out[4]) {
+  if (!c || !out) return fail_invalid("ns3d_counts: null argument");
+  out[0] = c->sys[3].n; out[1] = c->sys[0].n; out[2] = c->sys[1].n; out[3] = c->sys[2].n;
+  return NS3D_OK;
+}
+
+ns3d_status ns3d_get_index(ns3d_ctx* c, int32_t* p, int32_t* u, int32_t* v, int32_t* w) {
+  if (!c) return fail_invalid("ns3d_get_index: null context");
+  return guarded([&] {
+    const size_t nc = (size_t)c->ncell * 4, nuc = (size_t)(c->nx + 1) * c->ny * c->nz * 4;
+    if (p) HCHK(hipMemcpyAsync(p, c->pg, nc, hipMemcpyDeviceToHost, c->stream));
+    if (u) HCHK(hipMemcpyAsync(u, c->ug, nuc, hipMemcpyDeviceToHost, c->stream));
+    if (v) HCHK(hipMemcpyAsync(v, c->vg, nc, hipMemcpyDeviceToHost, c->stream));
+    if (w) HCHK(hipMemcpyAsync(w, c->wg, nc, hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+ns3d_status ns3d_set_params(ns3d_ctx* c, const ns3d_params* prm) {
+  if (!c || !prm) return fail_invalid("ns3d_set_params: null argument");
+  if (!(prm->dx > 0) || !(prm->dy > 0) || !(prm->dz > 0)) return fail_invalid("ns3d_set_params: spacing must be > 0");
+  if (prm->border < 0 || 2 * prm->border >= c->nx) return fail_invalid("ns3d_set_params: border does not fit the image");
+  c->prm = *prm;
+  c->have_params = true;
+  return NS3D_OK;
+}
+
+ns3d_status ns3d_zero_fields(ns3d_ctx* c) {
+  if (!c) return fail_invalid("ns3d_zero_fields: null context");
+  return guarded([&] {
+    for (int set = 0; set < 3; ++set)
+      for (int comp = 0; comp < 3; ++comp) HCHK(launch_fill(c->vel[set][comp], c->sys[comp].n, 0.0, c->stream));
+    HCHK(launch_fill(c->p, c->sys[3].n, 0.0, c->stream));
+    HCHK(launch_fill(c->pc, c->sys[3].n, 0.0, c->stream));
+    HCHK(launch_fill(c->b, c->sys[3].n, 0.0, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+ns3d_status ns3d_init_pressure_linear(ns3d_ctx* c) {
+  if (!c || !c->have_params) return fail_invalid("ns3d_init_pressure_linear: parameters not set");
+  return guarded([&] {
+    HCHK(launch_init_pressure_linear(c->geo(), c->sys[3].cell, c->sys[3].n, c->p, c->prm.p0, c->prm.p1,
+                                     c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+ns3d_status ns3d_set_vector(ns3d_ctx* c, int which, const double* host, int64_t n) {
+  if (!c || !host) return fail_invalid("ns3d_set_vector: null argument");
+  return guarded([&] {
+    int64_t len;
+    double* d = vec_ptr(c, which, len);
+    need(n == len, "ns3d_set_vector: length mismatch");
+    if (len) HCHK(hipMemcpyAsync(d, host, (size_t)len * 8, hipMemcpyHostToDevice, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+ns3d_status ns3d_get_vector(ns3d_ctx* c, int which, double* host, int64_t n) {
+  if (!c || !host) return fail_invalid("ns3d_get_vector: null argument");
+  return guarded([&] {
+    int64_t len;
+    double* d = vec_ptr(c, which, len);
+    need(n == len, "ns3d_get_vector: length mismatch");
+    if (len) HCHK(hipMemcpyAsync(host, d, (size_t)len * 8, hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+ns3d_status ns3d_copy_velocity(ns3d_ctx* c, int dst, int src) {
+  if (!c || dst < 0 || dst > 2 || src < 0 || src > 2) return fail_invalid("ns3d_copy_velocity: bad set");
+  return guarded([&] {
+    if (dst == src) return;
+    for (int q = 0; q < 3; ++q)
+      if (c->sys[q].n)
+        HCHK(hipMemcpyAsync(c->vel[dst][q], c->vel[src][q], (size_t)c->sys[q].n * 8, hipMemcpyDeviceToDevice,
+                            c->stream));
+  });
+}
+
+ns3d_status ns3d_assemble_momentum(ns3d_ctx* c, double alpha) {
+  if (!c || !c->have_params) return fail_invalid("ns3d_assemble_momentum: parameters not set");
+  return guarded([&] {
+    const Phys ph = phys_of(c, alpha, 1.0);
+    const Geo g = c->geo();
+    const Vel3 vc = vel3(c, NS3D_SET_VEL);
+    HCHK(launch_assemble_momentum(0, g, c->sys[0].view(), vc, ph, c->stream));
+    HCHK(launch_assemble_momentum(1, g, c->sys[1].view(), vc, ph, c->stream));
+    if (c->sys[2].n > 0) HCHK(launch_assemble_momentum(2, g, c->sys[2].view(), vc, ph, c->stream));
+  });
+}
+
+ns3d_status ns3d_compute_velocity_field(ns3d_ctx* c, double alpha, int limit, double dvmax, double tol,
+                                        int64_t maxit, ns3d_solve_info info[3]) {
+  if (!c || !c->have_params || !info) return fail_invalid("ns3d_compute_velocity_field: bad arguments");
+  return guarded([&] {
+    const Phys ph = phys_of(c, alpha, 1.0);
+    const Geo g = c->geo();
+    hipStream_t s = c->stream;
+    for (int q = 0; q < 3; ++q) {
+      SysMem& S = c->sys[q];
+      if (q == 2 && S.n <= 0) break;
+      double* x = c->vel[NS3D_SET_VEL][q];
+      HCHK(launch_rhs(q, g, S.view(), c->p, x, c->vb, ph, s));
+      HCHK(hipMemcpyAsync(c->temp, x, (size_t)S.n * 8, hipMemcpyDeviceToDevice, s));
+      bicgstab(c, q, x, c->vb, tol, maxit, &info[q], approx_nnz(c, q));
+      if (info[q].iterations == 0 || info[q].info != 0)
+        HCHK(hipMemcpyAsync(x, c->temp, (size_t)S.n * 8, hipMemcpyDeviceToDevice, s));
+      if (limit) HCHK(launch_clamp(x, c->temp, S.n, dvmax, s));
+    }
+    HCHK(hipStreamSynchronize(s));
+  });
+}
+
+ns3d_status ns3d_compute_hat(ns3d_ctx* c, double alpha, int uc_set, int uold_set) {
+  if (!c || !c->have_params) return fail_invalid("ns3d_compute_hat: parameters not set");
+  return guarded([&] {
+    need(uc_set != NS3D_SET_HAT && uold_set != NS3D_SET_HAT, "ns3d_compute_hat: HAT is the output set");
+    const Phys ph = phys_of(c, alpha, 1.0);
+    const Geo g = c->geo();
+    for (int q = 0; q < 3; ++q)
+      HCHK(launch_hat(q, g, c->sys[q].view(), c->vel[uc_set][q], c->vel[uold_set][q], c->vel[NS3D_SET_HAT][q],
+                      ph, c->stream));
+  });
+}
+
+ns3d_status ns3d_assemble_pressure(ns3d_ctx* c, int kind, int relax, double beta, int vel_set) {
+  if (!c || !c->have_params) return fail_invalid("ns3d_assemble_pressure: parameters not set");
+  if (kind < 0 || kind > 2) return fail_invalid("ns3d_assemble_pressure: unknown kind");
+  return guarded([&] {
+    const Phys ph = phys_of(c, 1.0, beta);
+    HCHK(launch_assemble_pressure(c->geo(), c->sys[3].view(), c->sys[0].view(), c->sys[1].view(),
+                                  c->sys[2].view(), vel3(c, vel_set), c->p, c->b, kind, relax, ph, c->stream));
+  });
+}
+
+ns3d_status ns3d_compute_pressure_correction(ns3d_ctx* c, double tol, int64_t maxit, ns3d_solve_info* info) {
+  if (!c || !info) return fail_invalid("ns3d_compute_pressure_correction: bad arguments");
+  return guarded([&] {
+    bicgstab(c, 3, c->pc, c->b, tol, maxit, info, approx_nnz(c, 3));
+    HCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+ns3d_status ns3d_compute_pressure_field(ns3d_ctx* c, double tol, int64_t maxit, ns3d_solve_info* info) {
+  if (!c || !info) return fail_invalid("ns3d_compute_pressure_field: bad arguments");
+  return guarded([&] {
+    hipStream_t s = c->stream;
+    const int64_t n = c->sys[3].n;
+    double* ptemp = c->temp;  // temp holds max(n) doubles
+    HCHK(hipMemcpyAsync(ptemp, c->p, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    bicgstab(c, 3, ptemp, c->b, tol, maxit, info, approx_nnz(c, 3));
+    while (info->iterations == 0 && info->info == 0 && tol > 1e-15) {
+      tol *= 0.1;
+      bicgstab(c, 3, ptemp, c->b, tol, maxit, info, approx_nnz(c, 3));
+    }
+    if (info->iterations > 0) HCHK(hipMemcpyAsync(c->p, ptemp, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    HCHK(hipStreamSynchronize(s));
+  });
+}
+
+ns3d_status ns3d_correct_velocity(ns3d_ctx* c, int kind, double dvmax, int target_set, double p_0, double p_1) {
+  if (!c || !c->have_params) return fail_invalid("ns3d_correct_velocity: parameters not set");
+  if (kind < 0 || kind > 2 || target_set < 0 || target_set > 2) return fail_invalid("ns3d_correct_velocity: bad kind/set");
+  return guarded([&] {
+    const double* q = (kind == NS3D_CORR_SIMPLER) ? c->p : c->pc;
+    HCHK(launch_correct_velocity(c->geo(), c->sys[3].view(), c->sys[0].view(), c->sys[1].view(), c->sys[2].view(),
+                                 kind, q, c->vel[target_set][0], c->vel[target_set][1], c->vel[target_set][2],
+                                 c->vel[NS3D_SET_HAT][0], c->vel[NS3D_SET_HAT][1], c->vel[NS3D_SET_HAT][2], dvmax,
+                                 p_0, p_1, c->prm.dx, c->prm.dy, c->prm.dz, c->stream));
+  });
+}
+
+ns3d_status ns3d_correct_pressure(ns3d_ctx* c, double beta, double out[3]) {
+  if (!c || !out) return fail_invalid("ns3d_correct_pressure: bad arguments");
+  return guarded([&] {
+    const int64_t n = c->sys[3].n;
+    need(n > 0, "ns3d_correct_pressure: no pressure nodes");
+    HCHK(launch_correct_pressure(c->p, c->pc, n, beta, c->partials, &c->counters[0], c->result, c->stream));
+    HCHK(hipMemcpyAsync(c->hresult, c->result, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    out[0] = beta * c->hresult[0];
+    out[1] = beta * c->hresult[1];
+    out[2] = beta * std::sqrt(c->hresult[2]);
+  });
+}
+
+ns3d_status ns3d_compute_div(ns3d_ctx* c, int vel_set, double out[4]) {
+  if (!c || !c->have_params || !out) return fail_invalid("ns3d_compute_div: bad arguments");
+  return guarded([&] {
+    const Vel3 v = vel3(c, vel_set);
+    if (c->sys[3].n == 0) { out[0] = out[1] = out[2] = out[3] = 0; return; }
+    HCHK(launch_compute_div(c->geo(), c->sys[3].view(), v.u, v.v, v.w, c->prm.dx, c->prm.dy, c->prm.dz,
+                            c->partials, &c->counters[1], c->result, c->stream));
+    HCHK(hipMemcpyAsync(c->hresult, c->result, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    out[0] = c->hresult[0];
+    out[1] = std::sqrt(c->hresult[1]);
+    out[2] = c->hresult[2] / (double)c->sys[3].n;
+    out[3] = c->hresult[3];
+  });
+}
+
+ns3d_status ns3d_compute_permeability(ns3d_ctx* c, double out[5]) {
+  if (!c || !c->have_params || !out) return fail_invalid("ns3d_compute_permeability: bad arguments");
+  return guarded([&] {
+    const int border = c->prm.border;
+    HCHK(launch_permeability(c->geo(), c->vel[NS3D_SET_VEL][0], c->p, border, c->partials, &c->counters[2],
+                             c->result, c->stream));
+    HCHK(hipMemcpyAsync(c->hresult, c->result, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    const double* r = c->hresult;
+    double vm = r[0];
+    double p_in, p_out;
+    if (border <= 0) {
+      p_in = c->prm.p0;
+      p_out = c->prm.p1;
+    } else {
+      p_in = r[3];
+      p_out = r[1];
+      if (r[4] > 0) p_in /= r[4];
+      if (r[2] > 0) p_out /= r[2];
+    }
+    const double Q = vm * (c->prm.dy * c->prm.dz);
+    vm /= (double)(c->ny * c->nz);
+    double K = 0.0;
+    if (std::fabs(p_in - p_out) > 0.0)
+      K = vm * c->prm.mu * c->prm.dx * (double)(c->nx - 2 * border) / (p_in - p_out);
+    out[0] = Q; out[1] = vm; out[2] = K; out[3] = p_in; out[4] = p_out;
+  });
+}
+
+ns3d_status ns3d_get_matrix(ns3d_ctx* c, int sys, double* diag, double* off, int32_t* nbr, double* rowsum) {
+  if (!c || sys < 0 || sys > 3) return fail_invalid("ns3d_get_matrix: bad arguments");
+  return guarded([&] {
+    const SysMem& S = c->sys[sys];
+    const size_t n = (size_t)S.n;
+    hipStream_t s = c->stream;
+    if (diag) HCHK(hipMemcpyAsync(diag, S.coef, n * 8, hipMemcpyDeviceToHost, s));
+    if (off) HCHK(hipMemcpyAsync(off, S.coef + n, 6 * n * 8, hipMemcpyDeviceToHost, s));
+    if (nbr) HCHK(hipMemcpyAsync(nbr, S.nbr, 6 * n * 4, hipMemcpyDeviceToHost, s));
+    if (rowsum && S.rowsum) HCHK(hipMemcpyAsync(rowsum, S.rowsum, n * 8, hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+  });
+}
+
+ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, const int32_t* nbr,
+                                  const double* b, double* x, double tol, int64_t maxit, ns3d_solve_info* info) {
+  if (n <= 0 || !diag || !off || !nbr || !b || !x || !info) return fail_invalid("ns3d_bicgstab_stencil: bad arguments");
+  for (int64_t q = 0; q < (int64_t)6 * n; ++q)
+    if (nbr[q] < 0 || nbr[q] >= n) return fail_invalid("ns3d_bicgstab_stencil: neighbour index out of range");
+  // A throw-away context holding one "p" system of n rows.
+  ns3d_ctx* c = new ns3d_ctx();
+  ns3d_status st = guarded([&] {
+    HCHK(hipGetDevice(&c->device));
+    HCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HCHK(hipEventCreate(&c->ev[0]));
+    HCHK(hipEventCreate(&c->ev[1]));
+    hipStream_t s = c->stream;
+    SysMem& S = c->sys[3];
+    S.n = n;
+    S.nbr = c->alloc<int32_t>((int64_t)6 * n);
+    S.coef = c->alloc<double>((int64_t)7 * n);
+    S.invdiag = c->alloc<double>(n);
+    HCHK(hipMemcpyAsync(S.nbr, nbr, (size_t)6 * n * 4, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(S.coef, diag, (size_t)n * 8, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(S.coef + n, off, (size_t)6 * n * 8, hipMemcpyHostToDevice, s));
+    HCHK(launch_invdiag(S.view(), s));
+    double* db = c->alloc<double>(n);
+    double* dx = c->alloc<double>(n);
+    HCHK(hipMemcpyAsync(db, b, (size_t)n * 8, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(dx, x, (size_t)n * 8, hipMemcpyHostToDevice, s));
+    BWork& w = c->bw;
+    w.r = c->alloc<double>(n); w.r0 = c->alloc<double>(n);
+    w.p[0] = c->alloc<double>(n); w.p[1] = c->alloc<double>(n);
+    w.v[0] = c->alloc<double>(n); w.v[1] = c->alloc<double>(n);
+    w.t = c->alloc<double>(n);
+    c->partials = c->alloc<double>(8 * (int64_t)padded_blocks(n));
+    w.partials = c->partials;
+    w.st = c->alloc<BState>(1);
+    HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), s));
+    HCHK(hipHostMalloc((void**)&c->hstate, sizeof(BState), hipHostMallocDefault));
+    HCHK(hipHostMalloc((void**)&c->hresult, 16 * sizeof(double), hipHostMallocDefault));
+    bicgstab(c, 3, dx, db, tol, maxit, info, (int64_t)7 * n);
+    HCHK(hipMemcpyAsync(x, dx, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+  });
+  ns3d_destroy(c);
+  return st;
+}
+
+ns3d_status ns3d_timing(ns3d_ctx* c, int enable) {
+  if (!c) return fail_invalid("ns3d_timing: null context");
+  c->tm = Timing();
+  c->tm.on = enable != 0;
+  return NS3D_OK;
+}
+
+ns3d_status ns3d_timing_read(ns3d_ctx* c, double out[5]) {
+  if (!c || !out) return fail_invalid("ns3d_timing_read: bad arguments");
+  out[0] = c->tm.spmv_ms; out[1] = c->tm.spmv_launches; out[2] = c->tm.all_ms; out[3] = c->tm.all_launches;
+  out[4] = c->tm.spmv_bytes;
+  return NS3D_OK;
+}
+
+ns3d_status ns3d_sync(ns3d_ctx* c) {
+  if (!c) return fail_invalid("ns3d_sync: null context");
+  return guarded([&] { HCHK(hipStreamSynchronize(c->stream)); });
+}
+
+}  // extern "C"

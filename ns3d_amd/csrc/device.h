@@ -1,0 +1,185 @@
+// device.h -- device-side data views and helpers shared by the HIP kernels.
+//
+// Layout in HBM (one context = one image):
+//   * index grids, int32, C order (i slowest, k fastest): p, v, w over
+//     (nx, ny, nz); u over (nx+1, ny, nz); -1 = no node;
+//   * per linear system (u, v, w, p), structure-of-arrays over rows:
+//       cell[n]      row -> linear position in the system's grid
+//       nbr[6][n]    neighbour rows in the order E, W, N, S, T, B
+//                    (i+1, i-1, j+1, j-1, k+1, k-1); the row itself if absent
+//       coef[7][n]   0: diagonal, 1..6: E, W, N, S, T, B off-diagonals
+//                    (0 where absent)
+//       invdiag[n]   Jacobi preconditioner, rowsum[n] (momentum only)
+//   * vectors are dense FP64 over the system's rows.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ns3d {
+
+constexpr int kBlock = 256;   // threads per block for row kernels (4 wave64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kXcd = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+enum Dir { DE = 0, DW = 1, DN = 2, DS = 3, DT = 4, DB = 5 };
+
+struct SysView {
+  int n;
+  int d0;                 // grid extent along x (nx+1 for u)
+  const int32_t* grid;
+  const int32_t* cell;
+  const int32_t* nbr;     // [6][n]
+  double* coef;           // [7][n]
+  double* invdiag;
+  double* rowsum;
+};
+
+struct Geo {
+  int nx, ny, nz;
+  const int32_t* pg;
+  const int32_t* ug;
+  const int32_t* vg;
+  const int32_t* wg;
+  __device__ __forceinline__ int64_t lin(int i, int j, int k) const {
+    return ((int64_t)i * ny + j) * nz + k;
+  }
+  __device__ __forceinline__ bool in3(int i, int j, int k, int d0) const {
+    return i >= 0 && i < d0 && j >= 0 && j < ny && k >= 0 && k < nz;
+  }
+  __device__ __forceinline__ int P(int i, int j, int k) const {
+    return in3(i, j, k, nx) ? pg[lin(i, j, k)] : -1;
+  }
+  __device__ __forceinline__ int U(int i, int j, int k) const {
+    return in3(i, j, k, nx + 1) ? ug[lin(i, j, k)] : -1;
+  }
+  __device__ __forceinline__ int V(int i, int j, int k) const {
+    return in3(i, j, k, nx) ? vg[lin(i, j, k)] : -1;
+  }
+  __device__ __forceinline__ int W(int i, int j, int k) const {
+    return in3(i, j, k, nx) ? wg[lin(i, j, k)] : -1;
+  }
+  __device__ __forceinline__ void decode(int64_t c, int& i, int& j, int& k) const {
+    k = (int)(c % nz);
+    int64_t t = c / nz;
+    j = (int)(t % ny);
+    i = (int)(t / ny);
+  }
+};
+
+// SafeVectorValue (NSBase.cxx:1757-1771)
+__device__ __forceinline__ double svv(const double* vec, int idx) { return idx > -1 ? vec[idx] : 0.0; }
+
+// Maps the hardware block index to a logical block so that each XCD sweeps a
+// contiguous 1/8 of the rows (blocks are dispatched to XCDs round-robin and
+// gridDim.x is padded to a multiple of 8 by the launcher).
+__device__ __forceinline__ int logical_block() {
+  const int nb = gridDim.x;
+  const int per = nb / kXcd;
+  const int b = blockIdx.x;
+  return (b % kXcd) * per + b / kXcd;
+}
+
+inline int padded_blocks(int64_t n) {
+  int64_t nb = (n + kBlock - 1) / kBlock;
+  if (nb < 1) nb = 1;
+  nb = (nb + kXcd - 1) / kXcd * kXcd;
+  return (int)nb;
+}
+
+enum RedOp { RSUM = 0, RMAX = 1, RMIN = 2 };
+
+template <int OP>
+__device__ __forceinline__ double red_apply(double a, double b) {
+  if (OP == RSUM) return a + b;
+  if (OP == RMAX) return a > b ? a : b;
+  return a < b ? a : b;
+}
+
+template <int OP>
+__device__ __forceinline__ double red_identity() {
+  if (OP == RSUM) return 0.0;
+  if (OP == RMAX) return -__builtin_huge_val();
+  return __builtin_huge_val();
+}
+
+template <int OP>
+__device__ __forceinline__ double wave_reduce(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = red_apply<OP>(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block reduction in a fixed order (deterministic); result valid in thread 0.
+template <int OP>
+__device__ __forceinline__ double block_reduce(double v, double* sh) {
+  v = wave_reduce<OP>(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double r = v;
+  if (threadIdx.x == 0) {
+    r = sh[0];
+#pragma unroll
+    for (int q = 1; q < kWaves; ++q) r = red_apply<OP>(r, sh[q]);
+  }
+  return r;
+}
+
+__device__ __forceinline__ double red_rt(int op, double a, double b) {
+  return op == RSUM ? a + b : (op == RMAX ? (a > b ? a : b) : (a < b ? a : b));
+}
+__device__ __forceinline__ double red_id_rt(int op) {
+  return op == RSUM ? 0.0 : (op == RMAX ? -__builtin_huge_val() : __builtin_huge_val());
+}
+
+__device__ __forceinline__ double block_reduce_rt(int op, double v, double* sh) {
+  if (op == RSUM) return block_reduce<RSUM>(v, sh);
+  if (op == RMAX) return block_reduce<RMAX>(v, sh);
+  return block_reduce<RMIN>(v, sh);
+}
+
+// Grid-wide reduction of NV values (ops[q] = RSUM/RMAX/RMIN) with the "last
+// block finishes" scheme.  Each block writes its partials; the block that
+// arrives last re-reduces all partials in block order (deterministic for a
+// given grid size) and returns true in all of its threads, with out[] valid in
+// thread 0.  counter must be 0 on entry; the last block resets it to 0.
+template <int NV>
+__device__ bool grid_reduce(double (&v)[NV], const int (&ops)[NV], double* partials,
+                            unsigned int* counter, double (&out)[NV]) {
+  __shared__ double sh[NV][kWaves];
+  __shared__ int last;
+  double bv[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) bv[q] = block_reduce_rt(ops[q], v[q], sh[q]);
+  const int nb = gridDim.x;
+  const int lb = blockIdx.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) partials[(int64_t)q * nb + lb] = bv[q];
+    __threadfence();
+    unsigned int t = atomicAdd(counter, 1u);
+    last = (t == (unsigned int)(nb - 1));
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double a = red_id_rt(ops[q]);
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+      double x = __hip_atomic_load(&partials[(int64_t)q * nb + b], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+      a = red_rt(ops[q], a, x);
+    }
+    acc[q] = a;
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) out[q] = block_reduce_rt(ops[q], acc[q], sh[q]);
+  if (threadIdx.x == 0) atomicExch(counter, 0u);
+  return true;
+}
+
+}  // namespace ns3d

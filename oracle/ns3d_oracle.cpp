@@ -1,0 +1,1438 @@
+// ns3d_oracle.cpp -- CPU restatement of the NS3D SIMPLE-family outer iteration.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity oracle and the timed CPU
+// baseline; it is never linked into, loaded by or called from the product
+// library (ns3d_amd/).  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load it, and only as the checker.
+//
+// It restates, function by function, the algorithm of the reference sources
+// (file:line cited on every function) with the reference's parallel structure:
+// OpenMP over grid sweeps and sparse-matrix rows, serial dot products and
+// vector updates (Eigen's dense vector expressions are single-threaded).
+// Every floating-point expression keeps the reference's association order and
+// the file is compiled with -ffp-contract=off, so coefficient arrays are
+// bit-reproducible against any implementation that does the same.
+//
+// The sparse matrices follow Eigen's RowMajor SparseMatrix as filled by the
+// reference: at most 7 entries per row, column indices kept sorted (Eigen's
+// insert() keeps inner indices sorted), products and row sums accumulate in
+// ascending column order starting from 0.
+//
+// Third-party arithmetic restated here: Eigen 3.3/3.4 BiCGSTAB
+// (unsupported by a lockfile; the reference only requires ">= 3.3",
+// README.md:54) with DiagonalPreconditioner.  Dot products are summed
+// serially in index order; Eigen's packet-wise order differs at the rounding
+// level only.
+//
+// Parity pins (tests/test_oracle_golden.py):
+//   * index arrays: closed-form rule + fixture counts (INFO-1-spheres_128.log)
+//   * assembly + BiCGSTAB + permeability: fixture initial K and early RES rows
+//   * BiCGSTAB: dense numpy solves on small systems, Eigen zero-RHS quirk
+//   * ComputeDiv: divergence of the fixture's final fields (golden scalar)
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <list>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <chrono>
+#include <omp.h>
+
+namespace {
+
+enum { SYS_U = 0, SYS_V = 1, SYS_W = 2, SYS_P = 3 };
+enum { SET_VEL = 0, SET_TEMP = 1, SET_HAT = 2 };
+enum { PRESSURE_FULL = 0, PC_SIMPLE = 1, PC_SIMPLEC = 2 };
+enum { CORR_SIMPLE = 0, CORR_SIMPLEC = 1, CORR_SIMPLER = 2 };
+enum { METHOD_SIMPLEC = 0, METHOD_SIMPLER = 1, METHOD_SIMPLERR = 2, METHOD_IDEAL = 3 };
+
+// Vector selectors shared with the product's C ABI (include/ns3d.h).
+enum {
+  VEC_U = 0, VEC_V, VEC_W, VEC_UT, VEC_VT, VEC_WT, VEC_UH, VEC_VH, VEC_WH,
+  VEC_P, VEC_PC, VEC_B
+};
+
+// Sparse matrix in Eigen's reserved (uncompressed) RowMajor layout: 7 slots
+// per row, cnt[r] used, columns sorted ascending.
+struct Mat {
+  int n = 0;
+  std::vector<int32_t> cnt;
+  std::vector<int32_t> col;
+  std::vector<double> val;
+  void resize(int nn) {
+    n = nn;
+    cnt.assign((size_t)nn, 0);
+    col.assign((size_t)nn * 7, 0);
+    val.assign((size_t)nn * 7, 0.0);
+  }
+  double coeff(int r, int c) const {
+    for (int e = 0; e < cnt[r]; ++e)
+      if (col[(size_t)r * 7 + e] == c) return val[(size_t)r * 7 + e];
+    return 0.0;
+  }
+  double diag(int r) const { return coeff(r, r); }
+  // uA.row(r).sum(): sequential sum of the stored entries, ascending column.
+  double rowsum(int r) const {
+    double s = 0.0;
+    for (int e = 0; e < cnt[r]; ++e) s += val[(size_t)r * 7 + e];
+    return s;
+  }
+};
+
+// One matrix row being assembled: the diagonal accumulates in place, the
+// off-diagonals are inserted; finish() sorts by column like Eigen's insert().
+struct RowBuilder {
+  int row;
+  double diag = 0.0;
+  int n = 0;
+  int cols[6];
+  double vals[6];
+  explicit RowBuilder(int r) : row(r) {}
+  void insert(int c, double v) { cols[n] = c; vals[n] = v; ++n; }
+  void finish(Mat& A) const {
+    int cs[7];
+    double vs[7];
+    cs[0] = row;
+    vs[0] = diag;
+    for (int e = 0; e < n; ++e) { cs[e + 1] = cols[e]; vs[e + 1] = vals[e]; }
+    int m = n + 1;
+    for (int a = 1; a < m; ++a) {  // insertion sort by column
+      int c = cs[a];
+      double v = vs[a];
+      int b = a - 1;
+      while (b >= 0 && cs[b] > c) { cs[b + 1] = cs[b]; vs[b + 1] = vs[b]; --b; }
+      cs[b + 1] = c;
+      vs[b + 1] = v;
+    }
+    A.cnt[row] = m;
+    for (int e = 0; e < m; ++e) {
+      A.col[(size_t)row * 7 + e] = cs[e];
+      A.val[(size_t)row * 7 + e] = vs[e];
+    }
+  }
+};
+
+// Eigen::BiCGSTAB<SparseMatrix<double,RowMajor>> with DiagonalPreconditioner.
+struct Solver {
+  const Mat* A = nullptr;
+  std::vector<double> invdiag;
+  double tol = 2.220446049250313e-16;  // Eigen's default: epsilon
+  long maxit = -1;
+  long iterations = -1;  // -1: never solved (the reference reads garbage)
+  int info = 0;
+  double error = 0.0;
+};
+
+struct Params {
+  double dx, dy, dz, mu, rho, p0, gradp, alpha, beta;
+  double vsolvertol, psolvertol, pcsolvertol, deltaRMR;
+  int maxitsolver, border, save_step;
+};
+
+struct ResRow {
+  int it;
+  double Q, v_moy, RMR, K, gradP, normdiv, reldiv, velocityNorm, maxdiv;
+  long it_u, it_v, it_w, it_pc;
+};
+
+struct Ctx {
+  int nx = 0, ny = 0, nz = 0;
+  int np = 0, nu = 0, nv = 0, nw = 0;
+  std::vector<int32_t> pidx, uidx, vidx, widx;
+  // physics / numerics
+  double dx = 1, dy = 1, dz = 1, mu = 1e-3, rho = 1e3, p0 = 1e5, p1 = 0, gradp = 1e2;
+  double vsolvertol = 1e-5, psolvertol = 1e-6, pcsolvertol = 1e-3, deltaRMR = 1e-4;
+  int maxitsolver = 1000, border = 0, save_step = 20;
+  double alpha = 1, beta = 1, alpha0 = 1, beta0 = 1;
+  double p_inlet = 0, p_outlet = 0;
+  // relaxation control (NSBase.cxx:171-179)
+  double schemefactor = 0, min_alpha = 0.001, min_beta = 0.001, relax = 0.05;
+  double maxconvdelta = 2.5, RMR = 0, old_RMR = 0;
+  std::list<double> avgRMR;
+  double pcmax = 0, pcmin = 0, pcnorm = 0;
+  // matrices and solvers
+  Mat uA, vA, wA, pA;
+  Solver us, vs, ws, ps;
+  // fields
+  std::vector<double> u, v, w, ut, vt, wt, uh, vh, wh, p, pc, b;
+  // run log
+  std::vector<ResRow> rows;
+  std::vector<double> timers;  // per-phase seconds: asm, vel, hat, pasm, psolve, corr, diag
+  int final_it = 0;
+  double final_K = 0;
+
+  size_t g(int i, int j, int k) const { return ((size_t)i * ny + j) * nz + k; }
+  // index lookups that return -1 outside the grid
+  int P(int i, int j, int k) const {
+    if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz) return -1;
+    return pidx[g(i, j, k)];
+  }
+  int U(int i, int j, int k) const {
+    if (i < 0 || i > nx || j < 0 || j >= ny || k < 0 || k >= nz) return -1;
+    return uidx[g(i, j, k)];
+  }
+  int V(int i, int j, int k) const {
+    if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz) return -1;
+    return vidx[g(i, j, k)];
+  }
+  int W(int i, int j, int k) const {
+    if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz) return -1;
+    return widx[g(i, j, k)];
+  }
+  std::vector<double>& vec(int which) {
+    switch (which) {
+      case VEC_U: return u; case VEC_V: return v; case VEC_W: return w;
+      case VEC_UT: return ut; case VEC_VT: return vt; case VEC_WT: return wt;
+      case VEC_UH: return uh; case VEC_VH: return vh; case VEC_WH: return wh;
+      case VEC_P: return p; case VEC_PC: return pc; default: return b;
+    }
+  }
+};
+
+inline double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------------------
+// Image preparation (host side of the reference constructor)
+// ---------------------------------------------------------------------------
+
+// NSBase.cxx:2451-2492 (Rotation).  direction 2: temp(j, nx-1-i, k) = img(i,j,k),
+// dims (ny,nx,nz); direction 3: temp(k,j,i) = img(i,j,k), dims (nz,ny,nx).
+void rotate(const uint8_t* in, int nx, int ny, int nz, int dir, std::vector<uint8_t>& out,
+            int& ox, int& oy, int& oz) {
+  if (dir == 2) {
+    ox = ny; oy = nx; oz = nz;
+    out.assign((size_t)ox * oy * oz, 0);
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k)
+          out[((size_t)j * oy + (nx - 1 - i)) * oz + k] = in[((size_t)i * ny + j) * nz + k];
+  } else if (dir == 3) {
+    ox = nz; oy = ny; oz = nx;
+    out.assign((size_t)ox * oy * oz, 0);
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k)
+          out[((size_t)k * oy + j) * oz + i] = in[((size_t)i * ny + j) * nz + k];
+  } else {
+    ox = nx; oy = ny; oz = nz;
+    out.assign(in, in + (size_t)nx * ny * nz);
+  }
+}
+
+// NSBase.cxx:2425-2448 (AddBorder): pads with 0 (= fluid) on both x sides.
+void add_border(const std::vector<uint8_t>& in, int nx, int ny, int nz, int b,
+                std::vector<uint8_t>& out) {
+  out.assign((size_t)(nx + 2 * b) * ny * nz, 0);
+  for (int i = 0; i < nx; ++i)
+    std::memcpy(&out[(size_t)(i + b) * ny * nz], &in[(size_t)i * ny * nz], (size_t)ny * nz);
+}
+
+// NSBase.cxx:2495-2573 (InitIndexArrays): serial lexicographic numbering.
+// Per fluid cell, in this order: v (j>0 and fluid below in y), w (k>0 and
+// fluid below in z), u(i) (i==0 or fluid at i-1), u(i+1) (outlet face, only
+// at i==nx-1), p (any fluid neighbour or i==0 or i==nx-1).
+void init_index(Ctx& c, const uint8_t* img) {
+  const int nx = c.nx, ny = c.ny, nz = c.nz;
+  c.pidx.assign((size_t)nx * ny * nz, -1);
+  c.vidx.assign((size_t)nx * ny * nz, -1);
+  c.widx.assign((size_t)nx * ny * nz, -1);
+  c.uidx.assign((size_t)(nx + 1) * ny * nz, -1);
+  int np = -1, nu = -1, nv = -1, nw = -1;
+  auto fl = [&](int i, int j, int k) { return img[((size_t)i * ny + j) * nz + k] == 0; };
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        if (!fl(i, j, k)) continue;
+        bool hn = false;
+        if (j > 0 && fl(i, j - 1, k)) { nv += 1; c.vidx[c.g(i, j, k)] = nv; hn = true; }
+        if (j < ny - 1 && fl(i, j + 1, k)) hn = true;
+        if (k > 0 && fl(i, j, k - 1)) { nw += 1; c.widx[c.g(i, j, k)] = nw; hn = true; }
+        if (nz > 0) {
+          if (k < nz - 1 && fl(i, j, k + 1)) hn = true;
+        }
+        if (i == 0) { nu += 1; c.uidx[c.g(i, j, k)] = nu; hn = true; }
+        if (i > 0 && fl(i - 1, j, k)) { nu += 1; c.uidx[c.g(i, j, k)] = nu; hn = true; }
+        if (i == nx - 1) { nu += 1; c.uidx[c.g(i + 1, j, k)] = nu; hn = true; }
+        if (i < nx - 1 && fl(i + 1, j, k)) hn = true;
+        if (hn) { np += 1; c.pidx[c.g(i, j, k)] = np; }
+      }
+  c.np = np + 1; c.nu = nu + 1; c.nv = nv + 1; c.nw = nw + 1;
+}
+
+// ---------------------------------------------------------------------------
+// Sparse kernels and the Eigen BiCGSTAB restatement
+// ---------------------------------------------------------------------------
+
+// y = A*x, rows in parallel (Eigen's RowMajor sparse*dense product), each row
+// accumulated from 0 in ascending column order.
+void spmv(const Mat& A, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+  for (int r = 0; r < A.n; ++r) {
+    double t = 0.0;
+    const size_t o = (size_t)r * 7;
+    for (int e = 0; e < A.cnt[r]; ++e) t += A.val[o + e] * x[A.col[o + e]];
+    y[r] = t;
+  }
+}
+
+// Dot product in the summation order of Eigen's vectorised sum reduction
+// (Redux.h, LinearVectorizedTraversal, no unrolling) for 2-lane SSE2 packets,
+// the default x86-64 target the reference is built for: two packet
+// accumulators over blocks of 4, merged, one trailing packet, horizontal add
+// lane0+lane1, then the odd tail element.  Products are rounded before the
+// sums (no FMA).
+double dot(const std::vector<double>& a, const std::vector<double>& b) {
+  const size_t n = a.size();
+  if (n == 0) return 0.0;
+  if (n < 2) return a[0] * b[0];
+  const size_t end2 = (n / 4) * 4, end1 = (n / 2) * 2;
+  double p0 = a[0] * b[0], p1 = a[1] * b[1];
+  if (end1 > 2) {
+    double q0 = a[2] * b[2], q1 = a[3] * b[3];
+    for (size_t i = 4; i < end2; i += 4) {
+      p0 += a[i] * b[i];
+      p1 += a[i + 1] * b[i + 1];
+      q0 += a[i + 2] * b[i + 2];
+      q1 += a[i + 3] * b[i + 3];
+    }
+    p0 = p0 + q0;
+    p1 = p1 + q1;
+    if (end1 > end2) {
+      p0 += a[end2] * b[end2];
+      p1 += a[end2 + 1] * b[end2 + 1];
+    }
+  }
+  double s = p0 + p1;
+  for (size_t i = end1; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+// DiagonalPreconditioner::factorize: 1/a_jj if stored and non-zero, else 1.
+void jacobi_compute(Solver& s, const Mat& A) {
+  s.A = &A;
+  s.invdiag.assign((size_t)A.n, 1.0);
+  for (int r = 0; r < A.n; ++r) {
+    const size_t o = (size_t)r * 7;
+    for (int e = 0; e < A.cnt[r]; ++e)
+      if (A.col[o + e] == r) {
+        if (A.val[o + e] != 0.0) s.invdiag[r] = 1.0 / A.val[o + e];
+        break;
+      }
+  }
+}
+
+// Eigen's internal::bicgstab (BiCGSTAB.h) with x as the initial guess.
+void bicgstab(const Mat& A, const std::vector<double>& invdiag, const std::vector<double>& rhs,
+              std::vector<double>& x, long& iters, double& tol_error) {
+  const double tol = tol_error;
+  const long maxIters = iters;
+  const int n = A.n;
+  std::vector<double> r(n), r0(n), tmpv(n);
+  spmv(A, x.data(), tmpv.data());
+  for (int i = 0; i < n; ++i) r[i] = rhs[i] - tmpv[i];
+  r0 = r;
+  double r0_sqnorm = dot(r0, r0);
+  const double rhs_sqnorm = dot(rhs, rhs);
+  if (rhs_sqnorm == 0) {
+    std::fill(x.begin(), x.end(), 0.0);
+    return;  // iters and tol_error keep maxIters and tol
+  }
+  double rho = 1, alpha = 1, w = 1;
+  std::vector<double> v(n, 0.0), p(n, 0.0), y(n), z(n), s(n), t(n);
+  const double tol2 = tol * tol * rhs_sqnorm;
+  const double eps = 2.220446049250313e-16;
+  const double eps2 = eps * eps;
+  long i = 0;
+  long restarts = 0;
+  while (dot(r, r) > tol2 && i < maxIters) {
+    double rho_old = rho;
+    rho = dot(r0, r);
+    if (std::fabs(rho) < eps2 * r0_sqnorm) {
+      spmv(A, x.data(), tmpv.data());
+      for (int k = 0; k < n; ++k) r[k] = rhs[k] - tmpv[k];
+      r0 = r;
+      rho = r0_sqnorm = dot(r, r);
+      if (restarts++ == 0) i = 0;
+    }
+    double beta = (rho / rho_old) * (alpha / w);
+    for (int k = 0; k < n; ++k) p[k] = r[k] + beta * (p[k] - w * v[k]);
+    for (int k = 0; k < n; ++k) y[k] = invdiag[k] * p[k];
+    spmv(A, y.data(), v.data());
+    alpha = rho / dot(r0, v);
+    for (int k = 0; k < n; ++k) s[k] = r[k] - alpha * v[k];
+    for (int k = 0; k < n; ++k) z[k] = invdiag[k] * s[k];
+    spmv(A, z.data(), t.data());
+    double tmp = dot(t, t);
+    if (tmp > 0.0) w = dot(t, s) / tmp;
+    else w = 0.0;
+    for (int k = 0; k < n; ++k) x[k] += alpha * y[k] + w * z[k];
+    for (int k = 0; k < n; ++k) r[k] = s[k] - w * t[k];
+    ++i;
+  }
+  tol_error = std::sqrt(dot(r, r) / rhs_sqnorm);
+  iters = i;
+}
+
+// BiCGSTAB::solveWithGuess: iterations/error/info as Eigen reports them.
+void solve_with_guess(Solver& s, const std::vector<double>& b, std::vector<double>& x) {
+  long iters = s.maxit < 0 ? 2L * s.A->n : s.maxit;
+  double err = s.tol;
+  bicgstab(*s.A, s.invdiag, b, x, iters, err);
+  s.iterations = iters;
+  s.error = err;
+  s.info = (err <= s.tol) ? 0 : 2;
+}
+
+// ---------------------------------------------------------------------------
+// NavierStokesBase restatement
+// ---------------------------------------------------------------------------
+
+// NSBase.cxx:1757-1771 (SafeVectorValue)
+inline double svv(const std::vector<double>& vec, int idx) { return idx > -1 ? vec[idx] : 0.0; }
+
+// Convection/diffusion contribution of one face, "axial" rule (the face lies
+// between the row's own node a and the same-component neighbour b).
+// plus = true for E/N/T faces.  NSBase.cxx:917-933 / :945-961 (u x-faces),
+// :1130-1146 / :1160-1175 (v y-faces), :1365-1381 / :1394-1409 (w z-faces).
+inline void axial_face(RowBuilder& rb, int nbr, bool plus, double a, double bn, double rho,
+                       double mu, double id, double id2, double h, double sf) {
+  double F;
+  if (plus) {
+    if (a > 0 && bn > 0) F = -a * rho * id;
+    else if (a < 0 && bn < 0) F = -bn * rho * id;
+    else F = 0;
+  } else {
+    if (a < 0 && bn < 0) F = a * rho * id;
+    else if (a > 0 && bn > 0) F = bn * rho * id;
+    else F = 0;
+  }
+  if (std::fabs(F) * h * h > 2 * mu * sf) {
+    rb.diag += -mu * id2 + std::min(F, 0.0);
+    rb.insert(nbr, mu * id2 + std::max(F, 0.0));
+  } else {
+    double m = (a + bn) * 0.5;
+    double Fd = plus ? -m * rho * id : m * rho * id;
+    rb.diag += -mu * id2 + Fd * 0.5;
+    rb.insert(nbr, mu * id2 + Fd * 0.5);
+  }
+}
+
+// "Transverse" rule: m is the average of the two transverse velocities on the
+// face.  cond_ab selects the reference's two spellings of the Peclet test
+// (|m|*rho*h vs |m|*h*rho); they only differ on underflow.
+inline void trans_face(RowBuilder& rb, int nbr, bool plus, double m, double rho, double mu,
+                       double id, double id2, double h, double sf, bool rho_first) {
+  double F = plus ? -m * rho * id : m * rho * id;
+  double test = rho_first ? std::fabs(m) * rho * h : std::fabs(m) * h * rho;
+  if (test > 2 * mu * sf) {
+    rb.diag += -mu * id2 + std::min(F, 0.0);
+    rb.insert(nbr, mu * id2 + std::max(F, 0.0));
+  } else {
+    rb.diag += -mu * id2 + F * 0.5;
+    rb.insert(nbr, mu * id2 + F * 0.5);
+  }
+}
+
+// NSBase.cxx:886-1057 (AssembleMomentum_x), rows of u.
+void assemble_mom_x(Ctx& c, const std::vector<double>& uc, const std::vector<double>& vc,
+                    const std::vector<double>& wc) {
+  const double idx2 = 1 / (c.dx * c.dx), idy2 = 1 / (c.dy * c.dy), idz2 = 1 / (c.dz * c.dz);
+  const double idx = 1 / c.dx, idy = 1 / c.dy, idz = 1 / c.dz;
+  const double rho = c.rho, mu = c.mu, sf = c.schemefactor;
+  c.uA.resize(c.nu);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c.nx + 1; ++i)
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        const int row = c.U(i, j, k);
+        if (row < 0) continue;
+        RowBuilder rb(row);
+        const double a = svv(uc, row);
+        // E face
+        if (i + 1 < c.nx + 1) {
+          const int e = c.U(i + 1, j, k);
+          if (e > -1) axial_face(rb, e, true, a, svv(uc, e), rho, mu, idx, idx2, c.dx, sf);
+          else rb.diag += -mu * idx2 - (0.25 * rho * idx) * (a);
+        } else {
+          rb.diag += -(rho * idx) * (a);
+        }
+        // W face
+        if (i - 1 >= 0) {
+          const int wn = c.U(i - 1, j, k);
+          if (wn > -1) axial_face(rb, wn, false, a, svv(uc, wn), rho, mu, idx, idx2, c.dx, sf);
+          else rb.diag += -mu * idx2 + (0.25 * rho * idx) * (a);
+        } else {
+          rb.diag += (rho * idx) * (a);
+        }
+        // N face
+        if (j + 1 < c.ny) {
+          const int nn = c.U(i, j + 1, k);
+          if (nn > -1) {
+            double vm = (svv(vc, c.V(i - 1, j + 1, k)) + svv(vc, c.V(i, j + 1, k))) * 0.5;
+            trans_face(rb, nn, true, vm, rho, mu, idy, idy2, c.dy, sf, true);
+          } else {
+            rb.diag += -2 * mu * idy2;
+          }
+        } else {
+          rb.diag += -2 * mu * idy2;
+        }
+        // S face
+        if (j - 1 >= 0) {
+          const int sn = c.U(i, j - 1, k);
+          if (sn > -1) {
+            double vm = (svv(vc, c.V(i - 1, j, k)) + svv(vc, c.V(i, j, k))) * 0.5;
+            trans_face(rb, sn, false, vm, rho, mu, idy, idy2, c.dy, sf, false);
+          } else {
+            rb.diag += -2 * mu * idy2;
+          }
+        } else {
+          rb.diag += -2 * mu * idy2;
+        }
+        if (c.nz > 1) {
+          // T face
+          if (k + 1 < c.nz) {
+            const int tn = c.U(i, j, k + 1);
+            if (tn > -1) {
+              double wm = (svv(wc, c.W(i - 1, j, k + 1)) + svv(wc, c.W(i, j, k + 1))) * 0.5;
+              trans_face(rb, tn, true, wm, rho, mu, idz, idz2, c.dz, sf, true);
+            } else {
+              rb.diag += -2 * mu * idz2;
+            }
+          } else {
+            rb.diag += -2 * mu * idz2;
+          }
+          // B face
+          if (k - 1 >= 0) {
+            const int bn = c.U(i, j, k - 1);
+            if (bn > -1) {
+              double wm = (svv(wc, c.W(i - 1, j, k)) + svv(wc, c.W(i, j, k))) * 0.5;
+              trans_face(rb, bn, false, wm, rho, mu, idz, idz2, c.dz, sf, true);
+            } else {
+              rb.diag += -2 * mu * idz2;
+            }
+          } else {
+            rb.diag += -2 * mu * idz2;
+          }
+        }
+        rb.diag /= c.alpha;
+        rb.finish(c.uA);
+      }
+}
+
+// NSBase.cxx:1059-1250 (AssembleMomentum_y), rows of v.
+void assemble_mom_y(Ctx& c, const std::vector<double>& uc, const std::vector<double>& vc,
+                    const std::vector<double>& wc) {
+  const double idx2 = 1 / (c.dx * c.dx), idy2 = 1 / (c.dy * c.dy), idz2 = 1 / (c.dz * c.dz);
+  const double idx = 1 / c.dx, idy = 1 / c.dy, idz = 1 / c.dz;
+  const double rho = c.rho, mu = c.mu, sf = c.schemefactor;
+  c.vA.resize(c.nv);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c.nx; ++i)
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        const int row = c.V(i, j, k);
+        if (row < 0) continue;
+        RowBuilder rb(row);
+        const double a = svv(vc, row);
+        // E face
+        if (i + 1 < c.nx) {
+          const int e = c.V(i + 1, j, k);
+          if (e > -1) {
+            double um = (svv(uc, c.U(i + 1, j - 1, k)) + svv(uc, c.U(i + 1, j, k))) * 0.5;
+            trans_face(rb, e, true, um, rho, mu, idx, idx2, c.dx, sf, false);
+          } else {
+            rb.diag += -2 * mu * idx2;
+          }
+        } else {
+          rb.diag += -(0.5 * rho * idx) * (svv(uc, c.U(i + 1, j - 1, k)) + svv(uc, c.U(i + 1, j, k)));
+        }
+        // W face
+        if (i - 1 >= 0) {
+          const int wn = c.V(i - 1, j, k);
+          if (wn > -1) {
+            double um = (svv(uc, c.U(i, j - 1, k)) + svv(uc, c.U(i, j, k))) * 0.5;
+            trans_face(rb, wn, false, um, rho, mu, idx, idx2, c.dx, sf, false);
+          } else {
+            rb.diag += -2 * mu * idx2;
+          }
+        } else {
+          rb.diag += -mu * idx2 + (0.25 * rho * idx) * (svv(uc, c.U(i, j - 1, k)) + svv(uc, c.U(i, j, k)));
+        }
+        // N face
+        if (j + 1 < c.ny) {
+          const int nn = c.V(i, j + 1, k);
+          if (nn > -1) axial_face(rb, nn, true, a, svv(vc, nn), rho, mu, idy, idy2, c.dy, sf);
+          else rb.diag += -mu * idy2 - (0.25 * rho * idy) * (a);
+        } else {
+          rb.diag += -mu * idy2 - (0.25 * rho * idy) * (a);
+        }
+        // S face
+        if (j - 1 >= 0) {
+          const int sn = c.V(i, j - 1, k);
+          if (sn > -1) axial_face(rb, sn, false, a, svv(vc, sn), rho, mu, idy, idy2, c.dy, sf);
+          else rb.diag += -mu * idy2 + (0.25 * rho * idy) * (a);
+        } else {
+          rb.diag += -mu * idy2 + (0.25 * rho * idy) * (a);
+        }
+        if (c.nz > 1) {
+          if (k + 1 < c.nz) {
+            const int tn = c.V(i, j, k + 1);
+            if (tn > -1) {
+              double wm = (svv(wc, c.W(i, j - 1, k + 1)) + svv(wc, c.W(i, j, k + 1))) * 0.5;
+              trans_face(rb, tn, true, wm, rho, mu, idz, idz2, c.dz, sf, true);
+            } else {
+              rb.diag += -2 * mu * idz2;
+            }
+          } else {
+            rb.diag += -2 * mu * idz2;
+          }
+          if (k - 1 >= 0) {
+            const int bn = c.V(i, j, k - 1);
+            if (bn > -1) {
+              double wm = (svv(wc, c.W(i, j - 1, k)) + svv(wc, c.W(i, j, k))) * 0.5;
+              trans_face(rb, bn, false, wm, rho, mu, idz, idz2, c.dz, sf, true);
+            } else {
+              rb.diag += -2 * mu * idz2;
+            }
+          } else {
+            rb.diag += -2 * mu * idz2;
+          }
+        }
+        rb.diag /= c.alpha;
+        rb.finish(c.vA);
+      }
+}
+
+// NSBase.cxx:1252-1442 (AssembleMomentum_z), rows of w (no nz>1 guard).
+void assemble_mom_z(Ctx& c, const std::vector<double>& uc, const std::vector<double>& vc,
+                    const std::vector<double>& wc) {
+  const double idx2 = 1 / (c.dx * c.dx), idy2 = 1 / (c.dy * c.dy), idz2 = 1 / (c.dz * c.dz);
+  const double idx = 1 / c.dx, idy = 1 / c.dy, idz = 1 / c.dz;
+  const double rho = c.rho, mu = c.mu, sf = c.schemefactor;
+  c.wA.resize(c.nw);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c.nx; ++i)
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        const int row = c.W(i, j, k);
+        if (row < 0) continue;
+        RowBuilder rb(row);
+        const double a = svv(wc, row);
+        // E face
+        if (i + 1 < c.nx) {
+          const int e = c.W(i + 1, j, k);
+          if (e > -1) {
+            double um = (svv(uc, c.U(i + 1, j, k - 1)) + svv(uc, c.U(i + 1, j, k))) * 0.5;
+            trans_face(rb, e, true, um, rho, mu, idx, idx2, c.dx, sf, false);
+          } else {
+            rb.diag += -2 * mu * idx2;
+          }
+        } else {
+          rb.diag += -(0.5 * rho * idx) * (svv(uc, c.U(i + 1, j, k - 1)) + svv(uc, c.U(i + 1, j, k)));
+        }
+        // W face
+        if (i - 1 >= 0) {
+          const int wn = c.W(i - 1, j, k);
+          if (wn > -1) {
+            double um = (svv(uc, c.U(i, j, k - 1)) + svv(uc, c.U(i, j, k))) * 0.5;
+            trans_face(rb, wn, false, um, rho, mu, idx, idx2, c.dx, sf, false);
+          } else {
+            rb.diag += -2 * mu * idx2;
+          }
+        } else {
+          rb.diag += -mu * idx2 + (0.25 * rho * idx) * (svv(uc, c.U(i, j, k - 1)) + svv(uc, c.U(i, j, k)));
+        }
+        // N face
+        if (j + 1 < c.ny) {
+          const int nn = c.W(i, j + 1, k);
+          if (nn > -1) {
+            double vm = (svv(vc, c.V(i, j + 1, k - 1)) + svv(vc, c.V(i, j + 1, k))) * 0.5;
+            trans_face(rb, nn, true, vm, rho, mu, idy, idy2, c.dy, sf, true);
+          } else {
+            rb.diag += -2 * mu * idy2;
+          }
+        } else {
+          rb.diag += -2 * mu * idy2;
+        }
+        // S face
+        if (j - 1 >= 0) {
+          const int sn = c.W(i, j - 1, k);
+          if (sn > -1) {
+            double vm = (svv(vc, c.V(i, j, k - 1)) + svv(vc, c.V(i, j, k))) * 0.5;
+            trans_face(rb, sn, false, vm, rho, mu, idy, idy2, c.dy, sf, true);
+          } else {
+            rb.diag += -2 * mu * idy2;
+          }
+        } else {
+          rb.diag += -2 * mu * idy2;
+        }
+        // T face
+        if (k + 1 < c.nz) {
+          const int tn = c.W(i, j, k + 1);
+          if (tn > -1) axial_face(rb, tn, true, a, svv(wc, tn), rho, mu, idz, idz2, c.dz, sf);
+          else rb.diag += -mu * idz2 - (0.25 * rho * idz) * (a);
+        } else {
+          rb.diag += -mu * idz2 - (0.25 * rho * idz) * (a);
+        }
+        // B face
+        if (k - 1 >= 0) {
+          const int bn = c.W(i, j, k - 1);
+          if (bn > -1) axial_face(rb, bn, false, a, svv(wc, bn), rho, mu, idz, idz2, c.dz, sf);
+          else rb.diag += -mu * idz2 + (0.25 * rho * idz) * (a);
+        } else {
+          rb.diag += -mu * idz2 + (0.25 * rho * idz) * (a);
+        }
+        rb.diag /= c.alpha;
+        rb.finish(c.wA);
+      }
+}
+
+// NSBase.cxx:853-884 (AssembleAllMomentum) with compute() = Jacobi setup.
+void assemble_all_momentum(Ctx& c) {
+  assemble_mom_x(c, c.u, c.v, c.w);
+  jacobi_compute(c.us, c.uA);
+  assemble_mom_y(c, c.u, c.v, c.w);
+  jacobi_compute(c.vs, c.vA);
+  if (c.nw > 0) {
+    assemble_mom_z(c, c.u, c.v, c.w);
+    jacobi_compute(c.ws, c.wA);
+  }
+}
+
+// NSBase.cxx:1445-1487 (Assemble_RHS_u)
+void rhs_u(Ctx& c, const std::vector<double>& uo, std::vector<double>& b) {
+  const double idx = 1 / c.dx;
+  b.assign((size_t)c.nu, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c.nx + 1; ++i)
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        const int r = c.U(i, j, k);
+        if (r < 0) continue;
+        double val;
+        if (i == 0) val = (c.p[c.P(i, j, k)] - c.p0) * idx;
+        else if (i == c.nx) val = (c.p1 - c.p[c.P(i - 1, j, k)]) * idx;
+        else val = (c.p[c.P(i, j, k)] - c.p[c.P(i - 1, j, k)]) * idx;
+        val += uo[r] * c.uA.diag(r) * (1 - c.alpha);
+        b[r] = val;
+      }
+}
+
+// NSBase.cxx:1489-1518 (Assemble_RHS_v)
+void rhs_v(Ctx& c, const std::vector<double>& vo, std::vector<double>& b) {
+  const double idy = 1 / c.dy;
+  b.assign((size_t)c.nv, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c.nx; ++i)
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        const int r = c.V(i, j, k);
+        if (r < 0) continue;
+        b[r] = (c.p[c.P(i, j, k)] - c.p[c.P(i, j - 1, k)]) * idy + vo[r] * c.vA.diag(r) * (1 - c.alpha);
+      }
+}
+
+// NSBase.cxx:1520-1546 (Assemble_RHS_w)
+void rhs_w(Ctx& c, const std::vector<double>& wo, std::vector<double>& b) {
+  const double idz = 1 / c.dz;
+  b.assign((size_t)c.nw, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c.nx; ++i)
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        const int r = c.W(i, j, k);
+        if (r < 0) continue;
+        b[r] = (c.p[c.P(i, j, k)] - c.p[c.P(i, j, k - 1)]) * idz + wo[r] * c.wA.diag(r) * (1 - c.alpha);
+      }
+}
+
+// NSBase.cxx:1548-1755 (Compute_u_hat / Compute_v_hat / Compute_w_hat).
+// Neighbour order E, W, N, S, T, B; z neighbours only when nz>1 for u and v.
+void compute_hat(Ctx& c, int sys, const std::vector<double>& uc, const std::vector<double>& uold,
+                 std::vector<double>& hat) {
+  const Mat& A = sys == SYS_U ? c.uA : (sys == SYS_V ? c.vA : c.wA);
+  const int n = A.n;
+  const int d0 = sys == SYS_U ? c.nx + 1 : c.nx;
+  auto IX = [&](int i, int j, int k) {
+    return sys == SYS_U ? c.U(i, j, k) : (sys == SYS_V ? c.V(i, j, k) : c.W(i, j, k));
+  };
+  const bool zguard = (sys != SYS_W);
+  hat.assign((size_t)n, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < d0; ++i)
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        const int r = IX(i, j, k);
+        if (r < 0) continue;
+        double h = 0.0;
+        int nb;
+        if ((nb = IX(i + 1, j, k)) > -1) h -= A.coeff(r, nb) * uc[nb];
+        if ((nb = IX(i - 1, j, k)) > -1) h -= A.coeff(r, nb) * uc[nb];
+        if ((nb = IX(i, j + 1, k)) > -1) h -= A.coeff(r, nb) * uc[nb];
+        if ((nb = IX(i, j - 1, k)) > -1) h -= A.coeff(r, nb) * uc[nb];
+        if (!zguard || c.nz > 1) {
+          if ((nb = IX(i, j, k + 1)) > -1) h -= A.coeff(r, nb) * uc[nb];
+          if ((nb = IX(i, j, k - 1)) > -1) h -= A.coeff(r, nb) * uc[nb];
+        }
+        hat[r] = h / (A.diag(r)) + uold[r] * (1 - c.alpha);
+      }
+}
+
+// Pressure-type rows, NSBase.cxx:478-610 (kind FULL), :612-732 (PC_SIMPLEC),
+// :734-851 (PC_SIMPLE).  a_f is the momentum diagonal, or the row sum for
+// SIMPLEC.  FULL adds p0/p1 boundary terms and optional beta relaxation.
+void assemble_pressure(Ctx& c, int kind, bool relax, const std::vector<double>& uc,
+                       const std::vector<double>& vc, const std::vector<double>& wc,
+                       std::vector<double>& b) {
+  const double dx2 = c.dx * c.dx, dy2 = c.dy * c.dy, dz2 = c.dz * c.dz;
+  const double idx = 1 / c.dx, idy = 1 / c.dy, idz = 1 / c.dz;
+  const bool rs = (kind == PC_SIMPLEC);
+  auto af = [&](const Mat& A, int r) { return rs ? A.rowsum(r) : A.diag(r); };
+  c.pA.resize(c.np);
+  b.assign((size_t)c.np, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c.nx; ++i)
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        const int row = c.P(i, j, k);
+        if (row < 0) continue;
+        RowBuilder rb(row);
+        double bb = 0.0;
+        int f, q;
+        if (i > 0) {
+          q = c.P(i - 1, j, k); f = c.U(i, j, k);
+          if (q > -1 && f > -1) {
+            rb.diag += 1 / ((af(c.uA, f)) * dx2);
+            rb.insert(q, -1 / ((af(c.uA, f)) * dx2));
+            bb -= uc[f] * idx;
+          }
+        }
+        if (i < c.nx - 1) {
+          q = c.P(i + 1, j, k); f = c.U(i + 1, j, k);
+          if (q > -1 && f > -1) {
+            rb.diag += 1 / ((af(c.uA, f)) * dx2);
+            rb.insert(q, -1 / ((af(c.uA, f)) * dx2));
+            bb += uc[f] * idx;
+          }
+        }
+        if (j > 0) {
+          q = c.P(i, j - 1, k); f = c.V(i, j, k);
+          if (q > -1 && f > -1) {
+            rb.diag += 1 / ((af(c.vA, f)) * dy2);
+            rb.insert(q, -1 / ((af(c.vA, f)) * dy2));
+            bb -= vc[f] * idy;
+          }
+        }
+        if (j < c.ny - 1) {
+          q = c.P(i, j + 1, k); f = c.V(i, j + 1, k);
+          if (q > -1 && f > -1) {
+            rb.diag += 1 / ((af(c.vA, f)) * dy2);
+            rb.insert(q, -1 / ((af(c.vA, f)) * dy2));
+            bb += vc[f] * idy;
+          }
+        }
+        if (k > 0 && c.nz > 0) {
+          q = c.P(i, j, k - 1); f = c.W(i, j, k);
+          if (q > -1 && f > -1) {
+            rb.diag += 1 / ((af(c.wA, f)) * dz2);
+            rb.insert(q, -1 / ((af(c.wA, f)) * dz2));
+            bb -= wc[f] * idz;
+          }
+        }
+        if (k < c.nz - 1 && c.nz > 0) {
+          q = c.P(i, j, k + 1); f = c.W(i, j, k + 1);
+          if (q > -1 && f > -1) {
+            rb.diag += 1 / ((af(c.wA, f)) * dz2);
+            rb.insert(q, -1 / ((af(c.wA, f)) * dz2));
+            bb += wc[f] * idz;
+          }
+        }
+        if (i == 0 && (f = c.U(i, j, k)) > -1) {
+          rb.diag += 1 / ((af(c.uA, f)) * dx2);
+          if (kind == PRESSURE_FULL) bb -= uc[f] * idx - c.p0 / (c.uA.diag(f) * dx2);
+          else bb -= uc[f] * idx;
+        }
+        if (i == c.nx - 1 && (f = c.U(i + 1, j, k)) > -1) {
+          rb.diag += 1 / ((af(c.uA, f)) * dx2);
+          if (kind == PRESSURE_FULL) bb += uc[f] * idx + c.p1 / (c.uA.diag(f) * dx2);
+          else bb += uc[f] * idx;
+        }
+        if (kind == PRESSURE_FULL && relax) {
+          rb.diag /= c.beta;
+          bb += c.p[row] * rb.diag * (1 - c.beta);
+        }
+        b[row] = bb;
+        rb.finish(c.pA);
+      }
+  jacobi_compute(c.ps, c.pA);
+}
+
+// NSBase.cxx:1773-1884 (ComputeVelocityField)
+void compute_velocity_field(Ctx& c, bool limit, double dvmax) {
+  std::vector<double> temp, bb;
+  struct Comp { std::vector<double>* x; Solver* s; int sys; };
+  Comp comps[3] = {{&c.u, &c.us, SYS_U}, {&c.v, &c.vs, SYS_V}, {&c.w, &c.ws, SYS_W}};
+  for (int ci = 0; ci < 3; ++ci) {
+    if (ci == 2 && c.nw <= 0) break;
+    std::vector<double>& x = *comps[ci].x;
+    Solver& s = *comps[ci].s;
+    if (ci == 0) rhs_u(c, x, bb);
+    else if (ci == 1) rhs_v(c, x, bb);
+    else rhs_w(c, x, bb);
+    temp = x;
+    solve_with_guess(s, bb, x);
+    if (s.iterations == 0 || s.info != 0) x = temp;
+    if (limit) {
+      const int n = (int)x.size();
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < n; ++i) {
+        double delta = x[i] - temp[i];
+        if (std::fabs(delta) > dvmax) {
+          if (delta > 0) x[i] = temp[i] + dvmax;
+          else if (delta < 0) x[i] = temp[i] - dvmax;
+        }
+      }
+    }
+  }
+}
+
+// NSBase.cxx:2337-2365 (ComputePressureCorrection): pc from guess pc.
+void compute_pressure_correction(Ctx& c) {
+  c.ps.tol = c.pcsolvertol;
+  solve_with_guess(c.ps, c.b, c.pc);
+}
+
+// NSBase.cxx:2383-2423 (ComputePressureField)
+void compute_pressure_field(Ctx& c) {
+  double tol = c.psolvertol;
+  c.ps.tol = tol;
+  std::vector<double> ptemp = c.p;
+  solve_with_guess(c.ps, c.b, ptemp);
+  while (c.ps.iterations == 0 && c.ps.info == 0 && tol > 1e-15) {
+    tol *= 0.1;
+    c.ps.tol = tol;
+    solve_with_guess(c.ps, c.b, ptemp);
+  }
+  if (c.ps.iterations > 0) c.p = ptemp;
+}
+
+inline double clamp_dv(double d, double m) {
+  if (d < 0 && std::fabs(d) > m) d = -m;
+  else if (d > 0 && d > m) d = m;
+  return d;
+}
+
+// NSBase.cxx:2082-2172 (SIMPLE), :2174-2263 (SIMPLEC), :2265-2335 (SIMPLER).
+// SIMPLE/SIMPLEC add a clamped increment from pc to the target set; SIMPLER
+// overwrites the target set with hat - grad(p)/a.
+void correct_velocity(Ctx& c, int kind, double dvmax, int target, double p_0, double p_1) {
+  std::vector<double>& tu = target == SET_TEMP ? c.ut : (target == SET_HAT ? c.uh : c.u);
+  std::vector<double>& tv = target == SET_TEMP ? c.vt : (target == SET_HAT ? c.vh : c.v);
+  std::vector<double>& tw = target == SET_TEMP ? c.wt : (target == SET_HAT ? c.wh : c.w);
+  const bool rs = (kind == CORR_SIMPLEC);
+  auto af = [&](const Mat& A, int r) { return rs ? A.rowsum(r) : A.diag(r); };
+  const std::vector<double>& q = (kind == CORR_SIMPLER) ? c.p : c.pc;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c.nx; ++i)
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        const int C = c.P(i, j, k);
+        if (C < 0) continue;
+        int f, o;
+        if (i > 0 && (o = c.P(i - 1, j, k)) > -1 && (f = c.U(i, j, k)) > -1) {
+          if (kind == CORR_SIMPLER) tu[f] = c.uh[f] - (q[o] - q[C]) / (c.uA.diag(f) * c.dx);
+          else tu[f] += clamp_dv(-(q[o] - q[C]) / ((af(c.uA, f)) * c.dx), dvmax);
+        }
+        if (j > 0 && (o = c.P(i, j - 1, k)) > -1 && (f = c.V(i, j, k)) > -1) {
+          if (kind == CORR_SIMPLER) tv[f] = c.vh[f] - (q[o] - q[C]) / (c.vA.diag(f) * c.dy);
+          else tv[f] += clamp_dv(-(q[o] - q[C]) / ((af(c.vA, f)) * c.dy), dvmax);
+        }
+        if (k > 0 && (o = c.P(i, j, k - 1)) > -1 && (f = c.W(i, j, k)) > -1) {
+          if (kind == CORR_SIMPLER) tw[f] = c.wh[f] - (q[o] - q[C]) / (c.wA.diag(f) * c.dz);
+          else tw[f] += clamp_dv(-(q[o] - q[C]) / ((af(c.wA, f)) * c.dz), dvmax);
+        }
+        if (i == 0 && (f = c.U(i, j, k)) > -1) {
+          if (kind == CORR_SIMPLER) tu[f] = c.uh[f] - (p_0 - q[C]) / (c.uA.diag(f) * c.dx);
+          else tu[f] += clamp_dv((q[C]) / ((af(c.uA, f)) * c.dx), dvmax);
+        }
+        if (i == c.nx - 1 && (f = c.U(i + 1, j, k)) > -1) {
+          if (kind == CORR_SIMPLER) tu[f] = c.uh[f] - (q[C] - p_1) / (c.uA.diag(f) * c.dx);
+          else tu[f] += clamp_dv(-(q[C]) / ((af(c.uA, f)) * c.dx), dvmax);
+        }
+      }
+}
+
+// NSBase.cxx:2367-2380 (CorrectPressureField)
+void correct_pressure(Ctx& c) {
+  double mx = -INFINITY, mn = INFINITY, sq = 0.0;
+  for (double x : c.pc) { mx = std::max(mx, x); mn = std::min(mn, x); sq += x * x; }
+  c.pcmax = c.beta * mx;
+  c.pcmin = c.beta * mn;
+  c.pcnorm = c.beta * std::sqrt(sq);
+  for (size_t i = 0; i < c.p.size(); ++i) c.p[i] += c.beta * c.pc[i];
+}
+
+// NSBase.cxx:1958-2080 (ComputeDiv).  md starts at 0 (uninitialised in the
+// reference).  Cell sums are accumulated per i-slab then added in slab order.
+void compute_div(Ctx& c, const std::vector<double>& u, const std::vector<double>& v,
+                 const std::vector<double>& w, double out[4]) {
+  const double idx = 1 / c.dx, idy = 1 / c.dy, idz = 1 / c.dz;
+  std::vector<double> sdiv((size_t)c.nx, 0.0), svel((size_t)c.nx, 0.0), smd((size_t)c.nx, 0.0),
+      sdv((size_t)c.nx, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c.nx; ++i) {
+    double tdiv = 0, tvel = 0, md = 0, mdv = 0;
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        if (c.P(i, j, k) < 0) continue;
+        double div = 0, uu = 0, vv = 0, ww = 0, DV = 0;
+        int f;
+        if (i > 0 && c.P(i - 1, j, k) > -1 && (f = c.U(i, j, k)) > -1) { div += -u[f] * idx; DV += -u[f]; uu += u[f] * 0.5; }
+        if (i < c.nx - 1 && c.P(i + 1, j, k) > -1 && (f = c.U(i + 1, j, k)) > -1) { div += u[f] * idx; DV += u[f]; uu += u[f] * 0.5; }
+        if (j > 0 && c.P(i, j - 1, k) > -1 && (f = c.V(i, j, k)) > -1) { div += -v[f] * idy; DV += -v[f]; vv += v[f] * 0.5; }
+        if (j < c.ny - 1 && c.P(i, j + 1, k) > -1 && (f = c.V(i, j + 1, k)) > -1) { div += v[f] * idy; DV += v[f]; vv += v[f] * 0.5; }
+        if (k > 0 && c.nz > 0 && c.P(i, j, k - 1) > -1 && (f = c.W(i, j, k)) > -1) { div += -w[f] * idz; DV += -w[f]; ww += w[f] * 0.5; }
+        if (k < c.nz - 1 && c.nz > 0 && c.P(i, j, k + 1) > -1 && (f = c.W(i, j, k + 1)) > -1) { div += w[f] * idz; DV += w[f]; ww += w[f] * 0.5; }
+        if (i == 0 && (f = c.U(i, j, k)) > -1) { div += -u[f] * idx; DV += -u[f]; uu += u[f] * 0.5; }
+        if (i == c.nx - 1 && (f = c.U(i + 1, j, k)) > -1) { div += u[f] * idx; DV += u[f]; uu += u[f] * 0.5; }
+        DV = std::fabs(DV);
+        if (DV > mdv) mdv = DV;
+        div = std::fabs(div);
+        if (div > md) md = div;
+        tdiv += div * div;
+        tvel += std::sqrt(uu * uu + vv * vv + ww * ww);
+      }
+    sdiv[i] = tdiv; svel[i] = tvel; smd[i] = md; sdv[i] = mdv;
+  }
+  double td = 0, tv = 0, md = 0, mdv = 0;
+  for (int i = 0; i < c.nx; ++i) {
+    td += sdiv[i]; tv += svel[i];
+    md = std::max(md, smd[i]); mdv = std::max(mdv, sdv[i]);
+  }
+  out[0] = mdv;                 // MaxDeltaV
+  out[1] = std::sqrt(td);       // normdiv
+  out[2] = tv / c.np;           // velocityNorm
+  out[3] = md;                  // maxdiv
+}
+
+// NSBase.cxx:1886-1956 (ComputePermeability).  out: Q, v_moy, K, p_in, p_out.
+void compute_permeability(Ctx& c, double out[5]) {
+  const double sx = c.dy * c.dz;
+  double vm = 0;
+  int npin = 0, npout = 0;
+  if (c.border <= 0) {
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        int f = c.U(c.nx, j, k);
+        if (f > -1) vm += c.u[f];
+      }
+    c.p_inlet = c.p0;
+    c.p_outlet = c.p1;
+  } else {
+    c.p_inlet = 0;
+    c.p_outlet = 0;
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        int f = c.U(c.nx + 1 - c.border, j, k);
+        if (f > -1) vm += c.u[f];
+      }
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        int q = c.P(c.nx - c.border, j, k);
+        if (q > -1) { c.p_outlet += c.p[q]; npout += 1; }
+        q = c.P(c.border - 1, j, k);
+        if (q > -1) { c.p_inlet += c.p[q]; npin += 1; }
+      }
+    if (npin > 0) c.p_inlet /= npin;
+    if (npout > 0) c.p_outlet /= npout;
+  }
+  double Q = vm * sx;
+  vm /= (double)(c.ny * c.nz);
+  double K;
+  if (std::fabs(c.p_inlet - c.p_outlet) > 0.0)
+    K = vm * c.mu * c.dx * (double)(c.nx - 2 * c.border) / (c.p_inlet - c.p_outlet);
+  else
+    K = 0.0;
+  out[0] = Q; out[1] = vm; out[2] = K; out[3] = c.p_inlet; out[4] = c.p_outlet;
+}
+
+// NSBase.cxx:2671-2681 (Initpvector, linear profile)
+void init_pressure_linear(Ctx& c) {
+  c.p.assign((size_t)c.np, 0.0);
+  for (int i = 0; i < c.nx; ++i)
+    for (int j = 0; j < c.ny; ++j)
+      for (int k = 0; k < c.nz; ++k) {
+        int q = c.P(i, j, k);
+        if (q != -1) c.p[q] = (c.p1 - c.p0) * ((double)i + 1.0) / ((double)c.nx) + c.p0;
+      }
+}
+
+// NSBase.cxx:196-301 (RelaxControl).  The reference pops from a possibly
+// empty history; here an empty history is left alone.
+void relax_control(Ctx& c, bool beta_cst) {
+  const size_t size = 30;
+  bool isDecreasing = false;
+  if (c.RMR > c.old_RMR * c.maxconvdelta) {
+    c.alpha = std::max(c.min_alpha, c.alpha - c.relax);
+    if (!beta_cst) c.beta = std::max(c.min_beta, c.beta - c.relax);
+    c.old_RMR *= 1.1;
+    if (!c.avgRMR.empty()) c.avgRMR.pop_front();
+  }
+  if (c.avgRMR.size() > 0) isDecreasing = c.avgRMR.back() > c.RMR;
+  c.avgRMR.push_back(c.RMR);
+  if (c.avgRMR.size() > size) {
+    c.avgRMR.pop_front();
+    int i = 0, n1 = 0, n2 = 0;
+    double avg1 = 0, avg2 = 0;
+    for (double x : c.avgRMR) {
+      i += 1;
+      if (i < c.avgRMR.size() * 0.5) { avg1 += x; n1 += 1; }
+      else if (i >= c.avgRMR.size() * 0.5) { avg2 += x; n2 += 1; }
+    }
+    avg1 /= n1;
+    avg2 /= n2;
+    double delta = (avg2 - avg1) / std::max(avg1, avg2);
+    if (delta <= 0 && std::fabs(delta) < c.deltaRMR && !isDecreasing) {
+      for (int q = 0; q < 15 && !c.avgRMR.empty(); ++q) c.avgRMR.pop_front();
+      c.alpha = std::min(c.alpha0, c.alpha - c.relax * 0.25);
+      if (!beta_cst) c.beta = std::max(c.min_beta, c.beta - c.relax * 0.25);
+    } else if (delta > 0 && !isDecreasing) {
+      for (int q = 0; q < 15 && !c.avgRMR.empty(); ++q) c.avgRMR.pop_front();
+      c.alpha = std::max(c.min_alpha, c.alpha - c.relax);
+      if (!beta_cst) c.beta = std::max(c.min_beta, c.beta - c.relax);
+    }
+  }
+}
+
+// NSBase.cxx:304-328 (CheckConvergence); nz>0 always holds for a 3-D image.
+bool check_convergence(Ctx& c, double res, double tol) {
+  if (res < tol) return true;
+  if (c.us.iterations == 0 && c.vs.iterations == 0 && c.ws.iterations == 0 && c.ps.iterations == 0)
+    return true;
+  return false;
+}
+
+void zero_fields(Ctx& c) {
+  c.u.assign((size_t)c.nu, 0.0); c.v.assign((size_t)c.nv, 0.0); c.w.assign((size_t)c.nw, 0.0);
+  c.ut = c.u; c.vt = c.v; c.wt = c.w;
+  c.uh = c.u; c.vh = c.v; c.wh = c.w;
+  c.pc.assign((size_t)c.np, 0.0);
+  c.b.assign((size_t)c.np, 0.0);
+}
+
+void push_row(Ctx& c, int it, const double perm[5], double RMR, const double div[4], double reldiv) {
+  ResRow r;
+  r.it = it; r.Q = perm[0]; r.v_moy = perm[1]; r.RMR = RMR; r.K = perm[2];
+  r.gradP = (c.p_inlet - c.p_outlet) / (c.dx * (c.nx - 2 * c.border));
+  r.normdiv = div[1]; r.reldiv = reldiv; r.velocityNorm = div[2]; r.maxdiv = div[3];
+  r.it_u = c.us.iterations; r.it_v = c.vs.iterations; r.it_w = c.ws.iterations; r.it_pc = c.ps.iterations;
+  c.rows.push_back(r);
+}
+
+// The four outer-loop drivers: SIMPLEC.cxx:73-232, SIMPLER.cxx:75-270,
+// SIMPLERR.cxx:75-280, IDEAL.cxx:75-281.  Field output (FieldsToImage) is not
+// part of the restatement.
+void run_driver(Ctx& c, int method, double simpletol, int maxit, int N1, int N2) {
+  double perm[5] = {0, 0, 0, 0, 0}, div[4] = {0, 0, 0, 0};
+  double reldiv = 0;
+  c.rows.clear();
+  c.timers.assign(7, 0.0);
+  zero_fields(c);
+  init_pressure_linear(c);
+  c.ps.maxit = c.maxitsolver; c.ps.tol = c.psolvertol;
+  c.us.tol = c.vs.tol = c.ws.tol = c.vsolvertol;
+  c.us.maxit = c.vs.maxit = c.ws.maxit = c.maxitsolver;
+  c.us.iterations = c.vs.iterations = c.ws.iterations = c.ps.iterations = -1;
+  c.alpha = c.alpha0;
+  c.beta = c.beta0;
+  c.avgRMR.clear();
+  double t0;
+  auto tick = [&](int slot) { double t = now_s(); c.timers[slot] += t - t0; t0 = t; };
+  t0 = now_s();
+  assemble_all_momentum(c); tick(0);
+  compute_velocity_field(c, false, 0); tick(1);
+  compute_permeability(c, perm);
+  c.alpha = c.alpha0;
+  const double Lref = c.nx * c.dx + c.ny * c.dy + c.nz * c.dz;
+  int it;
+  for (it = 0; it < maxit; ++it) {
+    if (method == METHOD_SIMPLEC) {
+      if (it > 0) {
+        assemble_all_momentum(c); tick(0);
+        compute_velocity_field(c, true, perm[1] * std::min(1., c.alpha * 1.5)); tick(1);
+      }
+    } else {
+      assemble_all_momentum(c); tick(0);
+      c.ut = c.u; c.vt = c.v; c.wt = c.w;
+      for (int s1 = 0; s1 < N1; ++s1) {
+        compute_hat(c, SYS_U, c.ut, c.u, c.uh);
+        compute_hat(c, SYS_V, c.vt, c.v, c.vh);
+        compute_hat(c, SYS_W, c.wt, c.w, c.wh); tick(2);
+        assemble_pressure(c, PRESSURE_FULL, true, c.uh, c.vh, c.wh, c.b); tick(3);
+        compute_pressure_field(c); tick(4);
+        correct_velocity(c, CORR_SIMPLER, 0, SET_TEMP, c.p0, c.p1); tick(5);
+      }
+      compute_velocity_field(c, true, perm[1] * std::min(1., c.alpha * 1.5)); tick(1);
+    }
+    compute_permeability(c, perm);
+    compute_div(c, c.u, c.v, c.w, div); tick(6);
+    c.RMR = div[0] / std::fabs(perm[1]);
+    reldiv = Lref * div[3] / (3 * perm[1]);
+    if (it > 1) {
+      if (c.RMR < c.old_RMR) c.old_RMR = c.RMR;
+    } else {
+      c.old_RMR = c.RMR;
+    }
+    if (method == METHOD_SIMPLEC || method == METHOD_SIMPLER) push_row(c, it, perm, c.RMR, div, reldiv);
+    if (it > 1 && check_convergence(c, c.RMR, simpletol)) break;
+    if (method == METHOD_SIMPLEC) {
+      assemble_pressure(c, PC_SIMPLEC, false, c.u, c.v, c.w, c.b); tick(3);
+      compute_pressure_correction(c); tick(4);
+      correct_velocity(c, CORR_SIMPLEC, perm[1] * c.alpha, SET_VEL, 0, 0);
+      correct_pressure(c); tick(5);
+      compute_div(c, c.u, c.v, c.w, div); tick(6);
+      relax_control(c, true);
+    } else if (method == METHOD_SIMPLER || method == METHOD_SIMPLERR) {
+      assemble_pressure(c, PC_SIMPLE, false, c.u, c.v, c.w, c.b); tick(3);
+      compute_pressure_correction(c); tick(4);
+      correct_velocity(c, CORR_SIMPLE, perm[1] * 100, SET_VEL, 0, 0); tick(5);
+      compute_div(c, c.u, c.v, c.w, div); tick(6);
+      if (method == METHOD_SIMPLERR) {
+        compute_hat(c, SYS_U, c.u, c.u, c.uh);
+        compute_hat(c, SYS_V, c.v, c.v, c.vh);
+        compute_hat(c, SYS_W, c.w, c.w, c.wh); tick(2);
+        assemble_pressure(c, PRESSURE_FULL, true, c.uh, c.vh, c.wh, c.b); tick(3);
+        compute_pressure_field(c); tick(4);
+        push_row(c, it, perm, c.RMR, div, reldiv);
+        relax_control(c, false);
+      } else {
+        relax_control(c, true);
+      }
+    } else {  // IDEAL
+      c.ut = c.u; c.vt = c.v; c.wt = c.w;
+      for (int s2 = 0; s2 < N2; ++s2) {
+        compute_hat(c, SYS_U, c.u, c.ut, c.uh);
+        compute_hat(c, SYS_V, c.v, c.vt, c.vh);
+        compute_hat(c, SYS_W, c.w, c.wt, c.wh); tick(2);
+        assemble_pressure(c, PRESSURE_FULL, false, c.uh, c.vh, c.wh, c.b); tick(3);
+        compute_pressure_field(c); tick(4);
+        correct_velocity(c, CORR_SIMPLER, 0, SET_VEL, c.p0, c.p1); tick(5);
+      }
+      compute_div(c, c.u, c.v, c.w, div); tick(6);
+      relax_control(c, true);
+    }
+  }
+  c.final_it = it;
+  c.final_K = perm[2];
+}
+
+Ctx* make_ctx(const uint8_t* img, int nx, int ny, int nz) {
+  Ctx* c = new Ctx();
+  c->nx = nx; c->ny = ny; c->nz = nz;
+  init_index(*c, img);
+  zero_fields(*c);
+  c->p.assign((size_t)c->np, 0.0);
+  return c;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI for the test harness (ctypes) and the CPU-baseline timer.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int orc_prepare_image(const uint8_t* in, int nx, int ny, int nz, int direction, int border,
+                      uint8_t* out, int* dims) {
+  std::vector<uint8_t> rot, pad;
+  int ox, oy, oz;
+  rotate(in, nx, ny, nz, direction, rot, ox, oy, oz);
+  if (border > 0) {
+    add_border(rot, ox, oy, oz, border, pad);
+    ox += 2 * border;
+  } else {
+    pad.swap(rot);
+  }
+  dims[0] = ox; dims[1] = oy; dims[2] = oz;
+  if (out) std::memcpy(out, pad.data(), pad.size());
+  return 0;
+}
+
+void* orc_create(const uint8_t* img, int nx, int ny, int nz) { return make_ctx(img, nx, ny, nz); }
+void orc_destroy(void* h) { delete (Ctx*)h; }
+
+void orc_counts(void* h, int64_t* out) {
+  Ctx& c = *(Ctx*)h;
+  out[0] = c.np; out[1] = c.nu; out[2] = c.nv; out[3] = c.nw;
+}
+
+void orc_get_index(void* h, int32_t* p, int32_t* u, int32_t* v, int32_t* w) {
+  Ctx& c = *(Ctx*)h;
+  if (p) std::memcpy(p, c.pidx.data(), c.pidx.size() * 4);
+  if (u) std::memcpy(u, c.uidx.data(), c.uidx.size() * 4);
+  if (v) std::memcpy(v, c.vidx.data(), c.vidx.size() * 4);
+  if (w) std::memcpy(w, c.widx.data(), c.widx.size() * 4);
+}
+
+// params: dx dy dz mu rho p0 gradp alpha beta vsolvertol psolvertol pcsolvertol deltaRMR
+// iparams: maxitsolver border save_step
+void orc_set_params(void* h, const double* prm, const int* iprm) {
+  Ctx& c = *(Ctx*)h;
+  c.dx = prm[0]; c.dy = prm[1]; c.dz = prm[2]; c.mu = prm[3]; c.rho = prm[4];
+  c.p0 = prm[5]; c.gradp = prm[6]; c.alpha = c.alpha0 = prm[7]; c.beta = c.beta0 = prm[8];
+  c.vsolvertol = prm[9]; c.psolvertol = prm[10]; c.pcsolvertol = prm[11]; c.deltaRMR = prm[12];
+  c.maxitsolver = iprm[0]; c.border = iprm[1]; c.save_step = iprm[2];
+  // NSBase.cxx:155-161
+  c.p1 = -c.gradp * (c.dx * c.nx) + c.p0;
+  if (c.p1 < 0) { c.p0 -= c.p1; c.p1 = 0; }
+  c.us.maxit = c.vs.maxit = c.ws.maxit = c.ps.maxit = c.maxitsolver;
+  c.us.tol = c.vs.tol = c.ws.tol = c.vsolvertol;
+  c.ps.tol = c.psolvertol;
+}
+
+void orc_get_p0p1(void* h, double* out) { Ctx& c = *(Ctx*)h; out[0] = c.p0; out[1] = c.p1; }
+void orc_set_alpha_beta(void* h, double a, double b) { Ctx& c = *(Ctx*)h; c.alpha = a; c.beta = b; }
+
+int64_t orc_vec_len(void* h, int which) { return (int64_t)((Ctx*)h)->vec(which).size(); }
+void orc_set_vec(void* h, int which, const double* x) {
+  std::vector<double>& v = ((Ctx*)h)->vec(which);
+  std::memcpy(v.data(), x, v.size() * 8);
+}
+void orc_get_vec(void* h, int which, double* x) {
+  std::vector<double>& v = ((Ctx*)h)->vec(which);
+  std::memcpy(x, v.data(), v.size() * 8);
+}
+
+void orc_init_pressure_linear(void* h) { init_pressure_linear(*(Ctx*)h); }
+void orc_assemble_momentum(void* h) { assemble_all_momentum(*(Ctx*)h); }
+
+// Exports a matrix in ELL form sorted by column: cnt[n], col[7n], val[7n].
+int64_t orc_matrix_rows(void* h, int sys) {
+  Ctx& c = *(Ctx*)h;
+  const Mat& A = sys == SYS_U ? c.uA : sys == SYS_V ? c.vA : sys == SYS_W ? c.wA : c.pA;
+  return A.n;
+}
+void orc_get_matrix(void* h, int sys, int32_t* cnt, int32_t* col, double* val) {
+  Ctx& c = *(Ctx*)h;
+  const Mat& A = sys == SYS_U ? c.uA : sys == SYS_V ? c.vA : sys == SYS_W ? c.wA : c.pA;
+  std::memcpy(cnt, A.cnt.data(), (size_t)A.n * 4);
+  std::memcpy(col, A.col.data(), (size_t)A.n * 28);
+  std::memcpy(val, A.val.data(), (size_t)A.n * 56);
+}
+
+// info: iterations, info, error for u, v, w (w unset when nw==0)
+void orc_compute_velocity_field(void* h, int limit, double dvmax, double* info) {
+  Ctx& c = *(Ctx*)h;
+  compute_velocity_field(c, limit != 0, dvmax);
+  Solver* s[3] = {&c.us, &c.vs, &c.ws};
+  for (int q = 0; q < 3; ++q) { info[3 * q] = (double)s[q]->iterations; info[3 * q + 1] = s[q]->info; info[3 * q + 2] = s[q]->error; }
+}
+
+void orc_compute_hat(void* h, int uc_set, int uold_set) {
+  Ctx& c = *(Ctx*)h;
+  auto pick = [&](int set, int comp) -> std::vector<double>& {
+    int base = set == SET_VEL ? VEC_U : set == SET_TEMP ? VEC_UT : VEC_UH;
+    return c.vec(base + comp);
+  };
+  std::vector<double> uc, uo;
+  uc = pick(uc_set, 0); uo = pick(uold_set, 0); compute_hat(c, SYS_U, uc, uo, c.uh);
+  uc = pick(uc_set, 1); uo = pick(uold_set, 1); compute_hat(c, SYS_V, uc, uo, c.vh);
+  uc = pick(uc_set, 2); uo = pick(uold_set, 2); compute_hat(c, SYS_W, uc, uo, c.wh);
+}
+
+void orc_assemble_pressure(void* h, int kind, int relax, int vel_set) {
+  Ctx& c = *(Ctx*)h;
+  int base = vel_set == SET_VEL ? VEC_U : vel_set == SET_TEMP ? VEC_UT : VEC_UH;
+  std::vector<double> uc = c.vec(base), vc = c.vec(base + 1), wc = c.vec(base + 2);
+  assemble_pressure(c, kind, relax != 0, uc, vc, wc, c.b);
+}
+
+void orc_compute_pressure_correction(void* h, double* info) {
+  Ctx& c = *(Ctx*)h;
+  compute_pressure_correction(c);
+  info[0] = (double)c.ps.iterations; info[1] = c.ps.info; info[2] = c.ps.error;
+}
+void orc_compute_pressure_field(void* h, double* info) {
+  Ctx& c = *(Ctx*)h;
+  compute_pressure_field(c);
+  info[0] = (double)c.ps.iterations; info[1] = c.ps.info; info[2] = c.ps.error;
+}
+void orc_correct_velocity(void* h, int kind, double dvmax, int target, double p_0, double p_1) {
+  correct_velocity(*(Ctx*)h, kind, dvmax, target, p_0, p_1);
+}
+void orc_correct_pressure(void* h, double* out) {
+  Ctx& c = *(Ctx*)h;
+  correct_pressure(c);
+  out[0] = c.pcmax; out[1] = c.pcmin; out[2] = c.pcnorm;
+}
+void orc_compute_div(void* h, int set, double* out) {
+  Ctx& c = *(Ctx*)h;
+  int base = set == SET_VEL ? VEC_U : set == SET_TEMP ? VEC_UT : VEC_UH;
+  compute_div(c, c.vec(base), c.vec(base + 1), c.vec(base + 2), out);
+}
+void orc_compute_permeability(void* h, double* out) { compute_permeability(*(Ctx*)h, out); }
+
+// Standalone Eigen-semantics BiCGSTAB on an ELL matrix (cnt/col/val with 7
+// slots per row, columns sorted), for unit tests.  info: iterations, info, error.
+void orc_bicgstab_ell(int n, const int32_t* cnt, const int32_t* col, const double* val,
+                      const double* b, double* x, double tol, int maxit, double* info) {
+  Mat A;
+  A.n = n;
+  A.cnt.assign(cnt, cnt + n);
+  A.col.assign(col, col + (size_t)7 * n);
+  A.val.assign(val, val + (size_t)7 * n);
+  Solver s;
+  jacobi_compute(s, A);
+  s.tol = tol;
+  s.maxit = maxit;
+  std::vector<double> bb(b, b + n), xx(x, x + n);
+  solve_with_guess(s, bb, xx);
+  std::memcpy(x, xx.data(), (size_t)n * 8);
+  info[0] = (double)s.iterations; info[1] = s.info; info[2] = s.error;
+}
+
+void orc_spmv_ell(int n, const int32_t* cnt, const int32_t* col, const double* val,
+                  const double* x, double* y) {
+  Mat A;
+  A.n = n;
+  A.cnt.assign(cnt, cnt + n);
+  A.col.assign(col, col + (size_t)7 * n);
+  A.val.assign(val, val + (size_t)7 * n);
+  spmv(A, x, y);
+}
+
+// Full driver.  Returns the number of RES rows; rows_out (may be null) gets
+// 14 doubles per row: it Q v_moy RMR K gradP normdiv reldiv velocityNorm
+// maxdiv it_u it_v it_w it_pc.  summary: final_it, final_K, seconds.
+int orc_solve(void* h, int method, double simpletol, int maxit, int N1, int N2, double* rows_out,
+              int max_rows, double* summary) {
+  Ctx& c = *(Ctx*)h;
+  double t0 = now_s();
+  run_driver(c, method, simpletol, maxit, N1, N2);
+  double t1 = now_s();
+  int nr = (int)c.rows.size();
+  if (rows_out) {
+    for (int r = 0; r < nr && r < max_rows; ++r) {
+      const ResRow& q = c.rows[r];
+      double* o = rows_out + 14 * r;
+      o[0] = q.it; o[1] = q.Q; o[2] = q.v_moy; o[3] = q.RMR; o[4] = q.K; o[5] = q.gradP;
+      o[6] = q.normdiv; o[7] = q.reldiv; o[8] = q.velocityNorm; o[9] = q.maxdiv;
+      o[10] = (double)q.it_u; o[11] = (double)q.it_v; o[12] = (double)q.it_w; o[13] = (double)q.it_pc;
+    }
+  }
+  if (summary) {
+    summary[0] = c.final_it; summary[1] = c.final_K; summary[2] = t1 - t0;
+    for (int q = 0; q < 7; ++q) summary[3 + q] = c.timers[q];
+  }
+  return nr;
+}
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"
