@@ -192,13 +192,17 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
                                   const double* b, double* x, double tol, int64_t maxit,
                                   ns3d_solve_info* info);
 
-/* Device time of the fused BiCGSTAB kernels since the last reset, measured
- * with HIP events on the context's stream: out[0] = ms in SpMV-carrying
- * kernels, out[1] = their launch count, out[2] = ms in all solver kernels,
- * out[3] = solver kernel launches, out[4] = algorithmic bytes moved by the
- * SpMV-carrying kernels.  Timing is only recorded after ns3d_timing(ctx, 1). */
+/* Device time of the fused BiCGSTAB kernels, measured with HIP events on
+ * the context's stream around every launch (enable with ns3d_timing(ctx, 1);
+ * launches after a solve has stopped are not counted).  out[18]: for system
+ * kind m (0 momentum, 1 pressure) and kernel k (0: A = p update + SpMV + r0.v,
+ * 1: B = s + SpMV + t.s/t.t, 2: C = x/r update + r0.r/r.r), at
+ * out[9m + 3k]: milliseconds, launches, algorithmic bytes. */
 ns3d_status ns3d_timing(ns3d_ctx* ctx, int enable);
-ns3d_status ns3d_timing_read(ns3d_ctx* ctx, double out[5]);
+ns3d_status ns3d_timing_read(ns3d_ctx* ctx, double out[18]);
+
+/* Stored entries (diagonal + existing neighbours) of the u, v, w, p systems. */
+ns3d_status ns3d_matrix_nnz(ns3d_ctx* ctx, int64_t out[4]);
 
 /* Synchronises the context's stream. */
 ns3d_status ns3d_sync(ns3d_ctx* ctx);

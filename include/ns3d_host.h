@@ -1,0 +1,81 @@
+/*
+ * ns3d_host.h -- C ABI of libns3d_host.so: a whole permeability solve (one of
+ * the four outer-loop drivers over one image) for callers that do not want
+ * to drive the per-method ABI of ns3d.h themselves.
+ *
+ * It replaces the reference's entry point: NavierStokesBase construction
+ * from an image + parameters (src/NSBase.cxx:50-194) followed by
+ * XSOLVER(...).Solve(base) (src/navierstokes3D.cxx:516-542).  The drivers are
+ * host C++ (ns3d_amd/host/) calling libns3d_gpu.so.
+ */
+#ifndef NS3D_HOST_H
+#define NS3D_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { NS3D_METHOD_SIMPLEC = 0, NS3D_METHOD_SIMPLER = 1, NS3D_METHOD_SIMPLERR = 2, NS3D_METHOD_IDEAL = 3 };
+
+/* The keys of the reference's .ini file (navierstokes3D.cxx:162-201), with
+ * the same meaning and defaults (see ns3d_default_run_params). */
+typedef struct {
+  double dx, dy, dz;            /* dy, dz <= 0: same as dx */
+  double mu, rho, p0, gradp;
+  double alpha, beta;
+  double vsolvertol, psolvertol, pcsolvertol, simpletol, deltaRMR;
+  int32_t maxitsolver, maxitSIMPLE, N1, N2, save_step;
+  int32_t porevalue, direction, border;
+  int32_t method;               /* NS3D_METHOD_* */
+  int32_t verbose;              /* 1: print progress like the reference */
+  int32_t write_files;          /* 1: write INFO/RES logs and .vti fields */
+  int32_t device;               /* GPU index, -1 = current */
+  const char* outdir;           /* directory for the output files */
+  const char* name;             /* image file name (output names derive from it) */
+} ns3d_run_params;
+
+typedef struct {
+  int32_t iterations;           /* outer iterations done (the drivers' it) */
+  int32_t converged;            /* CheckConvergence returned true */
+  double K, Q, v_moy, RMR;      /* last permeability / residual */
+  double normdiv, reldiv, maxdiv;
+  double seconds;               /* wall time of Solve */
+  int64_t np, nu, nv, nw;
+  int64_t solver_iterations[4]; /* summed BiCGSTAB iterations: u, v, w, p */
+  double phase_seconds[8];      /* momentum asm, velocity solves, hats, p asm,
+                                   p solves, corrections, diagnostics, other */
+} ns3d_run_result;
+
+typedef struct ns3d_solver ns3d_solver;
+
+void ns3d_default_run_params(ns3d_run_params* p);
+
+/* img is the raw image (voxel == porevalue is fluid), C order (nx, ny, nz)
+ * with x slowest; rotation and border are applied from the parameters. */
+int ns3d_solver_create(const ns3d_run_params* p, const uint8_t* img, int nx, int ny, int nz,
+                       ns3d_solver** out);
+/* Grid after rotation/border: dims[3]; counts[4] = np, nu, nv, nw. */
+int ns3d_solver_info(ns3d_solver* s, int dims[3], int64_t counts[4]);
+/* Runs the driver.  rows (may be NULL) receives up to max_rows RES rows of 14
+ * doubles: it Q v_moy RMR K gradP normdiv reldiv velocityNorm maxdiv it_u
+ * it_v it_w it_pc (it_pc = 0 before the first pressure solve). */
+int ns3d_solver_run(ns3d_solver* s, double* rows, int max_rows, int* nrows, ns3d_run_result* res);
+/* Final fields on the node sets (lengths from ns3d_solver_info). */
+int ns3d_solver_fields(ns3d_solver* s, double* p, double* u, double* v, double* w);
+void ns3d_solver_destroy(ns3d_solver* s);
+/* The GPU context behind a solver (for ns3d_timing / ns3d_matrix_nnz). */
+struct ns3d_ctx* ns3d_solver_context(ns3d_solver* s);
+const char* ns3d_host_last_error(void);
+
+/* Reads a .ini file with the reference's grammar (ReadInput,
+ * navierstokes3D.cxx:266-463); the image path is copied to image (size cap).
+ * Returns 0 on success, 1 on a fatal input error (message on stderr). */
+int ns3d_read_ini(const char* path, ns3d_run_params* p, char* image, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NS3D_HOST_H */

@@ -44,12 +44,17 @@ struct SysMem {
   double* invdiag = nullptr;
   double* rowsum = nullptr;
   int64_t last_iters = 16;   // chunk-size hint for the next solve
+  int64_t nnz = 0;           // stored entries (diagonal + existing neighbours)
   SysView view() const { return SysView{n, d0, grid, cell, nbr, coef, invdiag, rowsum}; }
 };
 
+// Device time of the fused BiCGSTAB kernels, per system kind (0 momentum,
+// 1 pressure) and kernel (A, B, C): milliseconds, launches that did work,
+// algorithmic bytes.
+constexpr int kTimedChunk = 64;
 struct Timing {
   bool on = false;
-  double spmv_ms = 0, spmv_launches = 0, all_ms = 0, all_launches = 0, spmv_bytes = 0;
+  double ms[2][3] = {{0}}, launches[2][3] = {{0}}, bytes[2][3] = {{0}};
 };
 
 }  // namespace
@@ -81,6 +86,7 @@ struct ns3d_ctx {
   std::vector<void*> allocs;
   Timing tm;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<hipEvent_t> tev;  // 4 per timed iteration
 
   Geo geo() const { return Geo{nx, ny, nz, pg, ug, vg, wg}; }
 
@@ -157,35 +163,20 @@ Vel3 vel3(ns3d_ctx* c, int set) {
   return Vel3{c->vel[set][0], c->vel[set][1], c->vel[set][2]};
 }
 
-// Records an event pair around one launch when timing is enabled.
-struct KTimer {
-  ns3d_ctx* c;
-  bool on;
-  explicit KTimer(ns3d_ctx* cc) : c(cc), on(cc->tm.on) {
-    if (on) HCHK(hipEventRecord(c->ev[0], c->stream));
-  }
-  float stop() {
-    if (!on) return 0.f;
-    HCHK(hipEventRecord(c->ev[1], c->stream));
-    HCHK(hipEventSynchronize(c->ev[1]));
-    float ms = 0.f;
-    HCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    return ms;
-  }
-};
-
-// Algorithmic bytes of one SpMV-carrying BiCGSTAB kernel (A or B): 7 stored
-// coefficients + 6 neighbour indices per row are counted as nnz*(8+4) with
-// nnz the rows' stored entries; A streams r, p, v, M^-1, r0 and writes p, v;
-// B streams r, v, M^-1 and writes t.
-double kernel_bytes(int64_t n, int64_t nnz, bool is_a) {
-  const double mat = (double)nnz * 12.0;
-  return mat + (is_a ? 7.0 : 4.0) * 8.0 * (double)n;
+// Algorithmic bytes of the fused BiCGSTAB kernels per launch: the matrix is
+// nnz*(8+4) (coefficient + 32-bit column per stored entry), plus 8 bytes per
+// row per vector stream -- A: r, p, v, M^-1, r0 read, p, v written (7);
+// B: r, v, M^-1 read, t written (4); C: x, p, r, v, t, M^-1, r0 read, x, r
+// written (9).
+double kernel_bytes(int64_t n, int64_t nnz, int k) {
+  static const double streams[3] = {7.0, 4.0, 9.0};
+  const double mat = (k < 2) ? (double)nnz * 12.0 : 0.0;
+  return mat + streams[k] * 8.0 * (double)n;
 }
 
 // One BiCGSTAB solve on system sid with x as guess (Eigen semantics).
 void bicgstab(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int64_t maxit,
-              ns3d_solve_info* info, int64_t nnz_hint) {
+              ns3d_solve_info* info) {
   SysMem& S = c->sys[sid];
   const SysView V = S.view();
   hipStream_t s = c->stream;
@@ -207,29 +198,31 @@ void bicgstab(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int6
     }
     if (chunk > maxit + 1) chunk = maxit + 1;
     if (c->tm.on) {
-      // time A and B launches individually (they carry the SpMV)
+      // event pairs around every kernel; read back after the chunk's sync,
+      // counting only the iterations that did work
+      if (chunk > kTimedChunk) chunk = kTimedChunk;
       for (int64_t q = 0; q < chunk; ++q) {
-        KTimer ta(c);
+        HCHK(hipEventRecord(c->tev[4 * q + 0], s));
         HCHK(launch_bicg_A(V, w, s));
-        float ma = ta.stop();
-        KTimer tb(c);
+        HCHK(hipEventRecord(c->tev[4 * q + 1], s));
         HCHK(launch_bicg_B(V, w, s));
-        float mb = tb.stop();
-        KTimer tc(c);
+        HCHK(hipEventRecord(c->tev[4 * q + 2], s));
         HCHK(launch_bicg_C(V, x, w, s));
-        float mc = tc.stop();
-        HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
-        HCHK(hipStreamSynchronize(s));
-        if (c->hstate->nexec > exec_before) {
-          exec_before = c->hstate->nexec;
-          c->tm.spmv_ms += ma + mb;
-          c->tm.spmv_launches += 2;
-          c->tm.all_ms += ma + mb + mc;
-          c->tm.all_launches += 3;
-          c->tm.spmv_bytes += kernel_bytes(S.n, nnz_hint, true) + kernel_bytes(S.n, nnz_hint, false);
-        }
-        if (c->hstate->status != BS_RUNNING) break;
+        HCHK(hipEventRecord(c->tev[4 * q + 3], s));
       }
+      HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
+      HCHK(hipStreamSynchronize(s));
+      const long long did = c->hstate->nexec - exec_before;
+      exec_before = c->hstate->nexec;
+      const int kind = (sid == 3) ? 1 : 0;
+      for (long long q = 0; q < did && q < chunk; ++q)
+        for (int k = 0; k < 3; ++k) {
+          float ms = 0.f;
+          HCHK(hipEventElapsedTime(&ms, c->tev[4 * q + k], c->tev[4 * q + k + 1]));
+          c->tm.ms[kind][k] += ms;
+          c->tm.launches[kind][k] += 1;
+          c->tm.bytes[kind][k] += kernel_bytes(S.n, S.nnz, k);
+        }
     } else {
       for (int64_t q = 0; q < chunk; ++q) {
         HCHK(launch_bicg_A(V, w, s));
@@ -257,11 +250,6 @@ void bicgstab(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int6
     info->restarts = (int32_t)st.restarts;
     S.last_iters = st.nexec > 0 ? st.nexec : 4;
   }
-}
-
-int64_t approx_nnz(ns3d_ctx* c, int sid) {
-  // 6.6 stored entries per row on sphere packs; used only for byte accounting.
-  return (int64_t)((double)c->sys[sid].n * 6.6);
 }
 
 void alloc_system(ns3d_ctx* c, int sid, int n, int d0, int32_t* grid, bool momentum) {
@@ -359,7 +347,14 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
     HCHK(launch_scan_write(c->img, nx, ny, nz, sb, c->pg, c->ug, c->vg, c->wg, c->sys[3].cell,
                            c->sys[0].cell, c->sys[1].cell, c->sys[2].cell, s));
     const Geo g = c->geo();
-    for (int q = 0; q < 4; ++q) HCHK(launch_build_nbr(g, q, c->sys[q].cell, c->sys[q].n, c->sys[q].nbr, s));
+    unsigned long long* dnnz = c->alloc<unsigned long long>(4);
+    HCHK(hipMemsetAsync(dnnz, 0, 4 * sizeof(unsigned long long), s));
+    for (int q = 0; q < 4; ++q)
+      HCHK(launch_build_nbr(g, q, c->sys[q].cell, c->sys[q].n, c->sys[q].nbr, dnnz + q, s));
+    unsigned long long hnnz[4];
+    HCHK(hipMemcpyAsync(hnnz, dnnz, sizeof(hnnz), hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    for (int q = 0; q < 4; ++q) c->sys[q].nnz = (int64_t)hnnz[q];
     for (int set = 0; set < 3; ++set)
       for (int comp = 0; comp < 3; ++comp) c->vel[set][comp] = c->alloc<double>(c->sys[comp].n);
     c->p = c->alloc<double>(np);
@@ -409,6 +404,8 @@ void ns3d_destroy(ns3d_ctx* c) {
   if (c->hstate) (void)hipHostFree(c->hstate);
   if (c->hresult) (void)hipHostFree(c->hresult);
   for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->tev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -520,7 +517,7 @@ ns3d_status ns3d_compute_velocity_field(ns3d_ctx* c, double alpha, int limit, do
       double* x = c->vel[NS3D_SET_VEL][q];
       HCHK(launch_rhs(q, g, S.view(), c->p, x, c->vb, ph, s));
       HCHK(hipMemcpyAsync(c->temp, x, (size_t)S.n * 8, hipMemcpyDeviceToDevice, s));
-      bicgstab(c, q, x, c->vb, tol, maxit, &info[q], approx_nnz(c, q));
+      bicgstab(c, q, x, c->vb, tol, maxit, &info[q]);
       if (info[q].iterations == 0 || info[q].info != 0)
         HCHK(hipMemcpyAsync(x, c->temp, (size_t)S.n * 8, hipMemcpyDeviceToDevice, s));
       if (limit) HCHK(launch_clamp(x, c->temp, S.n, dvmax, s));
@@ -554,7 +551,7 @@ ns3d_status ns3d_assemble_pressure(ns3d_ctx* c, int kind, int relax, double beta
 ns3d_status ns3d_compute_pressure_correction(ns3d_ctx* c, double tol, int64_t maxit, ns3d_solve_info* info) {
   if (!c || !info) return fail_invalid("ns3d_compute_pressure_correction: bad arguments");
   return guarded([&] {
-    bicgstab(c, 3, c->pc, c->b, tol, maxit, info, approx_nnz(c, 3));
+    bicgstab(c, 3, c->pc, c->b, tol, maxit, info);
     HCHK(hipStreamSynchronize(c->stream));
   });
 }
@@ -566,10 +563,10 @@ ns3d_status ns3d_compute_pressure_field(ns3d_ctx* c, double tol, int64_t maxit, 
     const int64_t n = c->sys[3].n;
     double* ptemp = c->temp;  // temp holds max(n) doubles
     HCHK(hipMemcpyAsync(ptemp, c->p, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-    bicgstab(c, 3, ptemp, c->b, tol, maxit, info, approx_nnz(c, 3));
+    bicgstab(c, 3, ptemp, c->b, tol, maxit, info);
     while (info->iterations == 0 && info->info == 0 && tol > 1e-15) {
       tol *= 0.1;
-      bicgstab(c, 3, ptemp, c->b, tol, maxit, info, approx_nnz(c, 3));
+      bicgstab(c, 3, ptemp, c->b, tol, maxit, info);
     }
     if (info->iterations > 0) HCHK(hipMemcpyAsync(c->p, ptemp, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
     HCHK(hipStreamSynchronize(s));
@@ -698,7 +695,7 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
     HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), s));
     HCHK(hipHostMalloc((void**)&c->hstate, sizeof(BState), hipHostMallocDefault));
     HCHK(hipHostMalloc((void**)&c->hresult, 16 * sizeof(double), hipHostMallocDefault));
-    bicgstab(c, 3, dx, db, tol, maxit, info, (int64_t)7 * n);
+    bicgstab(c, 3, dx, db, tol, maxit, info);
     HCHK(hipMemcpyAsync(x, dx, (size_t)n * 8, hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
   });
@@ -708,15 +705,30 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
 
 ns3d_status ns3d_timing(ns3d_ctx* c, int enable) {
   if (!c) return fail_invalid("ns3d_timing: null context");
-  c->tm = Timing();
-  c->tm.on = enable != 0;
+  return guarded([&] {
+    c->tm = Timing();
+    c->tm.on = enable != 0;
+    if (c->tm.on && c->tev.empty()) {
+      c->tev.resize(4 * kTimedChunk, nullptr);
+      for (auto& e : c->tev) HCHK(hipEventCreate(&e));
+    }
+  });
+}
+
+ns3d_status ns3d_timing_read(ns3d_ctx* c, double out[18]) {
+  if (!c || !out) return fail_invalid("ns3d_timing_read: bad arguments");
+  for (int kind = 0; kind < 2; ++kind)
+    for (int k = 0; k < 3; ++k) {
+      out[kind * 9 + k * 3 + 0] = c->tm.ms[kind][k];
+      out[kind * 9 + k * 3 + 1] = c->tm.launches[kind][k];
+      out[kind * 9 + k * 3 + 2] = c->tm.bytes[kind][k];
+    }
   return NS3D_OK;
 }
 
-ns3d_status ns3d_timing_read(ns3d_ctx* c, double out[5]) {
-  if (!c || !out) return fail_invalid("ns3d_timing_read: bad arguments");
-  out[0] = c->tm.spmv_ms; out[1] = c->tm.spmv_launches; out[2] = c->tm.all_ms; out[3] = c->tm.all_launches;
-  out[4] = c->tm.spmv_bytes;
+ns3d_status ns3d_matrix_nnz(ns3d_ctx* c, int64_t out[4]) {
+  if (!c || !out) return fail_invalid("ns3d_matrix_nnz: bad arguments");
+  for (int q = 0; q < 4; ++q) out[q] = c->sys[q].nnz;
   return NS3D_OK;
 }
 

@@ -26,7 +26,7 @@ hipError_t launch_scan_write(const uint8_t* img, int nx, int ny, int nz, ScanBuf
                              int32_t* pg, int32_t* ug, int32_t* vg, int32_t* wg, int32_t* pcell,
                              int32_t* ucell, int32_t* vcell, int32_t* wcell, hipStream_t s);
 hipError_t launch_build_nbr(const Geo& g, int sys, const int32_t* cell, int n, int32_t* nbr,
-                            hipStream_t s);
+                            unsigned long long* nnz, hipStream_t s);
 
 // ---- assembly (kernels_assemble.hip) ----------------------------------------
 struct Phys {

@@ -176,7 +176,8 @@ scan_write_kernel(const uint8_t* __restrict__ img, int nx, int ny, int nz, int64
 // Neighbour rows of each row in its own system's grid, order E W N S T B,
 // the row itself where the neighbour node does not exist.
 __global__ void __launch_bounds__(kBlock)
-build_nbr_kernel(Geo g, int sys, const int32_t* __restrict__ cell, int n, int32_t* __restrict__ nbr) {
+build_nbr_kernel(Geo g, int sys, const int32_t* __restrict__ cell, int n, int32_t* __restrict__ nbr,
+                 unsigned long long* nnz) {
   const int row = blockIdx.x * kBlock + threadIdx.x;
   if (row >= n) return;
   int i, j, k;
@@ -201,6 +202,9 @@ build_nbr_kernel(Geo g, int sys, const int32_t* __restrict__ cell, int n, int32_
   nbr[(int64_t)DS * n + row] = s > -1 ? s : row;
   nbr[(int64_t)DT * n + row] = t > -1 ? t : row;
   nbr[(int64_t)DB * n + row] = b > -1 ? b : row;
+  // stored entries of the row: diagonal + existing neighbours
+  const unsigned long long cnt = 1ull + (e > -1) + (w > -1) + (nn > -1) + (s > -1) + (t > -1) + (b > -1);
+  atomicAdd(nnz, cnt);
 }
 
 }  // namespace
@@ -223,10 +227,11 @@ hipError_t launch_scan_write(const uint8_t* img, int nx, int ny, int nz, ScanBuf
   return hipGetLastError();
 }
 
-hipError_t launch_build_nbr(const Geo& g, int sys, const int32_t* cell, int n, int32_t* nbr, hipStream_t s) {
+hipError_t launch_build_nbr(const Geo& g, int sys, const int32_t* cell, int n, int32_t* nbr,
+                            unsigned long long* nnz, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(build_nbr_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, sys, cell,
-                     n, nbr);
+                     n, nbr, nnz);
   return hipGetLastError();
 }
 

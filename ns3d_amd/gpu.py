@@ -30,7 +30,7 @@ EXPORTS = [
     "ns3d_compute_velocity_field", "ns3d_compute_hat", "ns3d_assemble_pressure",
     "ns3d_compute_pressure_correction", "ns3d_compute_pressure_field", "ns3d_correct_velocity",
     "ns3d_correct_pressure", "ns3d_compute_div", "ns3d_compute_permeability", "ns3d_get_matrix",
-    "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_sync",
+    "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_sync",
 ]
 
 
@@ -96,6 +96,7 @@ def lib():
                                             P(SolveInfo)]
         L.ns3d_timing.argtypes = [vp, i32]
         L.ns3d_timing_read.argtypes = [vp, P(dbl)]
+        L.ns3d_matrix_nnz.argtypes = [vp, P(i64)]
         L.ns3d_sync.argtypes = [vp]
         _lib = L
     return _lib
@@ -261,10 +262,21 @@ class NavierStokesGPU:
         _check(lib().ns3d_timing(self.h, int(enable)))
 
     def timing_read(self):
-        out = np.zeros(5)
+        """{(kind, kernel): (ms, launches, bytes)} for kind in momentum/pressure
+        and kernel in A/B/C of the fused BiCGSTAB."""
+        out = np.zeros(18)
         _check(lib().ns3d_timing_read(self.h, _p(out, C.c_double)))
-        return dict(spmv_ms=out[0], spmv_launches=out[1], all_ms=out[2], all_launches=out[3],
-                    spmv_bytes=out[4])
+        res = {}
+        for m, kind in enumerate(("momentum", "pressure")):
+            for k, name in enumerate("ABC"):
+                o = 9 * m + 3 * k
+                res[(kind, name)] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
+        return res
+
+    def nnz(self):
+        out = np.zeros(4, np.int64)
+        _check(lib().ns3d_matrix_nnz(self.h, _p(out, C.c_int64)))
+        return [int(x) for x in out]
 
 
 def bicgstab_stencil(diag, off, nbr, b, x0, tol, maxit):

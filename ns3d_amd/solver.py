@@ -1,0 +1,173 @@
+"""ctypes binding of libns3d_host.so (include/ns3d_host.h): one complete
+permeability solve -- NavierStokesBase + SIMPLEC/SIMPLER/SIMPLERR/IDEAL driver,
+host C++ over the GPU C ABI -- on an in-memory image.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import gpu as _gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libns3d_host.so")
+METHODS = {"SIMPLEC": 0, "SIMPLER": 1, "SIMPLERR": 2, "IDEAL": 3}
+
+EXPORTS = ["ns3d_default_run_params", "ns3d_solver_create", "ns3d_solver_info", "ns3d_solver_run",
+           "ns3d_solver_fields", "ns3d_solver_destroy", "ns3d_solver_context", "ns3d_host_last_error",
+           "ns3d_read_ini"]
+
+
+class RunParams(C.Structure):
+    _fields_ = [("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("mu", C.c_double),
+                ("rho", C.c_double), ("p0", C.c_double), ("gradp", C.c_double), ("alpha", C.c_double),
+                ("beta", C.c_double), ("vsolvertol", C.c_double), ("psolvertol", C.c_double),
+                ("pcsolvertol", C.c_double), ("simpletol", C.c_double), ("deltaRMR", C.c_double),
+                ("maxitsolver", C.c_int32), ("maxitSIMPLE", C.c_int32), ("N1", C.c_int32), ("N2", C.c_int32),
+                ("save_step", C.c_int32), ("porevalue", C.c_int32), ("direction", C.c_int32),
+                ("border", C.c_int32), ("method", C.c_int32), ("verbose", C.c_int32),
+                ("write_files", C.c_int32), ("device", C.c_int32), ("outdir", C.c_char_p),
+                ("name", C.c_char_p)]
+
+
+class RunResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("K", C.c_double), ("Q", C.c_double),
+                ("v_moy", C.c_double), ("RMR", C.c_double), ("normdiv", C.c_double), ("reldiv", C.c_double),
+                ("maxdiv", C.c_double), ("seconds", C.c_double), ("np", C.c_int64), ("nu", C.c_int64),
+                ("nv", C.c_int64), ("nw", C.c_int64), ("solver_iterations", C.c_int64 * 4),
+                ("phase_seconds", C.c_double * 8)]
+
+
+PHASES = ["momentum_assembly", "velocity_solves", "pseudo_velocities", "pressure_assembly",
+          "pressure_solves", "corrections", "diagnostics", "other"]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _gpu.lib()  # fails loudly when the GPU library is missing
+        if not os.path.exists(LIB_PATH):
+            raise _gpu.NS3DError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.ns3d_default_run_params.argtypes = [P(RunParams)]
+        L.ns3d_default_run_params.restype = None
+        L.ns3d_solver_create.argtypes = [P(RunParams), P(C.c_uint8), C.c_int, C.c_int, C.c_int, P(C.c_void_p)]
+        L.ns3d_solver_create.restype = C.c_int
+        L.ns3d_solver_info.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int64)]
+        L.ns3d_solver_info.restype = C.c_int
+        L.ns3d_solver_run.argtypes = [C.c_void_p, P(C.c_double), C.c_int, P(C.c_int), P(RunResult)]
+        L.ns3d_solver_run.restype = C.c_int
+        L.ns3d_solver_fields.argtypes = [C.c_void_p] + [P(C.c_double)] * 4
+        L.ns3d_solver_fields.restype = C.c_int
+        L.ns3d_solver_destroy.argtypes = [C.c_void_p]
+        L.ns3d_solver_destroy.restype = None
+        L.ns3d_solver_context.argtypes = [C.c_void_p]
+        L.ns3d_solver_context.restype = C.c_void_p
+        L.ns3d_host_last_error.restype = C.c_char_p
+        L.ns3d_read_ini.argtypes = [C.c_char_p, P(RunParams), C.c_char_p, C.c_int]
+        L.ns3d_read_ini.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def default_params(**kw) -> RunParams:
+    p = RunParams()
+    lib().ns3d_default_run_params(C.byref(p))
+    p.verbose = 0
+    p.write_files = 0
+    for k, v in kw.items():
+        if k == "method":
+            v = METHODS[v.upper()] if isinstance(v, str) else int(v)
+        if k in ("outdir", "name") and isinstance(v, str):
+            v = v.encode()
+        setattr(p, k, v)
+    return p
+
+
+class Solver:
+    """One image, one driver; ``run()`` returns the RES rows and a summary."""
+
+    def __init__(self, img: np.ndarray, **params):
+        self.params = default_params(**params)
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        h = C.c_void_p()
+        rc = lib().ns3d_solver_create(C.byref(self.params), img.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                      *img.shape, C.byref(h))
+        if rc != 0:
+            raise _gpu.NS3DError(lib().ns3d_host_last_error().decode())
+        self.h = h
+        dims = (C.c_int * 3)()
+        cnt = (C.c_int64 * 4)()
+        lib().ns3d_solver_info(self.h, dims, cnt)
+        self.dims = tuple(dims)
+        self.np, self.nu, self.nv, self.nw = (int(x) for x in cnt)
+
+    def run(self, max_rows=100000):
+        rows = np.zeros((max_rows, 14))
+        n = C.c_int(0)
+        res = RunResult()
+        rc = lib().ns3d_solver_run(self.h, rows.ctypes.data_as(C.POINTER(C.c_double)), max_rows, C.byref(n),
+                                   C.byref(res))
+        if rc != 0:
+            raise _gpu.NS3DError(lib().ns3d_host_last_error().decode())
+        summary = {f: getattr(res, f) for f, _ in RunResult._fields_
+                   if f not in ("solver_iterations", "phase_seconds")}
+        summary["solver_iterations"] = list(res.solver_iterations)
+        summary["phases"] = dict(zip(PHASES, list(res.phase_seconds)))
+        return rows[:min(n.value, max_rows)].copy(), summary
+
+    def context(self):
+        """ns3d_ctx* of the GPU library (for timing and structure queries)."""
+        return C.c_void_p(lib().ns3d_solver_context(self.h))
+
+    def timing(self, enable=True):
+        _gpu._check(_gpu.lib().ns3d_timing(self.context(), int(enable)))
+
+    def timing_read(self):
+        out = np.zeros(18)
+        _gpu._check(_gpu.lib().ns3d_timing_read(self.context(), out.ctypes.data_as(C.POINTER(C.c_double))))
+        res = {}
+        for m, kind in enumerate(("momentum", "pressure")):
+            for k, name in enumerate("ABC"):
+                o = 9 * m + 3 * k
+                res[(kind, name)] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
+        return res
+
+    def nnz(self):
+        out = np.zeros(4, np.int64)
+        _gpu._check(_gpu.lib().ns3d_matrix_nnz(self.context(), out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return [int(x) for x in out]
+
+    def fields(self):
+        p, u, v, w = (np.empty(n) for n in (self.np, self.nu, self.nv, self.nw))
+        ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        rc = lib().ns3d_solver_fields(self.h, ptr(p), ptr(u), ptr(v), ptr(w))
+        if rc != 0:
+            raise _gpu.NS3DError(lib().ns3d_host_last_error().decode())
+        return dict(p=p, u=u, v=v, w=w)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ns3d_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve(img, **params):
+    s = Solver(img, **params)
+    try:
+        rows, summ = s.run()
+        summ["fields"] = s.fields()
+        return rows, summ
+    finally:
+        s.close()

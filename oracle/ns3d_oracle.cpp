@@ -123,6 +123,7 @@ struct Solver {
   double tol = 2.220446049250313e-16;  // Eigen's default: epsilon
   long maxit = -1;
   long iterations = -1;  // -1: never solved (the reference reads garbage)
+  long total = 0;        // summed over the run
   int info = 0;
   double error = 0.0;
 };
@@ -385,6 +386,7 @@ void solve_with_guess(Solver& s, const std::vector<double>& b, std::vector<doubl
   double err = s.tol;
   bicgstab(*s.A, s.invdiag, b, x, iters, err);
   s.iterations = iters;
+  s.total += iters;
   s.error = err;
   s.info = (err <= s.tol) ? 0 : 2;
 }
@@ -1149,6 +1151,7 @@ void run_driver(Ctx& c, int method, double simpletol, int maxit, int N1, int N2)
   c.us.tol = c.vs.tol = c.ws.tol = c.vsolvertol;
   c.us.maxit = c.vs.maxit = c.ws.maxit = c.maxitsolver;
   c.us.iterations = c.vs.iterations = c.ws.iterations = c.ps.iterations = -1;
+  c.us.total = c.vs.total = c.ws.total = c.ps.total = 0;
   c.alpha = c.alpha0;
   c.beta = c.beta0;
   c.avgRMR.clear();
@@ -1409,7 +1412,8 @@ void orc_spmv_ell(int n, const int32_t* cnt, const int32_t* col, const double* v
 
 // Full driver.  Returns the number of RES rows; rows_out (may be null) gets
 // 14 doubles per row: it Q v_moy RMR K gradP normdiv reldiv velocityNorm
-// maxdiv it_u it_v it_w it_pc.  summary: final_it, final_K, seconds.
+// maxdiv it_u it_v it_w it_pc.  summary[14]: final_it, final_K, seconds,
+// 7 phase timers, summed solver iterations u, v, w, p.
 int orc_solve(void* h, int method, double simpletol, int maxit, int N1, int N2, double* rows_out,
               int max_rows, double* summary) {
   Ctx& c = *(Ctx*)h;
@@ -1429,6 +1433,8 @@ int orc_solve(void* h, int method, double simpletol, int maxit, int N1, int N2, 
   if (summary) {
     summary[0] = c.final_it; summary[1] = c.final_K; summary[2] = t1 - t0;
     for (int q = 0; q < 7; ++q) summary[3 + q] = c.timers[q];
+    summary[10] = (double)c.us.total; summary[11] = (double)c.vs.total;
+    summary[12] = (double)c.ws.total; summary[13] = (double)c.ps.total;
   }
   return nr;
 }

@@ -1,0 +1,97 @@
+"""Host-side pieces that need no GPU: the .ini grammar, the NS3D command
+line's error paths and the .vti codec (CPU only)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from ns3d_amd import solver, vti
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NS3D = os.path.join(ROOT, "ns3d_amd", "NS3D")
+
+
+def read_ini(tmp_path, text):
+    f = tmp_path / "case.ini"
+    f.write_text(text)
+    p = solver.RunParams()
+    buf = C.create_string_buffer(4096)
+    rc = solver.lib().ns3d_read_ini(str(f).encode(), C.byref(p), buf, 4096)
+    names = buf.raw.split(b"\0")[:5]
+    return rc, p, [n.decode() for n in names]
+
+
+def test_ini_defaults_and_grammar(tmp_path, capfd):
+    rc, p, names = read_ini(tmp_path, """
+# comment line
+INPUT = data/my#image.vti   # inline comment after whitespace
+Dx = 2e-6
+mu=1e-3
+rho = 1e3 # density
+pcSolverTol = 1e-4
+deltaRMR = 0.5
+maxitSIMPLE = 30
+method = simplerr
+bogus = 1
+no equals sign here
+""")
+    assert rc == 0
+    assert names[0] == "data/my#image.vti" and names[1:] == ["none"] * 4
+    assert p.dx == 2e-6 and p.dy == 2e-6 and p.dz == 2e-6      # dy/dz default to dx
+    assert p.pcsolvertol == 1e-4 and p.maxitSIMPLE == 30
+    assert p.method == solver.METHODS["SIMPLERR"]
+    assert p.deltaRMR == 1e-4   # the reference never matches this key (navierstokes3D.cxx:342)
+    assert p.vsolvertol == 1e-5 and p.psolvertol == 1e-6 and p.maxitsolver == 1000 and p.N1 == 2
+    assert p.alpha == 0.8 and p.beta == 1.0 and p.p0 == 1e5 and p.gradp == 1e2 and p.save_step == 20
+    err = capfd.readouterr().err
+    assert "unknown key 'deltarmr'" in err and "unknown key 'bogus'" in err and "no '=' found" in err
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("dx = 1\n", "required key 'input'"),
+    ("input = a.vti\nmu = abc\n", "expects a number"),
+    ("input = a.vti\nmaxitsolver = 1.5\n", "expects a integer"),
+    ("input = a.vti\ndirection = 4\n", "'direction' must be"),
+    ("input = a.vti\nmethod = PISO\n", "unknown method"),
+    ("input = a.vti\nrho = 0\n", "'rho' must be > 0"),
+])
+def test_ini_fatal_errors(tmp_path, capfd, text, msg):
+    rc, _, _ = read_ini(tmp_path, text)
+    assert rc == 1
+    assert msg in capfd.readouterr().err
+
+
+def test_ini_porevalue_warning(tmp_path, capfd):
+    rc, p, _ = read_ini(tmp_path, "input = a.vti\nporevalue = 7\n")
+    assert rc == 0 and p.porevalue == 0
+    assert "treating '7' as 0" in capfd.readouterr().err
+
+
+def test_cli_usage_and_errors(tmp_path):
+    r = subprocess.run([NS3D, "-h"], capture_output=True, text=True)
+    assert r.returncode == 0 and "Usage: NS3D" in r.stdout
+    r = subprocess.run([NS3D], capture_output=True, text=True)
+    assert r.returncode == 1
+    r = subprocess.run([NS3D, str(tmp_path / "missing.ini")], capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot open input file" in r.stderr
+    ini = tmp_path / "bad.ini"
+    ini.write_text("input = nowhere.vti\n")
+    r = subprocess.run([NS3D, str(ini)], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 1 and "cannot open nowhere.vti" in r.stderr
+
+
+def test_vti_roundtrip(tmp_path):
+    rng = np.random.default_rng(0)
+    a = (rng.random((7, 5, 3)) < 0.4).astype(np.uint8)
+    vti.write_vti(str(tmp_path / "a.vti"), a)
+    assert np.array_equal(vti.read_vti(str(tmp_path / "a.vti")), a)
+    f = rng.standard_normal((6, 4, 5))
+    vti.write_vti(str(tmp_path / "f.vti"), f)
+    assert np.array_equal(vti.read_vti(str(tmp_path / "f.vti")), f)
+    v = rng.standard_normal((3, 4, 2, 3))
+    vti.write_vti(str(tmp_path / "v.vti"), v, compress=False)
+    assert np.array_equal(vti.read_vti(str(tmp_path / "v.vti")), v)
