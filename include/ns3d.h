@@ -60,17 +60,23 @@ enum {
 /* Pressure-type systems (ns3d_assemble_pressure). */
 enum { NS3D_PRESSURE_FULL = 0, NS3D_PC_SIMPLE = 1, NS3D_PC_SIMPLEC = 2 };
 
+/* Arithmetic of the linear solves (ns3d_set_precision). */
+enum { NS3D_PRECISION_FP64 = 0, NS3D_PRECISION_MIXED = 1 };
+
 /* Velocity corrections (ns3d_correct_velocity). */
 enum { NS3D_CORR_SIMPLE = 0, NS3D_CORR_SIMPLEC = 1, NS3D_CORR_SIMPLER = 2 };
 
 /* Result of one linear solve, with Eigen's BiCGSTAB meaning: iterations(),
  * info() (0 Success, 1 NumericalIssue, 2 NoConvergence, 3 InvalidInput) and
- * error() = |r|/|b|. */
+ * error() = |r|/|b|.  work counts the iterations actually executed: 0 for a
+ * zero right-hand side (which Eigen reports as maxit iterations), summed over
+ * the tightened retries of ns3d_compute_pressure_field. */
 typedef struct {
   int64_t iterations;
   int32_t info;
   int32_t restarts;
   double error;
+  int64_t work;
 } ns3d_solve_info;
 
 /* Physical and numerical constants of one solve.  The per-iteration
@@ -192,14 +198,26 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
                                   const double* b, double* x, double tol, int64_t maxit,
                                   ns3d_solve_info* info);
 
+/* Linear-solve arithmetic.  NS3D_PRECISION_FP64 (default) is the reference's
+ * Eigen BiCGSTAB in FP64.  NS3D_PRECISION_MIXED keeps assembly, residuals,
+ * corrections and reductions in FP64 and solves each system by FP64 defect
+ * correction around FP32-storage BiCGSTAB sweeps (FP32 coefficients and
+ * vectors, FP64 arithmetic in registers): r = b - A x in FP64, inner solve of
+ * A d = r to max(inner_tol, tol |b| / 2|r|), x += d, until |b - A x| <= tol |b|.
+ * Reported iterations are summed inner iterations; error is the true FP64
+ * relative residual.  This form is not part of the reference. */
+ns3d_status ns3d_set_precision(ns3d_ctx* ctx, int mode, double inner_tol);
+
 /* Device time of the fused BiCGSTAB kernels, measured with HIP events on
  * the context's stream around every launch (enable with ns3d_timing(ctx, 1);
- * launches after a solve has stopped are not counted).  out[18]: for system
- * kind m (0 momentum, 1 pressure) and kernel k (0: A = p update + SpMV + r0.v,
- * 1: B = s + SpMV + t.s/t.t, 2: C = x/r update + r0.r/r.r), at
- * out[9m + 3k]: milliseconds, launches, algorithmic bytes. */
+ * launches after a solve has stopped are not counted).  out[40]: for storage
+ * precision q (0 FP64, 1 FP32), system kind m (0 momentum, 1 pressure) and
+ * kernel k (0: A = p update + SpMV + r0.v, 1: B = s + SpMV + t.s/t.t, 2: C =
+ * x/r update + r0.r/r.r), out[18q + 9m + 3k] = milliseconds, launches,
+ * algorithmic bytes of the row kernel; out[36 + 2q + m] = milliseconds in the
+ * one-block finalize kernels. */
 ns3d_status ns3d_timing(ns3d_ctx* ctx, int enable);
-ns3d_status ns3d_timing_read(ns3d_ctx* ctx, double out[18]);
+ns3d_status ns3d_timing_read(ns3d_ctx* ctx, double out[40]);
 
 /* Stored entries (diagonal + existing neighbours) of the u, v, w, p systems. */
 ns3d_status ns3d_matrix_nnz(ns3d_ctx* ctx, int64_t out[4]);

@@ -29,11 +29,15 @@ typedef struct {
   int32_t maxitsolver, maxitSIMPLE, N1, N2, save_step;
   int32_t porevalue, direction, border;
   int32_t method;               /* NS3D_METHOD_* */
-  int32_t verbose;              /* 1: print progress like the reference */
+  int32_t verbose;              /* 1: print like the reference (stdout); 2: one
+                                   line per outer iteration on stderr */
   int32_t write_files;          /* 1: write INFO/RES logs and .vti fields */
   int32_t device;               /* GPU index, -1 = current */
   const char* outdir;           /* directory for the output files */
   const char* name;             /* image file name (output names derive from it) */
+  int32_t precision;            /* NS3D_PRECISION_FP64 (default) or _MIXED (ns3d.h) */
+  int32_t reserved;
+  double inner_tol;             /* mixed form: relative tolerance of the FP32 sweeps */
 } ns3d_run_params;
 
 typedef struct {
@@ -46,6 +50,8 @@ typedef struct {
   int64_t solver_iterations[4]; /* summed BiCGSTAB iterations: u, v, w, p */
   double phase_seconds[8];      /* momentum asm, velocity solves, hats, p asm,
                                    p solves, corrections, diagnostics, other */
+  int64_t solver_work[4];       /* summed executed BiCGSTAB iterations (a zero
+                                   right-hand side counts 0, not maxit) */
 } ns3d_run_result;
 
 typedef struct ns3d_solver ns3d_solver;

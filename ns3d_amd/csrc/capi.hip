@@ -43,18 +43,23 @@ struct SysMem {
   double* coef = nullptr;
   double* invdiag = nullptr;
   double* rowsum = nullptr;
-  int64_t last_iters = 16;   // chunk-size hint for the next solve
+  int64_t last_iters = 16;   // chunk-size hint for the next solve (plain)
+  int64_t last_iters32 = 16; // same for the FP32 inner solves (mixed)
   int64_t nnz = 0;           // stored entries (diagonal + existing neighbours)
+  float* coef32 = nullptr;   // mixed form: FP32 copies
+  float* invdiag32 = nullptr;
   SysView view() const { return SysView{n, d0, grid, cell, nbr, coef, invdiag, rowsum}; }
+  SMat<float> mat32() const { return SMat<float>{n, nbr, coef32, invdiag32}; }
 };
 
-// Device time of the fused BiCGSTAB kernels, per system kind (0 momentum,
-// 1 pressure) and kernel (A, B, C): milliseconds, launches that did work,
-// algorithmic bytes.
+// Device time of the fused BiCGSTAB kernels, per storage precision (0 FP64,
+// 1 FP32), system kind (0 momentum, 1 pressure) and kernel (A, B, C):
+// milliseconds, launches that did work, algorithmic bytes.
 constexpr int kTimedChunk = 64;
 struct Timing {
   bool on = false;
-  double ms[2][3] = {{0}}, launches[2][3] = {{0}}, bytes[2][3] = {{0}};
+  double ms[2][2][3] = {}, launches[2][2][3] = {}, bytes[2][2][3] = {};
+  double fin_ms[2][2] = {};  // one-block finalize kernels, including launch gaps
 };
 
 }  // namespace
@@ -76,8 +81,13 @@ struct ns3d_ctx {
   double* temp = nullptr;   // saved field for revert/clamp
   // solver workspace
   BWork bw{};
+  SWork<float> bw32{};      // mixed form: FP32 inner-solve vectors
+  float* d32 = nullptr;     // inner-solve unknown (the correction)
+  float* r32 = nullptr;     // inner-solve right-hand side
+  double* rres = nullptr;   // FP64 residual of the defect correction
+  int precision = NS3D_PRECISION_FP64;
+  double inner_tol = 1e-4;
   double* partials = nullptr;
-  unsigned int* counters = nullptr;  // field reductions
   double* result = nullptr;          // reduction results (device)
   BState* hstate = nullptr;          // pinned copy of the solver state
   double* hresult = nullptr;         // pinned copy of results
@@ -163,71 +173,68 @@ Vel3 vel3(ns3d_ctx* c, int set) {
   return Vel3{c->vel[set][0], c->vel[set][1], c->vel[set][2]};
 }
 
-// Algorithmic bytes of the fused BiCGSTAB kernels per launch: the matrix is
-// nnz*(8+4) (coefficient + 32-bit column per stored entry), plus 8 bytes per
-// row per vector stream -- A: r, p, v, M^-1, r0 read, p, v written (7);
-// B: r, v, M^-1 read, t written (4); C: x, p, r, v, t, M^-1, r0 read, x, r
-// written (9).
-double kernel_bytes(int64_t n, int64_t nnz, int k) {
+// Algorithmic bytes of the fused BiCGSTAB kernels per launch, for storage of
+// vb bytes per value: the matrix is nnz*(vb+4) (coefficient + 32-bit column
+// per stored entry), plus vb bytes per row per vector stream -- A: r, p, v,
+// M^-1, r0 read, p, v written (7); B: r, v, M^-1 read, t written (4); C: x, p,
+// r, v, t, M^-1, r0 read, x, r written (9).
+double kernel_bytes(int64_t n, int64_t nnz, int k, int vb) {
   static const double streams[3] = {7.0, 4.0, 9.0};
-  const double mat = (k < 2) ? (double)nnz * 12.0 : 0.0;
-  return mat + streams[k] * 8.0 * (double)n;
+  const double mat = (k < 2) ? (double)nnz * (vb + 4.0) : 0.0;
+  return mat + streams[k] * vb * (double)n;
 }
 
-// One BiCGSTAB solve on system sid with x as guess (Eigen semantics).
-void bicgstab(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int64_t maxit,
-              ns3d_solve_info* info) {
-  SysMem& S = c->sys[sid];
-  const SysView V = S.view();
+// Queued BiCGSTAB iterations with status polling between chunks.  Leaves the
+// final solver state in c->hstate.  hint: iterations of the previous solve of
+// this system (first chunk size), updated on return.
+template <typename T>
+void bicg_loop(ns3d_ctx* c, const SysMem& S, SMat<T> M, T* x, const T* b, SWork<T> w, double tol, int64_t maxit,
+               int64_t& hint, int kind) {
   hipStream_t s = c->stream;
-  if (S.n == 0) {
-    info->iterations = 0; info->info = 0; info->restarts = 0; info->error = 0;
-    return;
-  }
-  BWork w = c->bw;
-  HCHK(launch_bicg_init(V, x, b, w, tol, (long long)maxit, s));
+  HCHK(launch_bicg_init<T>(M, x, b, w, tol, (long long)maxit, s));
   HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));
-  int64_t chunk = S.last_iters > 4 ? (S.last_iters * 3) / 4 : 4;
+  int64_t chunk = hint > 4 ? (hint * 3) / 4 : 4;
   int64_t launched = 0;
   const int64_t cap = 4 * (maxit > 0 ? maxit : 1) + 64;
   long long exec_before = 0;
+  const int prec = sizeof(T) == 4 ? 1 : 0;
   while (c->hstate->status == BS_RUNNING || c->hstate->status == BS_RESTART) {
-    if (c->hstate->status == BS_RESTART) {
-      HCHK(launch_bicg_restart(V, x, b, w, s));
-    }
+    if (c->hstate->status == BS_RESTART) HCHK(launch_bicg_restart<T>(M, x, b, w, s));
     if (chunk > maxit + 1) chunk = maxit + 1;
     if (c->tm.on) {
-      // event pairs around every kernel; read back after the chunk's sync,
+      // events around every kernel, read back after the chunk's sync,
       // counting only the iterations that did work
       if (chunk > kTimedChunk) chunk = kTimedChunk;
       for (int64_t q = 0; q < chunk; ++q) {
-        HCHK(hipEventRecord(c->tev[4 * q + 0], s));
-        HCHK(launch_bicg_A(V, w, s));
-        HCHK(hipEventRecord(c->tev[4 * q + 1], s));
-        HCHK(launch_bicg_B(V, w, s));
-        HCHK(hipEventRecord(c->tev[4 * q + 2], s));
-        HCHK(launch_bicg_C(V, x, w, s));
-        HCHK(hipEventRecord(c->tev[4 * q + 3], s));
+        hipEvent_t* e = &c->tev[7 * q];
+        HCHK(hipEventRecord(e[0], s));
+        HCHK(launch_bicg_A<T>(M, w, s, e[1]));
+        HCHK(hipEventRecord(e[2], s));
+        HCHK(launch_bicg_B<T>(M, w, s, e[3]));
+        HCHK(hipEventRecord(e[4], s));
+        HCHK(launch_bicg_C<T>(M, x, w, s, e[5]));
+        HCHK(hipEventRecord(e[6], s));
       }
       HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
       HCHK(hipStreamSynchronize(s));
       const long long did = c->hstate->nexec - exec_before;
       exec_before = c->hstate->nexec;
-      const int kind = (sid == 3) ? 1 : 0;
       for (long long q = 0; q < did && q < chunk; ++q)
         for (int k = 0; k < 3; ++k) {
-          float ms = 0.f;
-          HCHK(hipEventElapsedTime(&ms, c->tev[4 * q + k], c->tev[4 * q + k + 1]));
-          c->tm.ms[kind][k] += ms;
-          c->tm.launches[kind][k] += 1;
-          c->tm.bytes[kind][k] += kernel_bytes(S.n, S.nnz, k);
+          float ms = 0.f, mf = 0.f;
+          HCHK(hipEventElapsedTime(&ms, c->tev[7 * q + 2 * k], c->tev[7 * q + 2 * k + 1]));
+          HCHK(hipEventElapsedTime(&mf, c->tev[7 * q + 2 * k + 1], c->tev[7 * q + 2 * k + 2]));
+          c->tm.ms[prec][kind][k] += ms;
+          c->tm.fin_ms[prec][kind] += mf;
+          c->tm.launches[prec][kind][k] += 1;
+          c->tm.bytes[prec][kind][k] += kernel_bytes(S.n, S.nnz, k, (int)sizeof(T));
         }
     } else {
       for (int64_t q = 0; q < chunk; ++q) {
-        HCHK(launch_bicg_A(V, w, s));
-        HCHK(launch_bicg_B(V, w, s));
-        HCHK(launch_bicg_C(V, x, w, s));
+        HCHK(launch_bicg_A<T>(M, w, s));
+        HCHK(launch_bicg_B<T>(M, w, s));
+        HCHK(launch_bicg_C<T>(M, x, w, s));
       }
       HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
       HCHK(hipStreamSynchronize(s));
@@ -236,20 +243,104 @@ void bicgstab(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int6
     if (launched > cap) throw Invalid{"BiCGSTAB did not stop within its iteration cap"};
     chunk = 16;
   }
+  if (c->hstate->status != BS_ZERO_RHS) hint = c->hstate->nexec > 0 ? c->hstate->nexec : 4;
+}
+
+// One BiCGSTAB solve on system sid with x as guess, Eigen semantics (FP64).
+void bicgstab64(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int64_t maxit, ns3d_solve_info* info) {
+  SysMem& S = c->sys[sid];
+  if (S.n == 0) {
+    info->iterations = 0; info->info = 0; info->restarts = 0; info->error = 0; info->work = 0;
+    return;
+  }
+  bicg_loop<double>(c, S, mat64(S.view()), x, b, c->bw, tol, maxit, S.last_iters, sid == 3 ? 1 : 0);
   const BState& st = *c->hstate;
   if (st.status == BS_ZERO_RHS) {
-    HCHK(launch_fill(x, S.n, 0.0, s));
+    HCHK(launch_fill(x, S.n, 0.0, c->stream));
     info->iterations = maxit;
     info->error = tol;
     info->info = 0;
     info->restarts = 0;
+    info->work = 0;
   } else {
     info->iterations = st.i;
+    info->work = st.i;
     info->error = std::sqrt(st.rr / st.rhs_sqnorm);
     info->info = (info->error <= tol) ? 0 : 2;
     info->restarts = (int32_t)st.restarts;
-    S.last_iters = st.nexec > 0 ? st.nexec : 4;
   }
+}
+
+// Mixed form: FP64 defect correction around FP32 inner BiCGSTAB solves.
+//   r = b - A x (FP64); while |r| > tol |b|: solve A32 d = r32 to a relative
+//   tolerance max(inner_tol, tol |b| / (2 |r|)); x += d; r = b - A x.
+// iterations = summed inner iterations; error = |b - A x| / |b| (true FP64
+// residual); a zero right-hand side and an already-converged guess behave as
+// in the plain solve (x = 0 with iterations = maxit, and 0 iterations).
+void bicgstab_mixed(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int64_t maxit,
+                    ns3d_solve_info* info) {
+  SysMem& S = c->sys[sid];
+  hipStream_t s = c->stream;
+  info->restarts = 0;
+  info->work = 0;
+  if (S.n == 0) {
+    info->iterations = 0; info->info = 0; info->error = 0;
+    return;
+  }
+  const SMat<double> M64 = mat64(S.view());
+  const SMat<float> M32 = S.mat32();
+  auto residual = [&](double& rr, double& bb) {
+    HCHK(launch_residual64(M64, x, b, c->rres, c->r32, c->partials, c->result, s));
+    HCHK(hipMemcpyAsync(c->hresult, c->result, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    rr = c->hresult[0];
+    bb = c->hresult[1];
+  };
+  double rr, bb;
+  residual(rr, bb);
+  if (bb == 0) {
+    HCHK(launch_fill(x, S.n, 0.0, s));
+    info->iterations = maxit; info->error = tol; info->info = 0;
+    return;
+  }
+  const double tol2 = tol * tol * bb;
+  int64_t total = 0;
+  int outer = 0;
+  while (rr > tol2 && total < maxit && outer < 100) {
+    const double rel = std::sqrt(rr / bb);
+    const double tin = std::max(c->inner_tol, 0.5 * tol / rel);
+    HCHK(launch_fill_f32(c->d32, S.n, s));
+    bicg_loop<float>(c, S, M32, c->d32, c->r32, c->bw32, tin, maxit - total, S.last_iters32, sid == 3 ? 1 : 0);
+    const BState& st = *c->hstate;
+    const int64_t k = (st.status == BS_ZERO_RHS) ? 0 : st.i;
+    info->restarts += (int32_t)st.restarts;
+    total += k;
+    ++outer;
+    if (k == 0) break;  // no progress possible in FP32
+    HCHK(launch_add_f32(x, c->d32, S.n, s));
+    residual(rr, bb);
+  }
+  info->iterations = total;
+  info->work = total;
+  info->error = std::sqrt(rr / bb);
+  info->info = (info->error <= tol) ? 0 : 2;
+}
+
+void bicgstab(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int64_t maxit, ns3d_solve_info* info) {
+  if (c->precision == NS3D_PRECISION_MIXED && c->sys[sid].coef32) bicgstab_mixed(c, sid, x, b, tol, maxit, info);
+  else bicgstab64(c, sid, x, b, tol, maxit, info);
+}
+
+// FP32 copies of a system after its assembly (mixed form only).
+void refresh_f32(ns3d_ctx* c, int sid) {
+  if (c->precision != NS3D_PRECISION_MIXED) return;
+  SysMem& S = c->sys[sid];
+  if (S.n == 0) return;
+  if (!S.coef32) {
+    S.coef32 = c->alloc<float>((int64_t)7 * S.n);
+    S.invdiag32 = c->alloc<float>(S.n);
+  }
+  HCHK(launch_coef_to_f32(S.coef, S.invdiag, S.n, S.coef32, S.invdiag32, c->stream));
 }
 
 void alloc_system(ns3d_ctx* c, int sid, int n, int d0, int32_t* grid, bool momentum) {
@@ -372,13 +463,10 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
     w.v[0] = c->alloc<double>(nmax);
     w.v[1] = c->alloc<double>(nmax);
     w.t = c->alloc<double>(nmax);
-    const int64_t pb = padded_blocks(std::max<int64_t>(nmax, (int64_t)ny * nz));
-    c->partials = c->alloc<double>(8 * pb);
+    c->partials = c->alloc<double>(8 * kMaxRedBlocks);
     w.partials = c->partials;
     w.st = c->alloc<BState>(1);
     HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), s));
-    c->counters = c->alloc<unsigned int>(32);
-    HCHK(hipMemsetAsync(c->counters, 0, 32 * sizeof(unsigned int), s));
     c->result = c->alloc<double>(16);
     HCHK(hipHostMalloc((void**)&c->hstate, sizeof(BState), hipHostMallocDefault));
     HCHK(hipHostMalloc((void**)&c->hresult, 16 * sizeof(double), hipHostMallocDefault));
@@ -501,6 +589,7 @@ ns3d_status ns3d_assemble_momentum(ns3d_ctx* c, double alpha) {
     HCHK(launch_assemble_momentum(0, g, c->sys[0].view(), vc, ph, c->stream));
     HCHK(launch_assemble_momentum(1, g, c->sys[1].view(), vc, ph, c->stream));
     if (c->sys[2].n > 0) HCHK(launch_assemble_momentum(2, g, c->sys[2].view(), vc, ph, c->stream));
+    for (int q = 0; q < 3; ++q) refresh_f32(c, q);
   });
 }
 
@@ -511,6 +600,7 @@ ns3d_status ns3d_compute_velocity_field(ns3d_ctx* c, double alpha, int limit, do
     const Phys ph = phys_of(c, alpha, 1.0);
     const Geo g = c->geo();
     hipStream_t s = c->stream;
+    for (int q = 0; q < 3; ++q) info[q] = ns3d_solve_info{};
     for (int q = 0; q < 3; ++q) {
       SysMem& S = c->sys[q];
       if (q == 2 && S.n <= 0) break;
@@ -545,6 +635,7 @@ ns3d_status ns3d_assemble_pressure(ns3d_ctx* c, int kind, int relax, double beta
     const Phys ph = phys_of(c, 1.0, beta);
     HCHK(launch_assemble_pressure(c->geo(), c->sys[3].view(), c->sys[0].view(), c->sys[1].view(),
                                   c->sys[2].view(), vel3(c, vel_set), c->p, c->b, kind, relax, ph, c->stream));
+    refresh_f32(c, 3);
   });
 }
 
@@ -564,10 +655,13 @@ ns3d_status ns3d_compute_pressure_field(ns3d_ctx* c, double tol, int64_t maxit, 
     double* ptemp = c->temp;  // temp holds max(n) doubles
     HCHK(hipMemcpyAsync(ptemp, c->p, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
     bicgstab(c, 3, ptemp, c->b, tol, maxit, info);
+    int64_t work = info->work;
     while (info->iterations == 0 && info->info == 0 && tol > 1e-15) {
       tol *= 0.1;
       bicgstab(c, 3, ptemp, c->b, tol, maxit, info);
+      work += info->work;
     }
+    info->work = work;
     if (info->iterations > 0) HCHK(hipMemcpyAsync(c->p, ptemp, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
     HCHK(hipStreamSynchronize(s));
   });
@@ -590,7 +684,7 @@ ns3d_status ns3d_correct_pressure(ns3d_ctx* c, double beta, double out[3]) {
   return guarded([&] {
     const int64_t n = c->sys[3].n;
     need(n > 0, "ns3d_correct_pressure: no pressure nodes");
-    HCHK(launch_correct_pressure(c->p, c->pc, n, beta, c->partials, &c->counters[0], c->result, c->stream));
+    HCHK(launch_correct_pressure(c->p, c->pc, n, beta, c->partials, c->result, c->stream));
     HCHK(hipMemcpyAsync(c->hresult, c->result, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HCHK(hipStreamSynchronize(c->stream));
     out[0] = beta * c->hresult[0];
@@ -605,7 +699,7 @@ ns3d_status ns3d_compute_div(ns3d_ctx* c, int vel_set, double out[4]) {
     const Vel3 v = vel3(c, vel_set);
     if (c->sys[3].n == 0) { out[0] = out[1] = out[2] = out[3] = 0; return; }
     HCHK(launch_compute_div(c->geo(), c->sys[3].view(), v.u, v.v, v.w, c->prm.dx, c->prm.dy, c->prm.dz,
-                            c->partials, &c->counters[1], c->result, c->stream));
+                            c->partials, c->result, c->stream));
     HCHK(hipMemcpyAsync(c->hresult, c->result, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HCHK(hipStreamSynchronize(c->stream));
     out[0] = c->hresult[0];
@@ -619,8 +713,7 @@ ns3d_status ns3d_compute_permeability(ns3d_ctx* c, double out[5]) {
   if (!c || !c->have_params || !out) return fail_invalid("ns3d_compute_permeability: bad arguments");
   return guarded([&] {
     const int border = c->prm.border;
-    HCHK(launch_permeability(c->geo(), c->vel[NS3D_SET_VEL][0], c->p, border, c->partials, &c->counters[2],
-                             c->result, c->stream));
+    HCHK(launch_permeability(c->geo(), c->vel[NS3D_SET_VEL][0], c->p, border, c->partials, c->result, c->stream));
     HCHK(hipMemcpyAsync(c->hresult, c->result, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HCHK(hipStreamSynchronize(c->stream));
     const double* r = c->hresult;
@@ -689,7 +782,7 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
     w.p[0] = c->alloc<double>(n); w.p[1] = c->alloc<double>(n);
     w.v[0] = c->alloc<double>(n); w.v[1] = c->alloc<double>(n);
     w.t = c->alloc<double>(n);
-    c->partials = c->alloc<double>(8 * (int64_t)padded_blocks(n));
+    c->partials = c->alloc<double>(8 * kMaxRedBlocks);
     w.partials = c->partials;
     w.st = c->alloc<BState>(1);
     HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), s));
@@ -709,21 +802,55 @@ ns3d_status ns3d_timing(ns3d_ctx* c, int enable) {
     c->tm = Timing();
     c->tm.on = enable != 0;
     if (c->tm.on && c->tev.empty()) {
-      c->tev.resize(4 * kTimedChunk, nullptr);
+      c->tev.resize(7 * kTimedChunk, nullptr);
       for (auto& e : c->tev) HCHK(hipEventCreate(&e));
     }
   });
 }
 
-ns3d_status ns3d_timing_read(ns3d_ctx* c, double out[18]) {
+ns3d_status ns3d_timing_read(ns3d_ctx* c, double out[40]) {
   if (!c || !out) return fail_invalid("ns3d_timing_read: bad arguments");
-  for (int kind = 0; kind < 2; ++kind)
-    for (int k = 0; k < 3; ++k) {
-      out[kind * 9 + k * 3 + 0] = c->tm.ms[kind][k];
-      out[kind * 9 + k * 3 + 1] = c->tm.launches[kind][k];
-      out[kind * 9 + k * 3 + 2] = c->tm.bytes[kind][k];
+  for (int prec = 0; prec < 2; ++prec)
+    for (int kind = 0; kind < 2; ++kind) {
+      for (int k = 0; k < 3; ++k) {
+        double* o = out + prec * 18 + kind * 9 + k * 3;
+        o[0] = c->tm.ms[prec][kind][k];
+        o[1] = c->tm.launches[prec][kind][k];
+        o[2] = c->tm.bytes[prec][kind][k];
+      }
+      out[36 + prec * 2 + kind] = c->tm.fin_ms[prec][kind];
     }
   return NS3D_OK;
+}
+
+ns3d_status ns3d_set_precision(ns3d_ctx* c, int mode, double inner_tol) {
+  if (!c || (mode != NS3D_PRECISION_FP64 && mode != NS3D_PRECISION_MIXED))
+    return fail_invalid("ns3d_set_precision: bad arguments");
+  if (mode == NS3D_PRECISION_MIXED && !(inner_tol > 0 && inner_tol < 1))
+    return fail_invalid("ns3d_set_precision: inner tolerance must be in (0, 1)");
+  return guarded([&] {
+    if (mode == NS3D_PRECISION_MIXED && !c->d32) {
+      int64_t nmax = 1;
+      for (int q = 0; q < 4; ++q) nmax = std::max<int64_t>(nmax, c->sys[q].n);
+      SWork<float>& w = c->bw32;
+      w.r = c->alloc<float>(nmax);
+      w.r0 = c->alloc<float>(nmax);
+      w.p[0] = c->alloc<float>(nmax);
+      w.p[1] = c->alloc<float>(nmax);
+      w.v[0] = c->alloc<float>(nmax);
+      w.v[1] = c->alloc<float>(nmax);
+      w.t = c->alloc<float>(nmax);
+      w.partials = c->partials;
+      w.st = c->bw.st;
+      c->d32 = c->alloc<float>(nmax);
+      c->r32 = c->alloc<float>(nmax);
+      c->rres = c->alloc<double>(nmax);
+    }
+    c->precision = mode;
+    if (mode == NS3D_PRECISION_MIXED) c->inner_tol = inner_tol;
+    for (int q = 0; q < 4; ++q) refresh_f32(c, q);
+    HCHK(hipStreamSynchronize(c->stream));
+  });
 }
 
 ns3d_status ns3d_matrix_nnz(ns3d_ctx* c, int64_t out[4]) {

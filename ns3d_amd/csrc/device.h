@@ -140,46 +140,64 @@ __device__ __forceinline__ double block_reduce_rt(int op, double v, double* sh) 
   return block_reduce<RMIN>(v, sh);
 }
 
-// Grid-wide reduction of NV values (ops[q] = RSUM/RMAX/RMIN) with the "last
-// block finishes" scheme.  Each block writes its partials; the block that
-// arrives last re-reduces all partials in block order (deterministic for a
-// given grid size) and returns true in all of its threads, with out[] valid in
-// thread 0.  counter must be 0 on entry; the last block resets it to 0.
+// Grid-wide reductions are two-stage: every block of a row kernel stores its
+// partials with plain stores (slot q*gridDim.x + blockIdx.x), and a one-block
+// finalize kernel launched next on the same stream reduces them in block
+// order.  The kernel boundary makes the partials visible; no fences or
+// atomics are needed (a device-scope fence per block forces an L2 write-back
+// on every XCD and costs far more than the extra launch).  The result is
+// deterministic for a given grid size.
 template <int NV>
-__device__ bool grid_reduce(double (&v)[NV], const int (&ops)[NV], double* partials,
-                            unsigned int* counter, double (&out)[NV]) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], const int (&ops)[NV], double* partials) {
   __shared__ double sh[NV][kWaves];
-  __shared__ int last;
   double bv[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) bv[q] = block_reduce_rt(ops[q], v[q], sh[q]);
-  const int nb = gridDim.x;
-  const int lb = blockIdx.x;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) partials[(int64_t)q * nb + lb] = bv[q];
-    __threadfence();
-    unsigned int t = atomicAdd(counter, 1u);
-    last = (t == (unsigned int)(nb - 1));
+    for (int q = 0; q < NV; ++q) partials[(int64_t)q * gridDim.x + blockIdx.x] = bv[q];
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double acc[NV];
+}
+
+// In a one-block kernel: reduce nb partials of each of NV values; out[] is
+// valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void finalize_partials(const double* partials, int nb, const int (&ops)[NV],
+                                                  double (&out)[NV]) {
+  __shared__ double sh[NV][kWaves];
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     double a = red_id_rt(ops[q]);
-    for (int b = threadIdx.x; b < nb; b += kBlock) {
-      double x = __hip_atomic_load(&partials[(int64_t)q * nb + b], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-      a = red_rt(ops[q], a, x);
-    }
-    acc[q] = a;
+    for (int b = threadIdx.x; b < nb; b += kBlock) a = red_rt(ops[q], a, partials[(int64_t)q * nb + b]);
+    out[q] = block_reduce_rt(ops[q], a, sh[q]);
   }
-#pragma unroll
-  for (int q = 0; q < NV; ++q) out[q] = block_reduce_rt(ops[q], acc[q], sh[q]);
-  if (threadIdx.x == 0) atomicExch(counter, 0u);
-  return true;
+}
+
+// Row kernels that reduce run on at most kMaxRedBlocks blocks; each block
+// sweeps a contiguous range of rows (a multiple of kBlock long), and the XCD
+// remap of logical_block() gives every XCD one contiguous eighth.
+constexpr int kMaxRedBlocks = 1024;
+
+inline int reduce_blocks(int64_t n) {
+  int64_t nb = (n + kBlock - 1) / kBlock;
+  if (nb > kMaxRedBlocks) nb = kMaxRedBlocks;
+  if (nb < 1) nb = 1;
+  nb = (nb + kXcd - 1) / kXcd * kXcd;
+  return (int)nb;
+}
+
+struct RowRange {
+  int64_t begin, end;
+};
+
+__device__ __forceinline__ RowRange block_rows(int64_t n) {
+  const int64_t nb = gridDim.x;
+  const int64_t per = ((n + nb - 1) / nb + kBlock - 1) / kBlock * kBlock;
+  const int64_t b = (int64_t)logical_block() * per;
+  RowRange r;
+  r.begin = b < n ? b : n;
+  r.end = (b + per) < n ? (b + per) : n;
+  return r;
 }
 
 }  // namespace ns3d

@@ -55,25 +55,54 @@ struct BState {
   long long i, nexec, maxit, restarts;
   int status;
   int pad;
-  unsigned int counter[8];
 };
-struct BWork {
-  double* r;
-  double* r0;
-  double* p[2];
-  double* v[2];
-  double* t;
-  double* partials;   // [4][max blocks]
+// Matrix view used by the solver, with coefficient storage type T (double
+// for the plain path, float for the inner solves of the mixed form).
+template <typename T>
+struct SMat {
+  int n;
+  const int32_t* nbr;   // [6][n]
+  const T* coef;        // [7][n]
+  const T* invdiag;     // [n]
+};
+// BiCGSTAB vectors of storage type T; scalars and reductions are FP64.
+template <typename T>
+struct SWork {
+  T* r;
+  T* r0;
+  T* p[2];
+  T* v[2];
+  T* t;
+  double* partials;   // [values][blocks] of the last row kernel
   BState* st;
 };
-// Launchers take the matrix view S (coef/nbr/invdiag) and the work buffers.
-hipError_t launch_bicg_init(SysView S, const double* x, const double* b, BWork w, double tol,
-                            long long maxit, hipStream_t s);
-hipError_t launch_bicg_restart(SysView S, const double* x, const double* b, BWork w, hipStream_t s);
-hipError_t launch_bicg_A(SysView S, BWork w, hipStream_t s);
-hipError_t launch_bicg_B(SysView S, BWork w, hipStream_t s);
-hipError_t launch_bicg_C(SysView S, double* x, BWork w, hipStream_t s);
+using BWork = SWork<double>;
+
+inline SMat<double> mat64(const SysView& S) { return SMat<double>{S.n, S.nbr, S.coef, S.invdiag}; }
+
+template <typename T>
+hipError_t launch_bicg_init(SMat<T> S, const T* x, const T* b, SWork<T> w, double tol, long long maxit,
+                            hipStream_t s);
+template <typename T>
+hipError_t launch_bicg_restart(SMat<T> S, const T* x, const T* b, SWork<T> w, hipStream_t s);
+// mid (may be null) is recorded between the row kernel and its finalize kernel.
+template <typename T>
+hipError_t launch_bicg_A(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t mid = nullptr);
+template <typename T>
+hipError_t launch_bicg_B(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t mid = nullptr);
+template <typename T>
+hipError_t launch_bicg_C(SMat<T> S, T* x, SWork<T> w, hipStream_t s, hipEvent_t mid = nullptr);
 hipError_t launch_invdiag(SysView S, hipStream_t s);
+
+// Mixed form: FP32 copy of a system's coefficients and preconditioner.
+hipError_t launch_coef_to_f32(const double* coef, const double* invdiag, int64_t n, float* coef32,
+                              float* invdiag32, hipStream_t s);
+// r = b - A x in FP64, r32 = (float) r; result[0] = |r|^2, result[1] = |b|^2.
+hipError_t launch_residual64(SMat<double> A, const double* x, const double* b, double* r, float* r32,
+                             double* partials, double* result, hipStream_t s);
+// x += (double) d
+hipError_t launch_add_f32(double* x, const float* d, int64_t n, hipStream_t s);
+hipError_t launch_fill_f32(float* x, int64_t n, hipStream_t s);
 
 // ---- fields, corrections, diagnostics (kernels_field.hip) -------------------
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);
@@ -85,11 +114,10 @@ hipError_t launch_correct_velocity(const Geo& g, SysView P, SysView U, SysView V
                                    const double* vh, const double* wh, double dvmax, double p_0, double p_1,
                                    double dx, double dy, double dz, hipStream_t s);
 hipError_t launch_correct_pressure(double* p, const double* pc, int64_t n, double beta, double* partials,
-                                   unsigned int* counter, double* result, hipStream_t s);
+                                   double* result, hipStream_t s);
 hipError_t launch_compute_div(const Geo& g, SysView P, const double* u, const double* v, const double* w,
-                              double dx, double dy, double dz, double* partials, unsigned int* counter,
-                              double* result, hipStream_t s);
+                              double dx, double dy, double dz, double* partials, double* result, hipStream_t s);
 hipError_t launch_permeability(const Geo& g, const double* u, const double* p, int border, double* partials,
-                               unsigned int* counter, double* result, hipStream_t s);
+                               double* result, hipStream_t s);
 
 }  // namespace ns3d

@@ -93,39 +93,41 @@ correct_kernel(Geo g, SysView P, SysView U, SysView V, SysView W, int kind, cons
 // CorrectPressureField (NSBase.cxx:2367-2380): max, min and squared norm of
 // pc, then p += beta*pc.  result: max, min, sum of squares.
 __global__ void __launch_bounds__(kBlock)
-correct_p_kernel(double* __restrict__ p, const double* __restrict__ pc, int64_t n, double beta,
-                 double* partials, unsigned int* counter, double* result) {
-  const int64_t i = (int64_t)logical_block() * kBlock + threadIdx.x;
+correct_p_kernel(double* __restrict__ p, const double* __restrict__ pc, int64_t n, double beta, double* partials) {
+  const RowRange rr = block_rows(n);
   double v[3] = {-__builtin_huge_val(), __builtin_huge_val(), 0.0};
-  if (i < n) {
+  for (int64_t i = rr.begin + threadIdx.x; i < rr.end; i += kBlock) {
     const double x = pc[i];
-    v[0] = x;
-    v[1] = x;
-    v[2] = x * x;
+    v[0] = x > v[0] ? x : v[0];
+    v[1] = x < v[1] ? x : v[1];
+    v[2] += x * x;
     p[i] += beta * x;
   }
   const int ops[3] = {RMAX, RMIN, RSUM};
+  block_partials<3>(v, ops, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) correct_p_fin(const double* partials, int nb, double* result) {
+  const int ops[3] = {RMAX, RMIN, RSUM};
   double out[3];
-  if (grid_reduce<3>(v, ops, partials, counter, out) && threadIdx.x == 0) {
-    result[0] = out[0];
-    result[1] = out[1];
-    result[2] = out[2];
-  }
+  finalize_partials<3>(partials, nb, ops, out);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; ++q) result[q] = out[q];
 }
 
 // ComputeDiv (NSBase.cxx:1958-2080).  result: max |sum of outward face
 // velocities|, sum div^2, sum |cell velocity|, max |div|.
 __global__ void __launch_bounds__(kBlock)
 div_kernel(Geo g, SysView P, const double* __restrict__ u, const double* __restrict__ v,
-           const double* __restrict__ w, double dx, double dy, double dz, double* partials,
-           unsigned int* counter, double* result) {
-  const int C = logical_block() * kBlock + threadIdx.x;
+           const double* __restrict__ w, double dx, double dy, double dz, double* partials) {
+  const int64_t n = P.n;
+  const RowRange rr = block_rows(n);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
-  if (C < P.n) {
+  const double idx = 1 / dx, idy = 1 / dy, idz = 1 / dz;
+  for (int64_t C64 = rr.begin + threadIdx.x; C64 < rr.end; C64 += kBlock) {
+    const int C = (int)C64;
     int i, j, k;
     g.decode(P.cell[C], i, j, k);
-    const int64_t n = P.n;
-    const double idx = 1 / dx, idy = 1 / dy, idz = 1 / dz;
     double div = 0, uu = 0, vv = 0, ww = 0, DV = 0;
     int f;
     if (i > 0 && P.nbr[DW * n + C] != C && (f = g.U(i, j, k)) > -1) { div += -u[f] * idx; DV += -u[f]; uu += u[f] * 0.5; }
@@ -138,46 +140,55 @@ div_kernel(Geo g, SysView P, const double* __restrict__ u, const double* __restr
     if (i == g.nx - 1 && (f = g.U(i + 1, j, k)) > -1) { div += u[f] * idx; DV += u[f]; uu += u[f] * 0.5; }
     DV = fabs(DV);
     div = fabs(div);
-    red[0] = DV;
-    red[1] = div * div;
-    red[2] = sqrt(uu * uu + vv * vv + ww * ww);
-    red[3] = div;
+    red[0] = DV > red[0] ? DV : red[0];
+    red[1] += div * div;
+    red[2] += sqrt(uu * uu + vv * vv + ww * ww);
+    red[3] = div > red[3] ? div : red[3];
   }
   const int ops[4] = {RMAX, RSUM, RSUM, RMAX};
-  double out[4];
-  if (grid_reduce<4>(red, ops, partials, counter, out) && threadIdx.x == 0) {
-    for (int q = 0; q < 4; ++q) result[q] = out[q];
-  }
+  block_partials<4>(red, ops, partials);
 }
 
-// ComputePermeability sums (NSBase.cxx:1886-1956), one thread per (j, k) of
-// the outlet plane.  result: sum of outlet u, sum of p on plane nx-border and
-// its node count, sum of p on plane border-1 and its node count.
+__global__ void __launch_bounds__(kBlock) div_fin(const double* partials, int nb, double* result) {
+  const int ops[4] = {RMAX, RSUM, RSUM, RMAX};
+  double out[4];
+  finalize_partials<4>(partials, nb, ops, out);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 4; ++q) result[q] = out[q];
+}
+
+// ComputePermeability sums (NSBase.cxx:1886-1956), over the (j, k) points
+// of the outlet plane.  result: sum of outlet u, sum of p on plane nx-border
+// and its node count, sum of p on plane border-1 and its node count.
 __global__ void __launch_bounds__(kBlock)
-perm_kernel(Geo g, const double* __restrict__ u, const double* __restrict__ p, int border, double* partials,
-            unsigned int* counter, double* result) {
-  const int64_t q = (int64_t)logical_block() * kBlock + threadIdx.x;
-  double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+perm_kernel(Geo g, const double* __restrict__ u, const double* __restrict__ p, int border, double* partials) {
   const int64_t m = (int64_t)g.ny * g.nz;
-  if (q < m) {
+  const RowRange rr = block_rows(m);
+  double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int64_t q = rr.begin + threadIdx.x; q < rr.end; q += kBlock) {
     const int j = (int)(q / g.nz), k = (int)(q % g.nz);
     if (border <= 0) {
       const int f = g.U(g.nx, j, k);
-      if (f > -1) red[0] = u[f];
+      if (f > -1) red[0] += u[f];
     } else {
       const int f = g.U(g.nx + 1 - border, j, k);
-      if (f > -1) red[0] = u[f];
+      if (f > -1) red[0] += u[f];
       int c = g.P(g.nx - border, j, k);
-      if (c > -1) { red[1] = p[c]; red[2] = 1.0; }
+      if (c > -1) { red[1] += p[c]; red[2] += 1.0; }
       c = g.P(border - 1, j, k);
-      if (c > -1) { red[3] = p[c]; red[4] = 1.0; }
+      if (c > -1) { red[3] += p[c]; red[4] += 1.0; }
     }
   }
   const int ops[5] = {RSUM, RSUM, RSUM, RSUM, RSUM};
+  block_partials<5>(red, ops, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) perm_fin(const double* partials, int nb, double* result) {
+  const int ops[5] = {RSUM, RSUM, RSUM, RSUM, RSUM};
   double out[5];
-  if (grid_reduce<5>(red, ops, partials, counter, out) && threadIdx.x == 0) {
+  finalize_partials<5>(partials, nb, ops, out);
+  if (threadIdx.x == 0)
     for (int t = 0; t < 5; ++t) result[t] = out[t];
-  }
 }
 
 }  // namespace
@@ -214,25 +225,27 @@ hipError_t launch_correct_velocity(const Geo& g, SysView P, SysView U, SysView V
 }
 
 hipError_t launch_correct_pressure(double* p, const double* pc, int64_t n, double beta, double* partials,
-                                   unsigned int* counter, double* result, hipStream_t s) {
-  hipLaunchKernelGGL(correct_p_kernel, dim3(padded_blocks(n)), dim3(kBlock), 0, s, p, pc, n, beta, partials,
-                     counter, result);
+                                   double* result, hipStream_t s) {
+  const int nb = reduce_blocks(n);
+  hipLaunchKernelGGL(correct_p_kernel, dim3(nb), dim3(kBlock), 0, s, p, pc, n, beta, partials);
+  hipLaunchKernelGGL(correct_p_fin, dim3(1), dim3(kBlock), 0, s, partials, nb, result);
   return hipGetLastError();
 }
 
 hipError_t launch_compute_div(const Geo& g, SysView P, const double* u, const double* v, const double* w,
-                              double dx, double dy, double dz, double* partials, unsigned int* counter,
-                              double* result, hipStream_t s) {
-  hipLaunchKernelGGL(div_kernel, dim3(padded_blocks(P.n)), dim3(kBlock), 0, s, g, P, u, v, w, dx, dy, dz,
-                     partials, counter, result);
+                              double dx, double dy, double dz, double* partials, double* result, hipStream_t s) {
+  const int nb = reduce_blocks(P.n);
+  hipLaunchKernelGGL(div_kernel, dim3(nb), dim3(kBlock), 0, s, g, P, u, v, w, dx, dy, dz, partials);
+  hipLaunchKernelGGL(div_fin, dim3(1), dim3(kBlock), 0, s, partials, nb, result);
   return hipGetLastError();
 }
 
 hipError_t launch_permeability(const Geo& g, const double* u, const double* p, int border, double* partials,
-                               unsigned int* counter, double* result, hipStream_t s) {
+                               double* result, hipStream_t s) {
   const int64_t m = (int64_t)g.ny * g.nz;
-  hipLaunchKernelGGL(perm_kernel, dim3(padded_blocks(m)), dim3(kBlock), 0, s, g, u, p, border, partials,
-                     counter, result);
+  const int nb = reduce_blocks(m);
+  hipLaunchKernelGGL(perm_kernel, dim3(nb), dim3(kBlock), 0, s, g, u, p, border, partials);
+  hipLaunchKernelGGL(perm_fin, dim3(1), dim3(kBlock), 0, s, partials, nb, result);
   return hipGetLastError();
 }
 

@@ -30,7 +30,7 @@ EXPORTS = [
     "ns3d_compute_velocity_field", "ns3d_compute_hat", "ns3d_assemble_pressure",
     "ns3d_compute_pressure_correction", "ns3d_compute_pressure_field", "ns3d_correct_velocity",
     "ns3d_correct_pressure", "ns3d_compute_div", "ns3d_compute_permeability", "ns3d_get_matrix",
-    "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_sync",
+    "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_set_precision", "ns3d_sync",
 ]
 
 
@@ -39,7 +39,8 @@ class NS3DError(RuntimeError):
 
 
 class SolveInfo(C.Structure):
-    _fields_ = [("iterations", C.c_int64), ("info", C.c_int32), ("restarts", C.c_int32), ("error", C.c_double)]
+    _fields_ = [("iterations", C.c_int64), ("info", C.c_int32), ("restarts", C.c_int32), ("error", C.c_double),
+                ("work", C.c_int64)]
 
     def as_tuple(self):
         return (int(self.iterations), int(self.info), float(self.error))
@@ -97,6 +98,7 @@ def lib():
         L.ns3d_timing.argtypes = [vp, i32]
         L.ns3d_timing_read.argtypes = [vp, P(dbl)]
         L.ns3d_matrix_nnz.argtypes = [vp, P(i64)]
+        L.ns3d_set_precision.argtypes = [vp, i32, dbl]
         L.ns3d_sync.argtypes = [vp]
         _lib = L
     return _lib
@@ -264,14 +266,19 @@ class NavierStokesGPU:
     def timing_read(self):
         """{(kind, kernel): (ms, launches, bytes)} for kind in momentum/pressure
         and kernel in A/B/C of the fused BiCGSTAB."""
-        out = np.zeros(18)
+        out = np.zeros(40)
         _check(lib().ns3d_timing_read(self.h, _p(out, C.c_double)))
         res = {}
-        for m, kind in enumerate(("momentum", "pressure")):
-            for k, name in enumerate("ABC"):
-                o = 9 * m + 3 * k
-                res[(kind, name)] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
+        for q, prec in enumerate(("f64", "f32")):
+            for m, kind in enumerate(("momentum", "pressure")):
+                for k, name in enumerate("ABC"):
+                    o = 18 * q + 9 * m + 3 * k
+                    res[(prec, kind, name)] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
+                res[(prec, kind, "fin")] = (float(out[36 + 2 * q + m]), 0.0, 0.0)
         return res
+
+    def set_precision(self, mixed: bool, inner_tol: float = 1e-4):
+        _check(lib().ns3d_set_precision(self.h, int(bool(mixed)), inner_tol))
 
     def nnz(self):
         out = np.zeros(4, np.int64)

@@ -24,6 +24,15 @@ void NavierStokesBase::say(const char* fmt, ...) const {
   va_end(ap);
 }
 
+void NavierStokesBase::note(const char* fmt, ...) const {
+  if (!progress) return;
+  va_list ap;
+  va_start(ap, fmt);
+  vfprintf(stderr, fmt, ap);
+  va_end(ap);
+  fflush(stderr);
+}
+
 void NavierStokesBase::tick(Phase p) {
   const auto t = std::chrono::steady_clock::now();
   phase[p] += std::chrono::duration<double>(t - t0_).count();
@@ -37,7 +46,7 @@ NavierStokesBase::NavierStokesBase(const std::vector<uint8_t>& raw, int nx_, int
                                    double dx_, double dy_, double dz_, double vsolvertol_, double psolvertol_,
                                    double pcsolvertol_, int maxitsolver_, double alpha_, double beta_, double p0_,
                                    double gradp_, double mu_, double rho_, int porevalue_, int direction_,
-                                   double deltaRMR_, int save_step_, int border_, int device, bool verbose_,
+                                   double deltaRMR_, int save_step_, int border_, int device, int verbose_,
                                    bool write_files_, const std::string& outdir_)
     : maxitsolver(maxitsolver_), porevalue(porevalue_), gradient_direction(direction_), save_step(save_step_),
       border(border_), dx(dx_), dy(dy_), dz(dz_), gradp(gradp_), alpha0(alpha_), beta0(beta_), alpha(alpha_),
@@ -45,7 +54,7 @@ NavierStokesBase::NavierStokesBase(const std::vector<uint8_t>& raw, int nx_, int
       psolvertol(psolvertol_), pcsolvertol(pcsolvertol_), deltaRMR(deltaRMR_), nx(0), ny(0), nz(0), np(0), nu(0),
       nv(0), nw(0), schemefactor(0), pcmax(0), pcmin(0), pcnorm(0), min_alpha(0.001), min_beta(0.001),
       relax(0.05), maxconvdelta(2.5), RMR(0), old_RMR(0), outdir(outdir_), pfieldfilename(pfield),
-      vxfilename(vxf), vyfilename(vyf), vzfilename(vzf), verbose(verbose_), write_files(write_files_) {
+      vxfilename(vxf), vyfilename(vyf), vzfilename(vzf), verbose(verbose_ == 1), progress(verbose_ >= 2), write_files(write_files_) {
   say("Low Reynolds Steady Navier-Stokes simulation (MI355X)\n");
   say("--- Input parameters ---\n");
   say("Image %s\n", imagename.c_str());

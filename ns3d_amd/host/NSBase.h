@@ -25,6 +25,7 @@ struct SolverReport {
   int info_ = 0;
   double err = 0;
   int64_t total = 0;  // summed iterations over the run
+  int64_t work = 0;   // summed executed iterations (zero right-hand sides do none)
   int64_t iterations() const { return it; }
   int info() const { return info_; }
   double error() const { return err; }
@@ -33,6 +34,7 @@ struct SolverReport {
     info_ = s.info;
     err = s.error;
     total += s.iterations;
+    work += s.work;
   }
 };
 
@@ -57,7 +59,8 @@ class NavierStokesBase {
   std::string outfilename, info_filename, log_filename, outdir;
   std::string pfieldfilename, vxfilename, vyfilename, vzfilename;
   std::ofstream info_file, log_file;
-  bool verbose = true;
+  bool verbose = true;        // reference-style progress on stdout
+  bool progress = false;      // one line per outer iteration on stderr
   bool write_files = true;
   ns3d_ctx* ctx = nullptr;
   double phase[PH_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -68,7 +71,7 @@ class NavierStokesBase {
                    const std::string& vzf, double dx, double dy, double dz, double vsolvertol, double psolvertol,
                    double pcsolvertol, int maxitsolver, double alpha, double beta, double p0, double gradp,
                    double mu, double rho, int porevalue, int gradient_direction, double deltaRMR, int save_step,
-                   int border, int device, bool verbose, bool write_files, const std::string& outdir);
+                   int border, int device, int verbose, bool write_files, const std::string& outdir);
   ~NavierStokesBase();
   NavierStokesBase(const NavierStokesBase&) = delete;
   NavierStokesBase& operator=(const NavierStokesBase&) = delete;
@@ -98,6 +101,7 @@ class NavierStokesBase {
   // fields on the node sets (host copies)
   std::vector<double> vector(int which) const;
   void say(const char* fmt, ...) const;
+  void note(const char* fmt, ...) const;
 
  private:
   std::chrono::steady_clock::time_point t0_;

@@ -39,8 +39,9 @@ int ns3d_solver_create(const ns3d_run_params* p, const uint8_t* img, int nx, int
     s->base = std::make_unique<NavierStokesBase>(
         raw, nx, ny, nz, p->name ? p->name : "image.vti", "none", "none", "none", "none", p->dx, dy, dz,
         p->vsolvertol, p->psolvertol, p->pcsolvertol, p->maxitsolver, p->alpha, p->beta, p->p0, p->gradp, p->mu,
-        p->rho, p->porevalue, p->direction, p->deltaRMR, p->save_step, p->border, p->device, p->verbose != 0,
+        p->rho, p->porevalue, p->direction, p->deltaRMR, p->save_step, p->border, p->device, p->verbose,
         p->write_files != 0, p->outdir ? p->outdir : "");
+    if (p->precision != 0) check(ns3d_set_precision(s->base->ctx, p->precision, p->inner_tol), "ns3d_set_precision");
     *out = s.release();
     return 0;
   } catch (const std::exception& e) {
@@ -97,6 +98,10 @@ int ns3d_solver_run(ns3d_solver* s, double* rows, int max_rows, int* nrows, ns3d
       res->solver_iterations[1] = B.vsolver.total;
       res->solver_iterations[2] = B.wsolver.total;
       res->solver_iterations[3] = B.psolver.total;
+      res->solver_work[0] = B.usolver.work;
+      res->solver_work[1] = B.vsolver.work;
+      res->solver_work[2] = B.wsolver.work;
+      res->solver_work[3] = B.psolver.work;
       for (int q = 0; q < PH_COUNT; ++q) res->phase_seconds[q] = B.phase[q];
     }
     return 0;

@@ -84,6 +84,9 @@ extern "C" void ns3d_default_run_params(ns3d_run_params* p) {
   p->device = -1;
   p->outdir = nullptr;
   p->name = nullptr;
+  p->precision = 0;
+  p->reserved = 0;
+  p->inner_tol = 1e-4;
 }
 
 extern "C" int ns3d_read_ini(const char* path, ns3d_run_params* p, char* image, int cap) {

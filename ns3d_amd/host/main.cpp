@@ -77,7 +77,7 @@ int main(int argc, char* argv[]) {
     NavierStokesBase base(raw, nx, ny, nz, fields[0], fields[1], fields[2], fields[3], fields[4], p.dx, p.dy, p.dz,
                           p.vsolvertol, p.psolvertol, p.pcsolvertol, p.maxitsolver, p.alpha, p.beta, p.p0,
                           p.gradp, p.mu, p.rho, p.porevalue, p.direction, p.deltaRMR, p.save_step, p.border,
-                          -1, true, true, "");
+                          -1, 1, true, "");
     if (p.method == NS3D_METHOD_IDEAL) {
       IDEALSOLVER solver(p.simpletol, p.maxitSIMPLE, p.N1, p.N2);
       solver.Solve(base);

@@ -29,7 +29,8 @@ class RunParams(C.Structure):
                 ("save_step", C.c_int32), ("porevalue", C.c_int32), ("direction", C.c_int32),
                 ("border", C.c_int32), ("method", C.c_int32), ("verbose", C.c_int32),
                 ("write_files", C.c_int32), ("device", C.c_int32), ("outdir", C.c_char_p),
-                ("name", C.c_char_p)]
+                ("name", C.c_char_p), ("precision", C.c_int32), ("reserved", C.c_int32),
+                ("inner_tol", C.c_double)]
 
 
 class RunResult(C.Structure):
@@ -37,7 +38,7 @@ class RunResult(C.Structure):
                 ("v_moy", C.c_double), ("RMR", C.c_double), ("normdiv", C.c_double), ("reldiv", C.c_double),
                 ("maxdiv", C.c_double), ("seconds", C.c_double), ("np", C.c_int64), ("nu", C.c_int64),
                 ("nv", C.c_int64), ("nw", C.c_int64), ("solver_iterations", C.c_int64 * 4),
-                ("phase_seconds", C.c_double * 8)]
+                ("phase_seconds", C.c_double * 8), ("solver_work", C.c_int64 * 4)]
 
 
 PHASES = ["momentum_assembly", "velocity_solves", "pseudo_velocities", "pressure_assembly",
@@ -83,6 +84,8 @@ def default_params(**kw) -> RunParams:
     for k, v in kw.items():
         if k == "method":
             v = METHODS[v.upper()] if isinstance(v, str) else int(v)
+        if k == "precision":
+            v = {"fp64": 0, "mixed": 1}[v] if isinstance(v, str) else int(v)
         if k in ("outdir", "name") and isinstance(v, str):
             v = v.encode()
         setattr(p, k, v)
@@ -116,8 +119,9 @@ class Solver:
         if rc != 0:
             raise _gpu.NS3DError(lib().ns3d_host_last_error().decode())
         summary = {f: getattr(res, f) for f, _ in RunResult._fields_
-                   if f not in ("solver_iterations", "phase_seconds")}
+                   if f not in ("solver_iterations", "phase_seconds", "solver_work")}
         summary["solver_iterations"] = list(res.solver_iterations)
+        summary["solver_work"] = list(res.solver_work)
         summary["phases"] = dict(zip(PHASES, list(res.phase_seconds)))
         return rows[:min(n.value, max_rows)].copy(), summary
 
@@ -129,13 +133,15 @@ class Solver:
         _gpu._check(_gpu.lib().ns3d_timing(self.context(), int(enable)))
 
     def timing_read(self):
-        out = np.zeros(18)
+        out = np.zeros(40)
         _gpu._check(_gpu.lib().ns3d_timing_read(self.context(), out.ctypes.data_as(C.POINTER(C.c_double))))
         res = {}
-        for m, kind in enumerate(("momentum", "pressure")):
-            for k, name in enumerate("ABC"):
-                o = 9 * m + 3 * k
-                res[(kind, name)] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
+        for q, prec in enumerate(("f64", "f32")):
+            for m, kind in enumerate(("momentum", "pressure")):
+                for k, name in enumerate("ABC"):
+                    o = 18 * q + 9 * m + 3 * k
+                    res[(prec, kind, name)] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
+                res[(prec, kind, "fin")] = (float(out[36 + 2 * q + m]), 0.0, 0.0)
         return res
 
     def nnz(self):

@@ -123,7 +123,8 @@ struct Solver {
   double tol = 2.220446049250313e-16;  // Eigen's default: epsilon
   long maxit = -1;
   long iterations = -1;  // -1: never solved (the reference reads garbage)
-  long total = 0;        // summed over the run
+  long total = 0;        // summed reported iterations over the run
+  long work = 0;         // summed executed iterations (zero right-hand sides do none)
   int info = 0;
   double error = 0.0;
 };
@@ -384,9 +385,13 @@ void bicgstab(const Mat& A, const std::vector<double>& invdiag, const std::vecto
 void solve_with_guess(Solver& s, const std::vector<double>& b, std::vector<double>& x) {
   long iters = s.maxit < 0 ? 2L * s.A->n : s.maxit;
   double err = s.tol;
+  double bb = 0.0;
+  for (double q : b) bb += q * q;
+  const bool zero_rhs = (bb == 0.0);
   bicgstab(*s.A, s.invdiag, b, x, iters, err);
   s.iterations = iters;
   s.total += iters;
+  if (!zero_rhs) s.work += iters;
   s.error = err;
   s.info = (err <= s.tol) ? 0 : 2;
 }
@@ -1152,6 +1157,7 @@ void run_driver(Ctx& c, int method, double simpletol, int maxit, int N1, int N2)
   c.us.maxit = c.vs.maxit = c.ws.maxit = c.maxitsolver;
   c.us.iterations = c.vs.iterations = c.ws.iterations = c.ps.iterations = -1;
   c.us.total = c.vs.total = c.ws.total = c.ps.total = 0;
+  c.us.work = c.vs.work = c.ws.work = c.ps.work = 0;
   c.alpha = c.alpha0;
   c.beta = c.beta0;
   c.avgRMR.clear();
@@ -1412,8 +1418,8 @@ void orc_spmv_ell(int n, const int32_t* cnt, const int32_t* col, const double* v
 
 // Full driver.  Returns the number of RES rows; rows_out (may be null) gets
 // 14 doubles per row: it Q v_moy RMR K gradP normdiv reldiv velocityNorm
-// maxdiv it_u it_v it_w it_pc.  summary[14]: final_it, final_K, seconds,
-// 7 phase timers, summed solver iterations u, v, w, p.
+// maxdiv it_u it_v it_w it_pc.  summary[15]: final_it, final_K, seconds,
+// 7 phase timers, summed solver iterations u, v, w, p, executed iterations.
 int orc_solve(void* h, int method, double simpletol, int maxit, int N1, int N2, double* rows_out,
               int max_rows, double* summary) {
   Ctx& c = *(Ctx*)h;
@@ -1435,6 +1441,7 @@ int orc_solve(void* h, int method, double simpletol, int maxit, int N1, int N2, 
     for (int q = 0; q < 7; ++q) summary[3 + q] = c.timers[q];
     summary[10] = (double)c.us.total; summary[11] = (double)c.vs.total;
     summary[12] = (double)c.ws.total; summary[13] = (double)c.ps.total;
+    summary[14] = (double)(c.us.work + c.vs.work + c.ws.work + c.ps.work);
   }
   return nr;
 }

@@ -215,11 +215,12 @@ class Oracle:
 
     def solve(self, method="SIMPLEC", simpletol=1e-4, maxit=500, N1=2, N2=2, max_rows=100000):
         rows = np.zeros((max_rows, 14))
-        summ = np.zeros(14)
+        summ = np.zeros(15)
         nr = lib().orc_solve(self.h, METHODS[method], simpletol, maxit, N1, N2, _p(rows, C.c_double),
                              max_rows, _p(summ, C.c_double))
         return rows[:nr].copy(), dict(iterations=int(summ[0]), K=float(summ[1]), seconds=float(summ[2]),
-                                      phases=summ[3:10].copy(), solver_iterations=summ[10:14].astype(int).tolist())
+                                      phases=summ[3:10].copy(), solver_iterations=summ[10:14].astype(int).tolist(),
+                                      work=int(summ[14]))
 
 
 def bicgstab_ell(cnt, col, val, b, x0, tol, maxit):
