@@ -74,9 +74,24 @@ def cpu_baseline(img, params, args, gpu_work):
                 sample_seconds=round(dt, 3))
 
 
+def measured_traffic(args, kernel):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
+    kernel) recorded with rocprofv3 on this workload, from
+    profiles/hbm_traffic_128rsa.json; None for another workload."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_128rsa.json")
+    default = (args.size, args.radius, args.porosity, args.seed) == (128, 10.0, 0.75, 20261203)
+    if not default or not os.path.exists(path):
+        return None, None
+    k = json.load(open(path))["kernels"].get(kernel)
+    if not k:
+        return None, None
+    return k["fetch_bytes"] + k["write_bytes"], "profiles/hbm_traffic_128rsa.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
+
+
 def kernel_table(solvers):
     """Per-kernel averages over the timed solves from the in-library HIP
-    events (recorded on the solver's own stream around each launch)."""
+    events taken from the dispatches of one sampled BiCGSTAB iteration per
+    queued chunk of iterations, on the solver's own stream."""
     tot = {}
     for s in solvers:
         for key, (ms, launches, nbytes) in s.timing_read().items():
@@ -84,10 +99,7 @@ def kernel_table(solvers):
             a[0] += ms; a[1] += launches; a[2] += nbytes
     kernels = {}
     for (prec, kind, k), (ms, launches, nbytes) in sorted(tot.items()):
-        if k == "fin":
-            if ms > 0:
-                kernels[f"{prec}_{kind}_finalize_ms_total"] = round(ms, 3)
-        elif launches:
+        if launches:
             kernels[f"{prec}_{kind}_{k}"] = dict(avg_us=round(1e3 * ms / launches, 3), launches=int(launches),
                                                  GBs=round(nbytes / (ms * 1e-3) / 1e9, 1),
                                                  frac=round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4))
@@ -130,6 +142,20 @@ def run_form(img, run_kw, args, precision, label):
                 p99=float(np.percentile(lat_sorted, 99)), summs=summs, tot=tot, kernels=kernels, **info)
 
 
+def copy_calibration(mib):
+    """A plain copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE
+    and WRITE_SIZE per dispatch give the factor from counter KiB to bytes."""
+    import torch
+    nbytes = mib << 20
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    src.fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": nbytes, "bytes_written": nbytes, "copies": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,7 +174,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events (profiler runs)")
     ap.add_argument("--progress", action="store_true", help="one stderr line per outer iteration")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (scales HBM byte counters under a profiler)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part of "
               f"this build", file=sys.stderr)
@@ -177,10 +208,12 @@ def main():
         ms, launches, nbytes = base["tot"][("f64", "pressure", "A")]
         if launches:
             achieved = nbytes / (ms * 1e-3) / 1e9
+            traffic, tsrc = measured_traffic(args, "bicg_A_kernel<double, 2>")
             roof = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                        frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None,
-                        kernel="bicg_A_kernel<double> (pressure system): p update + 7-point SpMV + r0.v",
-                        bytes_per_launch=round(nbytes / launches), avg_launch_us=round(1e3 * ms / launches, 3))
+                        frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic,
+                        kernel="bicg_A_kernel<double, 2> (pressure system): p update + 7-point SpMV + r0.v",
+                        bytes_per_launch=round(nbytes / launches), avg_launch_us=round(1e3 * ms / launches, 3),
+                        traffic_source=tsrc)
 
     forms_out = {}
     for f, r in res.items():

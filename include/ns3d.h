@@ -208,16 +208,18 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
  * relative residual.  This form is not part of the reference. */
 ns3d_status ns3d_set_precision(ns3d_ctx* ctx, int mode, double inner_tol);
 
-/* Device time of the fused BiCGSTAB kernels, measured with HIP events on
- * the context's stream around every launch (enable with ns3d_timing(ctx, 1);
- * launches after a solve has stopped are not counted).  out[40]: for storage
- * precision q (0 FP64, 1 FP32), system kind m (0 momentum, 1 pressure) and
- * kernel k (0: A = p update + SpMV + r0.v, 1: B = s + SpMV + t.s/t.t, 2: C =
- * x/r update + r0.r/r.r), out[18q + 9m + 3k] = milliseconds, launches,
- * algorithmic bytes of the row kernel; out[36 + 2q + m] = milliseconds in the
- * one-block finalize kernels. */
+/* Device time of the fused BiCGSTAB kernels (enable with ns3d_timing(ctx,
+ * 1)): one iteration per queued chunk of iterations is sampled, its three
+ * kernels launched with start/stop events taken from their own dispatches
+ * on the context's stream; samples of launches after a solve has stopped
+ * are dropped.  out[36]: for storage precision q (0 FP64, 1 FP32), system
+ * kind m (0 momentum, 1 pressure) and kernel k (0: A = p update + SpMV +
+ * r0.v, 1: B = s + SpMV + t.s/t.t, 2: C = x/r update + r0.r/r.r),
+ * out[18q + 9m + 3k] = milliseconds, sampled launches, algorithmic bytes
+ * (nnz (s + 4) per SpMV plus s bytes per row per vector stream: A 7, B 4,
+ * C 9; s = 8 or 4). */
 ns3d_status ns3d_timing(ns3d_ctx* ctx, int enable);
-ns3d_status ns3d_timing_read(ns3d_ctx* ctx, double out[40]);
+ns3d_status ns3d_timing_read(ns3d_ctx* ctx, double out[36]);
 
 /* Stored entries (diagonal + existing neighbours) of the u, v, w, p systems. */
 ns3d_status ns3d_matrix_nnz(ns3d_ctx* ctx, int64_t out[4]);

@@ -48,18 +48,21 @@ struct SysMem {
   int64_t nnz = 0;           // stored entries (diagonal + existing neighbours)
   float* coef32 = nullptr;   // mixed form: FP32 copies
   float* invdiag32 = nullptr;
+  uint64_t* pk = nullptr;    // packed neighbour distances (layout != ML_INDEXED)
+  int layout = ML_INDEXED;   // how the solver reads the matrix (MatLayout)
   SysView view() const { return SysView{n, d0, grid, cell, nbr, coef, invdiag, rowsum}; }
-  SMat<float> mat32() const { return SMat<float>{n, nbr, coef32, invdiag32}; }
+  SMat<double> mat64() const { return SMat<double>{n, layout, nbr, pk, coef, invdiag}; }
+  SMat<float> mat32() const { return SMat<float>{n, layout, nbr, pk, coef32, invdiag32}; }
 };
 
 // Device time of the fused BiCGSTAB kernels, per storage precision (0 FP64,
-// 1 FP32), system kind (0 momentum, 1 pressure) and kernel (A, B, C):
-// milliseconds, launches that did work, algorithmic bytes.
-constexpr int kTimedChunk = 64;
+// 1 FP32), system kind (0 momentum, 1 pressure) and kernel (A, B, C), over
+// the sampled iterations (one per queued chunk; start and stop are the
+// dispatch's own timestamps): milliseconds, sampled launches, algorithmic
+// bytes.
 struct Timing {
   bool on = false;
   double ms[2][2][3] = {}, launches[2][2][3] = {}, bytes[2][2][3] = {};
-  double fin_ms[2][2] = {};  // one-block finalize kernels, including launch gaps
 };
 
 }  // namespace
@@ -96,7 +99,7 @@ struct ns3d_ctx {
   std::vector<void*> allocs;
   Timing tm;
   hipEvent_t ev[2] = {nullptr, nullptr};
-  std::vector<hipEvent_t> tev;  // 4 per timed iteration
+  std::vector<hipEvent_t> tev;  // start / stop of A, B, C of the sampled iteration
 
   Geo geo() const { return Geo{nx, ny, nz, pg, ug, vg, wg}; }
 
@@ -197,53 +200,36 @@ void bicg_loop(ns3d_ctx* c, const SysMem& S, SMat<T> M, T* x, const T* b, SWork<
   int64_t chunk = hint > 4 ? (hint * 3) / 4 : 4;
   int64_t launched = 0;
   const int64_t cap = 4 * (maxit > 0 ? maxit : 1) + 64;
-  long long exec_before = 0;
   const int prec = sizeof(T) == 4 ? 1 : 0;
   while (c->hstate->status == BS_RUNNING || c->hstate->status == BS_RESTART) {
     if (c->hstate->status == BS_RESTART) HCHK(launch_bicg_restart<T>(M, x, b, w, s));
     if (chunk > maxit + 1) chunk = maxit + 1;
-    if (c->tm.on) {
-      // events around every kernel, read back after the chunk's sync,
-      // counting only the iterations that did work
-      if (chunk > kTimedChunk) chunk = kTimedChunk;
-      for (int64_t q = 0; q < chunk; ++q) {
-        hipEvent_t* e = &c->tev[7 * q];
-        HCHK(hipEventRecord(e[0], s));
-        HCHK(launch_bicg_A<T>(M, w, s, e[1]));
-        HCHK(hipEventRecord(e[2], s));
-        HCHK(launch_bicg_B<T>(M, w, s, e[3]));
-        HCHK(hipEventRecord(e[4], s));
-        HCHK(launch_bicg_C<T>(M, x, w, s, e[5]));
-        HCHK(hipEventRecord(e[6], s));
-      }
-      HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
-      HCHK(hipStreamSynchronize(s));
-      const long long did = c->hstate->nexec - exec_before;
-      exec_before = c->hstate->nexec;
-      for (long long q = 0; q < did && q < chunk; ++q)
-        for (int k = 0; k < 3; ++k) {
-          float ms = 0.f, mf = 0.f;
-          HCHK(hipEventElapsedTime(&ms, c->tev[7 * q + 2 * k], c->tev[7 * q + 2 * k + 1]));
-          HCHK(hipEventElapsedTime(&mf, c->tev[7 * q + 2 * k + 1], c->tev[7 * q + 2 * k + 2]));
-          c->tm.ms[prec][kind][k] += ms;
-          c->tm.fin_ms[prec][kind] += mf;
-          c->tm.launches[prec][kind][k] += 1;
-          c->tm.bytes[prec][kind][k] += kernel_bytes(S.n, S.nnz, k, (int)sizeof(T));
-        }
-    } else {
-      for (int64_t q = 0; q < chunk; ++q) {
-        HCHK(launch_bicg_A<T>(M, w, s));
-        HCHK(launch_bicg_B<T>(M, w, s));
-        HCHK(launch_bicg_C<T>(M, x, w, s));
-      }
-      HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
-      HCHK(hipStreamSynchronize(s));
+    const long long done_before = c->hstate->it_c;
+    // with timing on, the kernels of one iteration in the middle of the
+    // chunk carry start/stop events, read back after the chunk's sync if
+    // that iteration did work
+    const int64_t qt = c->tm.on ? chunk / 2 : -1;
+    for (int64_t q = 0; q < chunk; ++q) {
+      hipEvent_t* e = (q == qt) ? c->tev.data() : nullptr;
+      HCHK(launch_bicg_A<T>(M, w, s, e));
+      HCHK(launch_bicg_B<T>(M, w, s, e ? e + 2 : nullptr));
+      HCHK(launch_bicg_C<T>(M, x, w, s, e ? e + 4 : nullptr));
     }
+    HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    if (qt >= 0 && c->hstate->it_c - done_before > qt)
+      for (int k = 0; k < 3; ++k) {
+        float ms = 0.f;
+        HCHK(hipEventElapsedTime(&ms, c->tev[2 * k], c->tev[2 * k + 1]));
+        c->tm.ms[prec][kind][k] += ms;
+        c->tm.launches[prec][kind][k] += 1;
+        c->tm.bytes[prec][kind][k] += kernel_bytes(S.n, S.nnz, k, (int)sizeof(T));
+      }
     launched += chunk;
     if (launched > cap) throw Invalid{"BiCGSTAB did not stop within its iteration cap"};
     chunk = 16;
   }
-  if (c->hstate->status != BS_ZERO_RHS) hint = c->hstate->nexec > 0 ? c->hstate->nexec : 4;
+  if (c->hstate->status != BS_ZERO_RHS) hint = c->hstate->it_c > 0 ? c->hstate->it_c : 4;
 }
 
 // One BiCGSTAB solve on system sid with x as guess, Eigen semantics (FP64).
@@ -253,7 +239,7 @@ void bicgstab64(ns3d_ctx* c, int sid, double* x, const double* b, double tol, in
     info->iterations = 0; info->info = 0; info->restarts = 0; info->error = 0; info->work = 0;
     return;
   }
-  bicg_loop<double>(c, S, mat64(S.view()), x, b, c->bw, tol, maxit, S.last_iters, sid == 3 ? 1 : 0);
+  bicg_loop<double>(c, S, S.mat64(), x, b, c->bw, tol, maxit, S.last_iters, sid == 3 ? 1 : 0);
   const BState& st = *c->hstate;
   if (st.status == BS_ZERO_RHS) {
     HCHK(launch_fill(x, S.n, 0.0, c->stream));
@@ -263,8 +249,8 @@ void bicgstab64(ns3d_ctx* c, int sid, double* x, const double* b, double tol, in
     info->restarts = 0;
     info->work = 0;
   } else {
-    info->iterations = st.i;
-    info->work = st.i;
+    info->iterations = st.it_c - st.ibase;
+    info->work = st.it_c;
     info->error = std::sqrt(st.rr / st.rhs_sqnorm);
     info->info = (info->error <= tol) ? 0 : 2;
     info->restarts = (int32_t)st.restarts;
@@ -287,7 +273,7 @@ void bicgstab_mixed(ns3d_ctx* c, int sid, double* x, const double* b, double tol
     info->iterations = 0; info->info = 0; info->error = 0;
     return;
   }
-  const SMat<double> M64 = mat64(S.view());
+  const SMat<double> M64 = S.mat64();
   const SMat<float> M32 = S.mat32();
   auto residual = [&](double& rr, double& bb) {
     HCHK(launch_residual64(M64, x, b, c->rres, c->r32, c->partials, c->result, s));
@@ -312,7 +298,7 @@ void bicgstab_mixed(ns3d_ctx* c, int sid, double* x, const double* b, double tol
     HCHK(launch_fill_f32(c->d32, S.n, s));
     bicg_loop<float>(c, S, M32, c->d32, c->r32, c->bw32, tin, maxit - total, S.last_iters32, sid == 3 ? 1 : 0);
     const BState& st = *c->hstate;
-    const int64_t k = (st.status == BS_ZERO_RHS) ? 0 : st.i;
+    const int64_t k = (st.status == BS_ZERO_RHS) ? 0 : st.it_c;
     info->restarts += (int32_t)st.restarts;
     total += k;
     ++outer;
@@ -446,6 +432,19 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
     HCHK(hipMemcpyAsync(hnnz, dnnz, sizeof(hnnz), hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
     for (int q = 0; q < 4; ++q) c->sys[q].nnz = (int64_t)hnnz[q];
+    // packed neighbour distances where every distance fits (always, for the
+    // monotone numbering of the index scan, unless a plane is very large)
+    int* dbad = c->alloc<int>(4);
+    HCHK(hipMemsetAsync(dbad, 0, 4 * sizeof(int), s));
+    for (int q = 0; q < 4; ++q) {
+      c->sys[q].pk = c->alloc<uint64_t>(c->sys[q].n);
+      HCHK(launch_pack_neighbours(c->sys[q].nbr, c->sys[q].n, c->sys[q].pk, dbad + q, s));
+    }
+    int hbad[4];
+    HCHK(hipMemcpyAsync(hbad, dbad, sizeof(hbad), hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    for (int q = 0; q < 4; ++q)
+      c->sys[q].layout = hbad[q] ? ML_INDEXED : (q == 3 ? ML_PACKED_SYM : ML_PACKED);
     for (int set = 0; set < 3; ++set)
       for (int comp = 0; comp < 3; ++comp) c->vel[set][comp] = c->alloc<double>(c->sys[comp].n);
     c->p = c->alloc<double>(np);
@@ -802,13 +801,13 @@ ns3d_status ns3d_timing(ns3d_ctx* c, int enable) {
     c->tm = Timing();
     c->tm.on = enable != 0;
     if (c->tm.on && c->tev.empty()) {
-      c->tev.resize(7 * kTimedChunk, nullptr);
+      c->tev.resize(6, nullptr);
       for (auto& e : c->tev) HCHK(hipEventCreate(&e));
     }
   });
 }
 
-ns3d_status ns3d_timing_read(ns3d_ctx* c, double out[40]) {
+ns3d_status ns3d_timing_read(ns3d_ctx* c, double out[36]) {
   if (!c || !out) return fail_invalid("ns3d_timing_read: bad arguments");
   for (int prec = 0; prec < 2; ++prec)
     for (int kind = 0; kind < 2; ++kind) {
@@ -818,7 +817,6 @@ ns3d_status ns3d_timing_read(ns3d_ctx* c, double out[40]) {
         o[1] = c->tm.launches[prec][kind][k];
         o[2] = c->tm.bytes[prec][kind][k];
       }
-      out[36 + prec * 2 + kind] = c->tm.fin_ms[prec][kind];
     }
   return NS3D_OK;
 }

@@ -173,10 +173,52 @@ __device__ __forceinline__ void finalize_partials(const double* partials, int nb
   }
 }
 
-// Row kernels that reduce run on at most kMaxRedBlocks blocks; each block
-// sweeps a contiguous range of rows (a multiple of kBlock long), and the XCD
-// remap of logical_block() gives every XCD one contiguous eighth.
-constexpr int kMaxRedBlocks = 1024;
+// In every block of a kernel: all threads get the NV sums of nb partials per
+// value (slot q at partials[q * nb + b]).  The order of the additions is
+// fixed and the same in every block, so every block computes the same
+// bits; this replaces a separate finalize launch between two row kernels.
+template <int NV>
+__device__ __forceinline__ void sum_partials_all(const double* __restrict__ partials, int nb, double (&out)[NV]) {
+  __shared__ double sh[NV][kWaves];
+  double a[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) a[q] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) a[q] += partials[(int64_t)q * nb + b];
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) a[q] = wave_reduce<RSUM>(a[q]);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sh[q][threadIdx.x >> 6] = a[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double r = sh[q][0];
+#pragma unroll
+    for (int wv = 1; wv < kWaves; ++wv) r += sh[q][wv];
+    out[q] = r;
+  }
+}
+
+// Row kernels that reduce run on at most kMaxRedBlocks blocks (a multiple of
+// 8, so the XCD remap of logical_block() gives each XCD the same number) and
+// sweep their rows with a grid-stride loop of a per-block RowIter.  Every XCD
+// owns one contiguous eighth of the rows, so neighbour gathers stay in that
+// XCD's L2.  Within an XCD (NS3D_ROW_MAP):
+//   0: each block owns one contiguous chunk (a multiple of kBlock rows);
+//   1: the XCD's blocks sweep its eighth together, block k taking the k-th
+//      kBlock-row tile of each window of (blocks per XCD) tiles, so rows
+//      visited at the same time are adjacent.
+#ifndef NS3D_ROW_MAP
+#define NS3D_ROW_MAP 0
+#endif
+#ifndef NS3D_MAX_RED_BLOCKS
+#define NS3D_MAX_RED_BLOCKS 1024
+#endif
+constexpr int kMaxRedBlocks = NS3D_MAX_RED_BLOCKS;
 
 inline int reduce_blocks(int64_t n) {
   int64_t nb = (n + kBlock - 1) / kBlock;
@@ -186,17 +228,29 @@ inline int reduce_blocks(int64_t n) {
   return (int)nb;
 }
 
-struct RowRange {
-  int64_t begin, end;
+struct RowIter {
+  int64_t begin, end, step;
 };
 
-__device__ __forceinline__ RowRange block_rows(int64_t n) {
+__device__ __forceinline__ RowIter block_rows(int64_t n) {
   const int64_t nb = gridDim.x;
+  RowIter r;
+#if NS3D_ROW_MAP == 1
+  const int64_t per_xcd = nb / kXcd;
+  const int64_t xcd = blockIdx.x % kXcd, k = blockIdx.x / kXcd;
+  const int64_t span = ((n + kXcd - 1) / kXcd + kBlock - 1) / kBlock * kBlock;
+  const int64_t xb = xcd * span;
+  const int64_t xe = xb + span < n ? xb + span : n;
+  r.begin = xb + k * kBlock < xe ? xb + k * kBlock : xe;
+  r.end = xe;
+  r.step = per_xcd * kBlock;
+#else
   const int64_t per = ((n + nb - 1) / nb + kBlock - 1) / kBlock * kBlock;
   const int64_t b = (int64_t)logical_block() * per;
-  RowRange r;
   r.begin = b < n ? b : n;
   r.end = (b + per) < n ? (b + per) : n;
+  r.step = kBlock;
+#endif
   return r;
 }
 

@@ -48,24 +48,48 @@ hipError_t launch_hat(int sys, const Geo& g, SysView S, const double* xc, const 
 
 // ---- BiCGSTAB (kernels_solver.hip) ------------------------------------------
 enum BStatus { BS_RUNNING = 0, BS_DONE = 1, BS_RESTART = 2, BS_ZERO_RHS = 3 };
+// Solver scalars in device memory.  The row kernels of one iteration never
+// read a field that their own blocks write: per-iteration scalars live in
+// two slots indexed by the parity of the executed-iteration count, it_a is
+// written by kernel A and read by B and C, it_c is written by C and read by
+// A.  Eigen's iteration counter is it_c - ibase (ibase moves on the first
+// restart, which resets Eigen's i to 0).
 struct BState {
-  double rho, rho_old, alpha, w;
+  double rho[2], alpha[2], w[2];
+  double rho_fresh;     // rho of iteration fresh_at, set by init / restart
   double r0_sqnorm, rhs_sqnorm, tol2, tol;
   double rr, r0v, ts, tt;
-  long long i, nexec, maxit, restarts;
+  long long it_a, it_c, ibase, fresh_at, maxit, restarts;
   int status;
   int pad;
 };
+// Matrix storage as the solver reads it.
+//   ML_INDEXED: nbr[6][n] int32 neighbour rows, coef[7][n];
+//   ML_PACKED: one 64-bit word of neighbour distances per row (see
+//     pack_neighbours), coef[7][n];
+//   ML_PACKED_SYM: packed distances, and only the diagonal and the E, N, T
+//     coefficients are read: the W/S/B coefficient of a row is the E/N/T
+//     coefficient of that neighbour (bit-symmetric pressure matrices).
+enum MatLayout { ML_INDEXED = 0, ML_PACKED = 1, ML_PACKED_SYM = 2 };
+// Packed neighbour distances: T, B in 2 bits each (1, or 2 next to the
+// outlet u plane), N, S in 11 bits, E, W in 19 bits; 0 = no neighbour.
+constexpr int kPkTB = 2, kPkNS = 11, kPkEW = 19;
+
 // Matrix view used by the solver, with coefficient storage type T (double
 // for the plain path, float for the inner solves of the mixed form).
 template <typename T>
 struct SMat {
   int n;
-  const int32_t* nbr;   // [6][n]
+  int layout;           // MatLayout
+  const int32_t* nbr;   // [6][n] (ML_INDEXED)
+  const uint64_t* pk;   // [n] (ML_PACKED*)
   const T* coef;        // [7][n]
   const T* invdiag;     // [n]
 };
 // BiCGSTAB vectors of storage type T; scalars and reductions are FP64.
+// partials: [kPartialSlots][kMaxRedBlocks] block partials (A: r0.v in slot
+// 0; B: t.s, t.t in 1-2; C: r0.r, r.r in 3-4; init / restart in 5-6).
+constexpr int kPartialSlots = 8;
 template <typename T>
 struct SWork {
   T* r;
@@ -73,26 +97,28 @@ struct SWork {
   T* p[2];
   T* v[2];
   T* t;
-  double* partials;   // [values][blocks] of the last row kernel
+  double* partials;
   BState* st;
 };
 using BWork = SWork<double>;
-
-inline SMat<double> mat64(const SysView& S) { return SMat<double>{S.n, S.nbr, S.coef, S.invdiag}; }
 
 template <typename T>
 hipError_t launch_bicg_init(SMat<T> S, const T* x, const T* b, SWork<T> w, double tol, long long maxit,
                             hipStream_t s);
 template <typename T>
 hipError_t launch_bicg_restart(SMat<T> S, const T* x, const T* b, SWork<T> w, hipStream_t s);
-// mid (may be null) is recorded between the row kernel and its finalize kernel.
+// ev (may be null): two events per kernel, {start, stop} taken from the
+// kernel's own dispatch (hipExtLaunchKernelGGL).
 template <typename T>
-hipError_t launch_bicg_A(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t mid = nullptr);
+hipError_t launch_bicg_A(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t* ev = nullptr);
 template <typename T>
-hipError_t launch_bicg_B(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t mid = nullptr);
+hipError_t launch_bicg_B(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t* ev = nullptr);
 template <typename T>
-hipError_t launch_bicg_C(SMat<T> S, T* x, SWork<T> w, hipStream_t s, hipEvent_t mid = nullptr);
+hipError_t launch_bicg_C(SMat<T> S, T* x, SWork<T> w, hipStream_t s, hipEvent_t* ev = nullptr);
 hipError_t launch_invdiag(SysView S, hipStream_t s);
+// Packs nbr[6][n] into pk[n]; *bad (zeroed by the caller) becomes nonzero if
+// a distance does not fit or a neighbour is not on its expected side.
+hipError_t launch_pack_neighbours(const int32_t* nbr, int n, uint64_t* pk, int* bad, hipStream_t s);
 
 // Mixed form: FP32 copy of a system's coefficients and preconditioner.
 hipError_t launch_coef_to_f32(const double* coef, const double* invdiag, int64_t n, float* coef32,

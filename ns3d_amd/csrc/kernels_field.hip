@@ -94,9 +94,9 @@ correct_kernel(Geo g, SysView P, SysView U, SysView V, SysView W, int kind, cons
 // pc, then p += beta*pc.  result: max, min, sum of squares.
 __global__ void __launch_bounds__(kBlock)
 correct_p_kernel(double* __restrict__ p, const double* __restrict__ pc, int64_t n, double beta, double* partials) {
-  const RowRange rr = block_rows(n);
+  const RowIter rr = block_rows(n);
   double v[3] = {-__builtin_huge_val(), __builtin_huge_val(), 0.0};
-  for (int64_t i = rr.begin + threadIdx.x; i < rr.end; i += kBlock) {
+  for (int64_t i = rr.begin + threadIdx.x; i < rr.end; i += rr.step) {
     const double x = pc[i];
     v[0] = x > v[0] ? x : v[0];
     v[1] = x < v[1] ? x : v[1];
@@ -121,10 +121,10 @@ __global__ void __launch_bounds__(kBlock)
 div_kernel(Geo g, SysView P, const double* __restrict__ u, const double* __restrict__ v,
            const double* __restrict__ w, double dx, double dy, double dz, double* partials) {
   const int64_t n = P.n;
-  const RowRange rr = block_rows(n);
+  const RowIter rr = block_rows(n);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
   const double idx = 1 / dx, idy = 1 / dy, idz = 1 / dz;
-  for (int64_t C64 = rr.begin + threadIdx.x; C64 < rr.end; C64 += kBlock) {
+  for (int64_t C64 = rr.begin + threadIdx.x; C64 < rr.end; C64 += rr.step) {
     const int C = (int)C64;
     int i, j, k;
     g.decode(P.cell[C], i, j, k);
@@ -163,9 +163,9 @@ __global__ void __launch_bounds__(kBlock) div_fin(const double* partials, int nb
 __global__ void __launch_bounds__(kBlock)
 perm_kernel(Geo g, const double* __restrict__ u, const double* __restrict__ p, int border, double* partials) {
   const int64_t m = (int64_t)g.ny * g.nz;
-  const RowRange rr = block_rows(m);
+  const RowIter rr = block_rows(m);
   double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int64_t q = rr.begin + threadIdx.x; q < rr.end; q += kBlock) {
+  for (int64_t q = rr.begin + threadIdx.x; q < rr.end; q += rr.step) {
     const int j = (int)(q / g.nz), k = (int)(q % g.nz);
     if (border <= 0) {
       const int f = g.U(g.nx, j, k);

@@ -11,22 +11,28 @@
 //     v = A y; alpha = rho / r0.v; s = r - alpha v; z = M^-1 s; t = A z
 //     w = t.s / t.t (0 if t.t == 0); x += alpha y + w z; r = s - w t; ++i
 //
-// One iteration is three kernels, each ending in one grid-wide reduction:
-//   A: p' = r + beta (p - w v) and y = M^-1 p' are recomputed on the fly at
-//      the row and its six neighbours, v' = A y, partial r0.v'
-//   B: s = r - alpha v' and z = M^-1 s on the fly, t = A z, partials t.s, t.t
-//   C: x += alpha M^-1 p' + w M^-1 s, r = s - w t, partials r0.r, r.r
+// One iteration is three row kernels; each writes block partials of its dot
+// products and the next one starts by summing them (every block sums the
+// same partials in the same order, so all blocks hold identical scalars):
+//   A: sum C's partials (r0.r, r.r): loop test, restart test, rho, beta;
+//      p' = r + beta (p - w v) and y = M^-1 p' recomputed at the row and its
+//      six neighbours, v' = A y, partial r0.v'
+//   B: sum A's partials: alpha; s = r - alpha v' and z = M^-1 s on the fly,
+//      t = A z, partials t.s, t.t
+//   C: sum B's partials: w; x += alpha M^-1 p' + w M^-1 s, r = s - w t,
+//      partials r0.r, r.r
 // p and v are double-buffered (buffer parity = executed iterations), so the
 // recomputation at neighbours never reads a value written in the same kernel.
+// Block (0, thread 0) of each kernel records the scalars in BState for the
+// host, which polls the status between chunks of queued iterations; kernels
+// launched after the solve has stopped exit at once.  init and restart keep
+// a one-block finalize kernel (once per solve / rare).
 // Vectors and coefficients are stored as T (double for the plain path, float
 // for the inner solves of the mixed form); every operation is evaluated in
 // FP64 registers and reductions accumulate in FP64, so the FP64
 // instantiation computes exactly the expressions above.
-// All scalars live in device memory (BState).  Each row kernel writes block
-// partials and is followed by a one-block finalize kernel that reduces them
-// and advances the scalars, so the host only polls the status between chunks
-// of queued iterations.  Kernels launched after the solve has stopped exit at
-// once.
+#include <hip/hip_ext.h>
+
 #include "kernels.h"
 
 namespace ns3d {
@@ -34,49 +40,140 @@ namespace ns3d {
 namespace {
 
 constexpr double kEps = 2.220446049250313e-16;  // NumTraits<double>::epsilon()
+constexpr int kSlotA = 0, kSlotB = 1, kSlotC = 3, kSlotI = 5;
 
-// SpMV row in ascending-column order for monotone numbering:
-// W, S, B, diagonal, T, N, E (i-1 < j-1 < k-1 < self < k+1 < j+1 < i+1).
+
+__device__ __forceinline__ double* slot(double* partials, int q) { return partials + (int64_t)q * kMaxRedBlocks; }
+__device__ __forceinline__ bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+// The seven points of a row in SpMV order: W, S, B, the row itself, T, N,
+// E -- ascending column order for monotone numbering (i-1 < j-1 < k-1 <
+// self < k+1 < j+1 < i+1).  An absent neighbour is the row itself with a
+// zero coefficient.  Coefficients are kept in their storage type.
+template <typename T>
+struct RowPts {
+  int q[7];
+  T c[7];
+};
+
+template <typename T, int L>
+__device__ __forceinline__ void row_pts(const SMat<T>& S, int row, int64_t n, RowPts<T>& P) {
+  int nW, nS, nB, nT, nN, nE;
+  bool hW = true, hS = true, hB = true;
+  if constexpr (L == ML_INDEXED) {
+    nW = S.nbr[DW * n + row]; nS = S.nbr[DS * n + row]; nB = S.nbr[DB * n + row];
+    nT = S.nbr[DT * n + row]; nN = S.nbr[DN * n + row]; nE = S.nbr[DE * n + row];
+  } else {
+    const uint64_t pw = S.pk[row];
+    const int dT = (int)(pw & 3u), dB = (int)((pw >> 2) & 3u);
+    const int dN = (int)((pw >> 4) & 0x7ffu), dS = (int)((pw >> 15) & 0x7ffu);
+    const int dE = (int)((pw >> 26) & 0x7ffffu), dW = (int)((pw >> 45) & 0x7ffffu);
+    nT = row + dT; nB = row - dB; nN = row + dN; nS = row - dS; nE = row + dE; nW = row - dW;
+    hW = dW != 0; hS = dS != 0; hB = dB != 0;
+  }
+  P.q[0] = nW; P.q[1] = nS; P.q[2] = nB; P.q[3] = row; P.q[4] = nT; P.q[5] = nN; P.q[6] = nE;
+  P.c[3] = S.coef[row];
+  P.c[6] = S.coef[(1 + DE) * n + row];
+  P.c[5] = S.coef[(1 + DN) * n + row];
+  P.c[4] = S.coef[(1 + DT) * n + row];
+  if constexpr (L == ML_PACKED_SYM) {
+    const T gW = S.coef[(1 + DE) * n + nW];
+    const T gS = S.coef[(1 + DN) * n + nS];
+    const T gB = S.coef[(1 + DT) * n + nB];
+    P.c[0] = hW ? gW : (T)0;
+    P.c[1] = hS ? gS : (T)0;
+    P.c[2] = hB ? gB : (T)0;
+  } else {
+    P.c[0] = S.coef[(1 + DW) * n + row];
+    P.c[1] = S.coef[(1 + DS) * n + row];
+    P.c[2] = S.coef[(1 + DB) * n + row];
+  }
+}
+
+// sum over the points in order of c_k * y(k), in FP64
 template <typename T, typename F>
-__device__ __forceinline__ double stencil_row(const SMat<T>& S, int row, int64_t n, F&& y) {
-  const int nW = S.nbr[DW * n + row], nS = S.nbr[DS * n + row], nB = S.nbr[DB * n + row];
-  const int nT = S.nbr[DT * n + row], nN = S.nbr[DN * n + row], nE = S.nbr[DE * n + row];
-  const double cC = S.coef[row];
-  const double cE = S.coef[(1 + DE) * n + row], cW = S.coef[(1 + DW) * n + row];
-  const double cN = S.coef[(1 + DN) * n + row], cS = S.coef[(1 + DS) * n + row];
-  const double cT = S.coef[(1 + DT) * n + row], cB = S.coef[(1 + DB) * n + row];
-  double t = cW * y(nW);
-  t += cS * y(nS);
-  t += cB * y(nB);
-  t += cC * y(row);
-  t += cT * y(nT);
-  t += cN * y(nN);
-  t += cE * y(nE);
+__device__ __forceinline__ double spmv_pts(const RowPts<T>& P, F&& y) {
+  double t = (double)P.c[0] * y(0);
+#pragma unroll
+  for (int k = 1; k < 7; ++k) t += (double)P.c[k] * y(k);
   return t;
 }
 
-// After |r|^2 and r0.r of the new residual are known: loop test, then
-// rho_old = rho, rho = r0.r and the restart test.
-__device__ __forceinline__ void loop_head(BState* st, double rr, double r0r) {
-  st->rr = rr;
-  if (!(rr > st->tol2 && st->i < st->maxit)) {
-    st->status = BS_DONE;
-    return;
+template <typename T, int L, typename F>
+__device__ __forceinline__ double stencil_row(const SMat<T>& S, int row, int64_t n, F&& x) {
+  RowPts<T> P;
+  row_pts<T, L>(S, row, n, P);
+  return spmv_pts(P, [&](int k) { return x(P.q[k]); });
+}
+
+// Loads of one row of kernel A (everything that does not depend on the
+// iteration's scalars): the points, r, p, v and M^-1 at each point, r0.
+template <typename T>
+struct AIn {
+  RowPts<T> P;
+  T r[7], p[7], v[7], m[7];
+  T r0;
+};
+template <typename T, int L>
+__device__ __forceinline__ void a_fetch(const SMat<T>& S, const SWork<T>& w, const T* __restrict__ pold,
+                                        const T* __restrict__ vold, int row, int64_t n, AIn<T>& f) {
+  row_pts<T, L>(S, row, n, f.P);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const int q = f.P.q[k];
+    f.r[k] = w.r[q];
+    f.p[k] = pold[q];
+    f.v[k] = vold[q];
+    f.m[k] = S.invdiag[q];
   }
-  st->rho_old = st->rho;
-  st->rho = r0r;
-  if (fabs(st->rho) < kEps * kEps * st->r0_sqnorm) st->status = BS_RESTART;
+  f.r0 = w.r0[row];
+}
+
+// Loads of one row of kernel B: the points, r, v' and M^-1 at each point.
+template <typename T>
+struct BIn {
+  RowPts<T> P;
+  T r[7], v[7], m[7];
+};
+template <typename T, int L>
+__device__ __forceinline__ void b_fetch(const SMat<T>& S, const SWork<T>& w, const T* __restrict__ vn, int row,
+                                        int64_t n, BIn<T>& f) {
+  row_pts<T, L>(S, row, n, f.P);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const int q = f.P.q[k];
+    f.r[k] = w.r[q];
+    f.v[k] = vn[q];
+    f.m[k] = S.invdiag[q];
+  }
+}
+
+// Loads of one row of kernel C.
+template <typename T>
+struct CIn {
+  T m, p, r, v, t, x, r0;
+};
+template <typename T>
+__device__ __forceinline__ void c_fetch(const T* __restrict__ invdiag, const T* __restrict__ x, const SWork<T>& w,
+                                        int par, int64_t row, CIn<T>& f) {
+  f.m = invdiag[row];
+  f.p = w.p[par][row];
+  f.r = w.r[row];
+  f.v = w.v[par][row];
+  f.t = w.t[row];
+  f.x = x[row];
+  f.r0 = w.r0[row];
 }
 
 // r = b - A x, r0 = r; zero p, v of the first buffer; partials |r|^2, |b|^2.
-template <typename T>
+template <typename T, int L>
 __global__ void __launch_bounds__(kBlock)
 bicg_init_kernel(SMat<T> S, const T* __restrict__ x, const T* __restrict__ b, SWork<T> w) {
   const int64_t n = S.n;
-  const RowRange rr = block_rows(n);
+  const RowIter rr = block_rows(n);
   double v2[2] = {0.0, 0.0};
-  for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += kBlock) {
-    const double ax = stencil_row(S, (int)row, n, [&](int q) { return (double)x[q]; });
+  for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += rr.step) {
+    const double ax = stencil_row<T, L>(S, (int)row, n, [&](int q) { return (double)x[q]; });
     const double bv = b[row];
     const double r = bv - ax;
     w.r[row] = (T)r;
@@ -88,44 +185,47 @@ bicg_init_kernel(SMat<T> S, const T* __restrict__ x, const T* __restrict__ b, SW
     v2[1] += bv * bv;
   }
   const int ops[2] = {RSUM, RSUM};
-  block_partials<2>(v2, ops, w.partials);
+  block_partials<2>(v2, ops, slot(w.partials, kSlotI));
 }
 
 __global__ void __launch_bounds__(kBlock)
 bicg_init_fin(double* partials, BState* st, int nb, double tol, long long maxit) {
   const int ops[2] = {RSUM, RSUM};
   double out[2];
-  finalize_partials<2>(partials, nb, ops, out);
+  finalize_partials<2>(slot(partials, kSlotI), nb, ops, out);
   if (threadIdx.x != 0) return;
   st->tol = tol;
   st->maxit = maxit;
   st->r0_sqnorm = out[0];
   st->rhs_sqnorm = out[1];
-  st->i = 0;
-  st->nexec = 0;
+  st->it_a = 0;
+  st->it_c = 0;
+  st->ibase = 0;
   st->restarts = 0;
   st->rr = out[0];
   if (out[1] == 0) {
     st->status = BS_ZERO_RHS;
     return;
   }
-  st->status = BS_RUNNING;
   st->tol2 = tol * tol * out[1];
-  st->rho = 1;
-  st->alpha = 1;
-  st->w = 1;
-  loop_head(st, out[0], out[0]);
+  // the slot of iteration -1 holds Eigen's initial rho = alpha = w = 1
+  st->rho[1] = 1;
+  st->alpha[1] = 1;
+  st->w[1] = 1;
+  st->rho_fresh = out[0];  // rho = r0.r = |r0|^2 (no restart possible at i = 0)
+  st->fresh_at = 0;
+  st->status = (out[0] > st->tol2 && 0 < maxit) ? BS_RUNNING : BS_DONE;
 }
 
-template <typename T>
+template <typename T, int L>
 __global__ void __launch_bounds__(kBlock)
 bicg_restart_kernel(SMat<T> S, const T* __restrict__ x, const T* __restrict__ b, SWork<T> w) {
   if (w.st->status != BS_RESTART) return;
   const int64_t n = S.n;
-  const RowRange rr = block_rows(n);
+  const RowIter rr = block_rows(n);
   double v1[1] = {0.0};
-  for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += kBlock) {
-    const double ax = stencil_row(S, (int)row, n, [&](int q) { return (double)x[q]; });
+  for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += rr.step) {
+    const double ax = stencil_row<T, L>(S, (int)row, n, [&](int q) { return (double)x[q]; });
     const double r = (double)b[row] - ax;
     w.r[row] = (T)r;
     w.r0[row] = (T)r;
@@ -133,7 +233,7 @@ bicg_restart_kernel(SMat<T> S, const T* __restrict__ x, const T* __restrict__ b,
     v1[0] += rs * rs;
   }
   const int ops[1] = {RSUM};
-  block_partials<1>(v1, ops, w.partials);
+  block_partials<1>(v1, ops, slot(w.partials, kSlotI));
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -141,129 +241,175 @@ bicg_restart_fin(double* partials, BState* st, int nb) {
   if (st->status != BS_RESTART) return;
   const int ops[1] = {RSUM};
   double out[1];
-  finalize_partials<1>(partials, nb, ops, out);
+  finalize_partials<1>(slot(partials, kSlotI), nb, ops, out);
   if (threadIdx.x != 0) return;
-  st->rho = out[0];
+  // the iteration that found the restart is re-run with the new residual;
+  // rho_old stays the pre-restart rho (slot of the previous iteration)
+  st->rho_fresh = out[0];
   st->r0_sqnorm = out[0];
-  if (st->restarts++ == 0) st->i = 0;
+  st->fresh_at = st->it_c;
+  if (st->restarts++ == 0) st->ibase = st->it_c;
   st->status = BS_RUNNING;
 }
 
-template <typename T>
+// The first row of every thread is fetched before the block sums the
+// previous kernel's partials, so the two latencies overlap.  With
+// NS3D_SOLVE_PIPE=1 each later row is also fetched one step ahead (the next
+// row's loads in flight while the current one is computed); that doubles the
+// registers of A and B and measured no faster on MI355X at 128^3, so it is
+// off by default.
+#ifndef NS3D_SOLVE_PIPE
+#define NS3D_SOLVE_PIPE 0
+#endif
+
+template <typename T, int L>
 __global__ void __launch_bounds__(kBlock)
 bicg_A_kernel(SMat<T> S, SWork<T> w) {
-  const BState* st = w.st;
+  BState* st = w.st;
   if (st->status != BS_RUNNING) return;
-  const double beta = (st->rho / st->rho_old) * (st->alpha / st->w);
-  const double om = st->w;
-  const int par = (int)(st->nexec & 1);
-  const T* __restrict__ r = w.r;
+  const long long it = st->it_c;
+  const int sl = (int)(it & 1);
+  const int par = sl;
   const T* __restrict__ pold = w.p[par];
   const T* __restrict__ vold = w.v[par];
   T* __restrict__ pnew = w.p[par ^ 1];
   T* __restrict__ vnew = w.v[par ^ 1];
-  const T* __restrict__ md = S.invdiag;
   const int64_t n = S.n;
-  const RowRange rr = block_rows(n);
-  double v1[1] = {0.0};
-  for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += kBlock) {
-    auto pn = [&](int q) {
-      return (double)(T)((double)r[q] + beta * ((double)pold[q] - om * (double)vold[q]));
-    };
-    auto y = [&](int q) { return (double)md[q] * pn(q); };
-    const double vn = stencil_row(S, (int)row, n, y);
-    pnew[row] = (T)pn((int)row);
-    vnew[row] = (T)vn;
-    v1[0] += (double)w.r0[row] * (double)(T)vn;
+  const RowIter rr = block_rows(n);
+  int64_t row = rr.begin + threadIdx.x;
+  AIn<T> cur;
+  if (row < rr.end) a_fetch<T, L>(S, w, pold, vold, (int)row, n, cur);
+  double rho;
+  if (st->fresh_at == it) {
+    rho = st->rho_fresh;
+  } else {
+    double red[2];
+    sum_partials_all<2>(slot(w.partials, kSlotC), gridDim.x, red);
+    const double r0r = red[0], rr2 = red[1];
+    if (!(rr2 > st->tol2 && it - st->ibase < st->maxit)) {
+      if (leader()) { st->rr = rr2; st->status = BS_DONE; }
+      return;
+    }
+    if (fabs(r0r) < kEps * kEps * st->r0_sqnorm) {
+      if (leader()) { st->rr = rr2; st->status = BS_RESTART; }
+      return;
+    }
+    rho = r0r;
+    if (leader()) st->rr = rr2;
   }
+  const double beta = (rho / st->rho[sl ^ 1]) * (st->alpha[sl ^ 1] / st->w[sl ^ 1]);
+  const double om = st->w[sl ^ 1];
+  if (leader()) {
+    st->rho[sl] = rho;
+    st->it_a = it;
+  }
+  double acc = 0.0;
+  while (row < rr.end) {
+    const int64_t nxt = row + rr.step;
+    AIn<T> nx;
+    if (NS3D_SOLVE_PIPE && nxt < rr.end) a_fetch<T, L>(S, w, pold, vold, (int)nxt, n, nx);
+    double pn[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+      pn[k] = (double)(T)((double)cur.r[k] + beta * ((double)cur.p[k] - om * (double)cur.v[k]));
+    const double vn = spmv_pts(cur.P, [&](int k) { return (double)cur.m[k] * pn[k]; });
+    pnew[row] = (T)pn[3];
+    vnew[row] = (T)vn;
+    acc += (double)cur.r0 * (double)(T)vn;
+    if (!NS3D_SOLVE_PIPE && nxt < rr.end) a_fetch<T, L>(S, w, pold, vold, (int)nxt, n, nx);
+    cur = nx;
+    row = nxt;
+  }
+  double v1[1] = {acc};
   const int ops[1] = {RSUM};
-  block_partials<1>(v1, ops, w.partials);
+  block_partials<1>(v1, ops, slot(w.partials, kSlotA));
 }
 
-__global__ void __launch_bounds__(kBlock)
-bicg_A_fin(double* partials, BState* st, int nb) {
-  if (st->status != BS_RUNNING) return;
-  const int ops[1] = {RSUM};
-  double out[1];
-  finalize_partials<1>(partials, nb, ops, out);
-  if (threadIdx.x != 0) return;
-  st->r0v = out[0];
-  st->alpha = st->rho / out[0];
-}
-
-template <typename T>
+template <typename T, int L>
 __global__ void __launch_bounds__(kBlock)
 bicg_B_kernel(SMat<T> S, SWork<T> w) {
-  const BState* st = w.st;
+  BState* st = w.st;
   if (st->status != BS_RUNNING) return;
-  const double alpha = st->alpha;
-  const int par = (int)(st->nexec & 1) ^ 1;
-  const T* __restrict__ r = w.r;
+  const long long it = st->it_a;
+  const int sl = (int)(it & 1);
+  const int par = sl ^ 1;
   const T* __restrict__ vn = w.v[par];
-  const T* __restrict__ md = S.invdiag;
   const int64_t n = S.n;
-  const RowRange rr = block_rows(n);
-  double v2[2] = {0.0, 0.0};
-  for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += kBlock) {
-    auto sv = [&](int q) { return (double)(T)((double)r[q] - alpha * (double)vn[q]); };
-    auto z = [&](int q) { return (double)md[q] * sv(q); };
-    const double tv = (double)(T)stencil_row(S, (int)row, n, z);
-    const double s = sv((int)row);
-    w.t[row] = (T)tv;
-    v2[0] += tv * s;
-    v2[1] += tv * tv;
+  const RowIter rr = block_rows(n);
+  int64_t row = rr.begin + threadIdx.x;
+  BIn<T> cur;
+  if (row < rr.end) b_fetch<T, L>(S, w, vn, (int)row, n, cur);
+  double red[1];
+  sum_partials_all<1>(slot(w.partials, kSlotA), gridDim.x, red);
+  const double alpha = st->rho[sl] / red[0];
+  if (leader()) {
+    st->r0v = red[0];
+    st->alpha[sl] = alpha;
   }
+  double a0 = 0.0, a1 = 0.0;
+  while (row < rr.end) {
+    const int64_t nxt = row + rr.step;
+    BIn<T> nx;
+    if (NS3D_SOLVE_PIPE && nxt < rr.end) b_fetch<T, L>(S, w, vn, (int)nxt, n, nx);
+    double sv[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) sv[k] = (double)(T)((double)cur.r[k] - alpha * (double)cur.v[k]);
+    const double tv = (double)(T)spmv_pts(cur.P, [&](int k) { return (double)cur.m[k] * sv[k]; });
+    w.t[row] = (T)tv;
+    a0 += tv * sv[3];
+    a1 += tv * tv;
+    if (!NS3D_SOLVE_PIPE && nxt < rr.end) b_fetch<T, L>(S, w, vn, (int)nxt, n, nx);
+    cur = nx;
+    row = nxt;
+  }
+  double v2[2] = {a0, a1};
   const int ops[2] = {RSUM, RSUM};
-  block_partials<2>(v2, ops, w.partials);
-}
-
-__global__ void __launch_bounds__(kBlock)
-bicg_B_fin(double* partials, BState* st, int nb) {
-  if (st->status != BS_RUNNING) return;
-  const int ops[2] = {RSUM, RSUM};
-  double out[2];
-  finalize_partials<2>(partials, nb, ops, out);
-  if (threadIdx.x != 0) return;
-  st->ts = out[0];
-  st->tt = out[1];
-  st->w = (out[1] > 0.0) ? out[0] / out[1] : 0.0;
+  block_partials<2>(v2, ops, slot(w.partials, kSlotB));
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-bicg_C_kernel(SMat<T> S, T* __restrict__ x, SWork<T> w) {
-  const BState* st = w.st;
+bicg_C_kernel(int n_, T* __restrict__ x, const T* __restrict__ invdiag, SWork<T> w) {
+  BState* st = w.st;
   if (st->status != BS_RUNNING) return;
-  const double alpha = st->alpha, om = st->w;
-  const int par = (int)(st->nexec & 1) ^ 1;
-  const int64_t n = S.n;
-  const RowRange rr = block_rows(n);
-  double v2[2] = {0.0, 0.0};
-  for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += kBlock) {
-    const double md = S.invdiag[row];
-    const double y = md * (double)w.p[par][row];
-    const double s = (double)(T)((double)w.r[row] - alpha * (double)w.v[par][row]);
-    const double z = md * s;
-    x[row] = (T)((double)x[row] + (alpha * y + om * z));
-    const double rn = (double)(T)(s - om * (double)w.t[row]);
-    w.r[row] = (T)rn;
-    v2[0] += (double)w.r0[row] * rn;
-    v2[1] += rn * rn;
+  const long long it = st->it_a;
+  const int sl = (int)(it & 1);
+  const int par = sl ^ 1;
+  const int64_t n = n_;
+  const RowIter rr = block_rows(n);
+  int64_t row = rr.begin + threadIdx.x;
+  CIn<T> cur;
+  if (row < rr.end) c_fetch(invdiag, x, w, par, row, cur);
+  double red[2];
+  sum_partials_all<2>(slot(w.partials, kSlotB), gridDim.x, red);
+  const double om = (red[1] > 0.0) ? red[0] / red[1] : 0.0;
+  const double alpha = st->alpha[sl];
+  if (leader()) {
+    st->ts = red[0];
+    st->tt = red[1];
+    st->w[sl] = om;
+    st->it_c = it + 1;
   }
+  double a0 = 0.0, a1 = 0.0;
+  while (row < rr.end) {
+    const int64_t nxt = row + rr.step;
+    CIn<T> nx;
+    if (nxt < rr.end) c_fetch(invdiag, x, w, par, nxt, nx);
+    const double md = cur.m;
+    const double y = md * (double)cur.p;
+    const double s = (double)(T)((double)cur.r - alpha * (double)cur.v);
+    const double z = md * s;
+    x[row] = (T)((double)cur.x + (alpha * y + om * z));
+    const double rn = (double)(T)(s - om * (double)cur.t);
+    w.r[row] = (T)rn;
+    a0 += (double)cur.r0 * rn;
+    a1 += rn * rn;
+    cur = nx;
+    row = nxt;
+  }
+  double v2[2] = {a0, a1};
   const int ops[2] = {RSUM, RSUM};
-  block_partials<2>(v2, ops, w.partials);
-}
-
-__global__ void __launch_bounds__(kBlock)
-bicg_C_fin(double* partials, BState* st, int nb) {
-  if (st->status != BS_RUNNING) return;
-  const int ops[2] = {RSUM, RSUM};
-  double out[2];
-  finalize_partials<2>(partials, nb, ops, out);
-  if (threadIdx.x != 0) return;
-  st->i += 1;
-  st->nexec += 1;
-  loop_head(st, out[1], out[0]);
+  block_partials<2>(v2, ops, slot(w.partials, kSlotC));
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -272,6 +418,29 @@ invdiag_kernel(SysView S) {
   if (row >= S.n) return;
   const double d = S.coef[row];
   S.invdiag[row] = (d != 0.0) ? 1.0 / d : 1.0;
+}
+
+// One word per row: T | B << 2 | N << 4 | S << 15 | E << 26 | W << 45 (bit
+// widths kPkTB, kPkNS, kPkEW), each the distance from the row to that
+// neighbour (E, N, T above the row, W, S, B below; 0 when absent).
+__global__ void __launch_bounds__(kBlock)
+pack_neighbours_kernel(const int32_t* __restrict__ nbr, int n, uint64_t* __restrict__ pk, int* bad) {
+  const int row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= n) return;
+  const int64_t m = n;
+  const long long dE = (long long)nbr[DE * m + row] - row, dW = (long long)row - nbr[DW * m + row];
+  const long long dN = (long long)nbr[DN * m + row] - row, dS = (long long)row - nbr[DS * m + row];
+  const long long dT = (long long)nbr[DT * m + row] - row, dB = (long long)row - nbr[DB * m + row];
+  const long long lTB = (1ll << kPkTB), lNS = (1ll << kPkNS), lEW = (1ll << kPkEW);
+  const bool ok = dT >= 0 && dT < lTB && dB >= 0 && dB < lTB && dN >= 0 && dN < lNS && dS >= 0 && dS < lNS &&
+                  dE >= 0 && dE < lEW && dW >= 0 && dW < lEW;
+  if (!ok) {
+    atomicOr(bad, 1);
+    pk[row] = 0;
+    return;
+  }
+  pk[row] = (uint64_t)dT | ((uint64_t)dB << 2) | ((uint64_t)dN << 4) | ((uint64_t)dS << 15) |
+            ((uint64_t)dE << 26) | ((uint64_t)dW << 45);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -285,14 +454,15 @@ coef_to_f32_kernel(const double* __restrict__ coef, const double* __restrict__ i
 }
 
 // FP64 residual of the defect correction: r = b - A x, r32 = (float) r.
+template <int L>
 __global__ void __launch_bounds__(kBlock)
 residual64_kernel(SMat<double> A, const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
                   float* __restrict__ r32, double* partials) {
   const int64_t n = A.n;
-  const RowRange rr = block_rows(n);
+  const RowIter rr = block_rows(n);
   double v2[2] = {0.0, 0.0};
-  for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += kBlock) {
-    const double ax = stencil_row(A, (int)row, n, [&](int q) { return x[q]; });
+  for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += rr.step) {
+    const double ax = stencil_row<double, L>(A, (int)row, n, [&](int q) { return x[q]; });
     const double bv = b[row];
     const double rv = bv - ax;
     r[row] = rv;
@@ -326,15 +496,26 @@ __global__ void __launch_bounds__(kBlock) fill_f32_kernel(float* __restrict__ x,
   if (i < n) x[i] = 0.0f;
 }
 
-inline hipError_t record(hipEvent_t e, hipStream_t s) { return e ? hipEventRecord(e, s) : hipSuccess; }
-
 }  // namespace
+
+// Kernel instance for a layout; the layout is a template parameter so the
+// row loop carries no per-row branches on it.
+#define NS3D_BY_LAYOUT(L, KERNEL, T)                                                                       \
+  ((L) == ML_PACKED_SYM ? KERNEL<T, ML_PACKED_SYM> : (L) == ML_PACKED ? KERNEL<T, ML_PACKED> : KERNEL<T, ML_INDEXED>)
+
+// Launch with optional start / stop events taken from the dispatch itself.
+template <typename K, typename... Args>
+hipError_t launch_timed(K kernel, int nb, hipStream_t s, hipEvent_t* ev, Args... args) {
+  if (ev) hipExtLaunchKernelGGL(kernel, dim3(nb), dim3(kBlock), 0, s, ev[0], ev[1], 0, args...);
+  else hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlock), 0, s, args...);
+  return hipGetLastError();
+}
 
 template <typename T>
 hipError_t launch_bicg_init(SMat<T> S, const T* x, const T* b, SWork<T> w, double tol, long long maxit,
                             hipStream_t s) {
   const int nb = reduce_blocks(S.n);
-  hipLaunchKernelGGL(bicg_init_kernel<T>, dim3(nb), dim3(kBlock), 0, s, S, x, b, w);
+  hipLaunchKernelGGL(NS3D_BY_LAYOUT(S.layout, bicg_init_kernel, T), dim3(nb), dim3(kBlock), 0, s, S, x, b, w);
   hipLaunchKernelGGL(bicg_init_fin, dim3(1), dim3(kBlock), 0, s, w.partials, w.st, nb, tol, maxit);
   return hipGetLastError();
 }
@@ -342,48 +523,33 @@ hipError_t launch_bicg_init(SMat<T> S, const T* x, const T* b, SWork<T> w, doubl
 template <typename T>
 hipError_t launch_bicg_restart(SMat<T> S, const T* x, const T* b, SWork<T> w, hipStream_t s) {
   const int nb = reduce_blocks(S.n);
-  hipLaunchKernelGGL(bicg_restart_kernel<T>, dim3(nb), dim3(kBlock), 0, s, S, x, b, w);
+  hipLaunchKernelGGL(NS3D_BY_LAYOUT(S.layout, bicg_restart_kernel, T), dim3(nb), dim3(kBlock), 0, s, S, x, b, w);
   hipLaunchKernelGGL(bicg_restart_fin, dim3(1), dim3(kBlock), 0, s, w.partials, w.st, nb);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_bicg_A(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t mid) {
-  const int nb = reduce_blocks(S.n);
-  hipLaunchKernelGGL(bicg_A_kernel<T>, dim3(nb), dim3(kBlock), 0, s, S, w);
-  hipError_t e = record(mid, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(bicg_A_fin, dim3(1), dim3(kBlock), 0, s, w.partials, w.st, nb);
-  return hipGetLastError();
+hipError_t launch_bicg_A(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t* ev) {
+  return launch_timed(NS3D_BY_LAYOUT(S.layout, bicg_A_kernel, T), reduce_blocks(S.n), s, ev, S, w);
 }
 
 template <typename T>
-hipError_t launch_bicg_B(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t mid) {
-  const int nb = reduce_blocks(S.n);
-  hipLaunchKernelGGL(bicg_B_kernel<T>, dim3(nb), dim3(kBlock), 0, s, S, w);
-  hipError_t e = record(mid, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(bicg_B_fin, dim3(1), dim3(kBlock), 0, s, w.partials, w.st, nb);
-  return hipGetLastError();
+hipError_t launch_bicg_B(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t* ev) {
+  return launch_timed(NS3D_BY_LAYOUT(S.layout, bicg_B_kernel, T), reduce_blocks(S.n), s, ev, S, w);
 }
 
 template <typename T>
-hipError_t launch_bicg_C(SMat<T> S, T* x, SWork<T> w, hipStream_t s, hipEvent_t mid) {
-  const int nb = reduce_blocks(S.n);
-  hipLaunchKernelGGL(bicg_C_kernel<T>, dim3(nb), dim3(kBlock), 0, s, S, x, w);
-  hipError_t e = record(mid, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(bicg_C_fin, dim3(1), dim3(kBlock), 0, s, w.partials, w.st, nb);
-  return hipGetLastError();
+hipError_t launch_bicg_C(SMat<T> S, T* x, SWork<T> w, hipStream_t s, hipEvent_t* ev) {
+  return launch_timed(bicg_C_kernel<T>, reduce_blocks(S.n), s, ev, S.n, x, S.invdiag, w);
 }
 
 #define NS3D_INSTANTIATE(T)                                                                                 \
   template hipError_t launch_bicg_init<T>(SMat<T>, const T*, const T*, SWork<T>, double, long long,       \
                                           hipStream_t);                                                     \
   template hipError_t launch_bicg_restart<T>(SMat<T>, const T*, const T*, SWork<T>, hipStream_t);           \
-  template hipError_t launch_bicg_A<T>(SMat<T>, SWork<T>, hipStream_t, hipEvent_t);                        \
-  template hipError_t launch_bicg_B<T>(SMat<T>, SWork<T>, hipStream_t, hipEvent_t);                        \
-  template hipError_t launch_bicg_C<T>(SMat<T>, T*, SWork<T>, hipStream_t, hipEvent_t);
+  template hipError_t launch_bicg_A<T>(SMat<T>, SWork<T>, hipStream_t, hipEvent_t*);                       \
+  template hipError_t launch_bicg_B<T>(SMat<T>, SWork<T>, hipStream_t, hipEvent_t*);                       \
+  template hipError_t launch_bicg_C<T>(SMat<T>, T*, SWork<T>, hipStream_t, hipEvent_t*);
 NS3D_INSTANTIATE(double)
 NS3D_INSTANTIATE(float)
 #undef NS3D_INSTANTIATE
@@ -402,10 +568,19 @@ hipError_t launch_coef_to_f32(const double* coef, const double* invdiag, int64_t
   return hipGetLastError();
 }
 
+hipError_t launch_pack_neighbours(const int32_t* nbr, int n, uint64_t* pk, int* bad, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pack_neighbours_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nbr, n, pk, bad);
+  return hipGetLastError();
+}
+
 hipError_t launch_residual64(SMat<double> A, const double* x, const double* b, double* r, float* r32, double* partials,
                              double* result, hipStream_t s) {
   const int nb = reduce_blocks(A.n);
-  hipLaunchKernelGGL(residual64_kernel, dim3(nb), dim3(kBlock), 0, s, A, x, b, r, r32, partials);
+  auto k = A.layout == ML_PACKED_SYM ? residual64_kernel<ML_PACKED_SYM>
+         : A.layout == ML_PACKED     ? residual64_kernel<ML_PACKED>
+                                     : residual64_kernel<ML_INDEXED>;
+  hipLaunchKernelGGL(k, dim3(nb), dim3(kBlock), 0, s, A, x, b, r, r32, partials);
   hipLaunchKernelGGL(sums_fin, dim3(1), dim3(kBlock), 0, s, partials, nb, result);
   return hipGetLastError();
 }

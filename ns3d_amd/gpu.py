@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libns3d_gpu.so")
+# NS3D_GPU_LIBRARY may name another build of the library (e.g. one compiled
+# with different tuning macros); the default is the in-tree build.
+LIB_PATH = os.environ.get("NS3D_GPU_LIBRARY") or os.path.join(HERE, "libns3d_gpu.so")
 
 SYS_U, SYS_V, SYS_W, SYS_P = 0, 1, 2, 3
 SET_VEL, SET_TEMP, SET_HAT = 0, 1, 2
@@ -264,9 +266,10 @@ class NavierStokesGPU:
         _check(lib().ns3d_timing(self.h, int(enable)))
 
     def timing_read(self):
-        """{(kind, kernel): (ms, launches, bytes)} for kind in momentum/pressure
-        and kernel in A/B/C of the fused BiCGSTAB."""
-        out = np.zeros(40)
+        """{(prec, kind, kernel): (ms, launches, bytes)} for prec in f64/f32,
+        kind in momentum/pressure and kernel in A/B/C of the fused BiCGSTAB
+        (sampled launches)."""
+        out = np.zeros(36)
         _check(lib().ns3d_timing_read(self.h, _p(out, C.c_double)))
         res = {}
         for q, prec in enumerate(("f64", "f32")):
@@ -274,7 +277,6 @@ class NavierStokesGPU:
                 for k, name in enumerate("ABC"):
                     o = 18 * q + 9 * m + 3 * k
                     res[(prec, kind, name)] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
-                res[(prec, kind, "fin")] = (float(out[36 + 2 * q + m]), 0.0, 0.0)
         return res
 
     def set_precision(self, mixed: bool, inner_tol: float = 1e-4):

@@ -133,7 +133,7 @@ class Solver:
         _gpu._check(_gpu.lib().ns3d_timing(self.context(), int(enable)))
 
     def timing_read(self):
-        out = np.zeros(40)
+        out = np.zeros(36)
         _gpu._check(_gpu.lib().ns3d_timing_read(self.context(), out.ctypes.data_as(C.POINTER(C.c_double))))
         res = {}
         for q, prec in enumerate(("f64", "f32")):
@@ -141,7 +141,6 @@ class Solver:
                 for k, name in enumerate("ABC"):
                     o = 18 * q + 9 * m + 3 * k
                     res[(prec, kind, name)] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
-                res[(prec, kind, "fin")] = (float(out[36 + 2 * q + m]), 0.0, 0.0)
         return res
 
     def nnz(self):

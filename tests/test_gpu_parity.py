@@ -234,3 +234,37 @@ def test_fixture_initial_permeability():
     assert [i[0] for i in info] == [11, 500, 500] and all(i[1] == 0 for i in info)
     K = g.compute_permeability()[2]
     assert abs(K / 9.14196330389082e-13 - 1) < 1e-9
+
+
+@pytest.mark.parametrize("kind", [O.PC_SIMPLEC, O.PRESSURE_FULL])
+def test_packed_symmetric_solve_matches_indexed_layout(kind):
+    """The context solves its pressure system from packed neighbour distances
+    and the upper coefficients only; the stand-alone entry point solves the
+    same matrix from the full int32-indexed storage.  Same arithmetic, same
+    reduction order: the results must agree bit for bit."""
+    img = small_image((22, 19, 17), seed=8)
+    o, g = pair(img)
+    f = vel_fields(o, 8)
+    for k in ("u", "v", "w", "uh", "vh", "wh", "p"):
+        g.set(k, f[k])
+    g.assemble_momentum(0.8)
+    g.assemble_pressure(kind, 1, 0.7, O.SET_HAT if kind == O.PRESSURE_FULL else O.SET_VEL)
+    diag, off, nbr, _ = g.matrix(O.SYS_P)
+    n = g.np
+    rows = np.arange(n)
+    for a, bdir in ((0, 1), (2, 3), (4, 5)):  # E-W, N-S, T-B pairs are transposes
+        has = nbr[bdir] != rows
+        assert bit_equal(off[bdir][has], off[a][nbr[bdir][has]])
+    b = g.get("b")
+    x0 = 1e-3 * np.random.default_rng(8).standard_normal(n)
+    if kind == O.PRESSURE_FULL:
+        g.set("p", x0)
+        info = g.compute_pressure_field(1e-10, 5000)
+        x_ctx = g.get("p")
+    else:
+        g.set("pc", x0)
+        info = g.compute_pressure_correction(1e-10, 5000)
+        x_ctx = g.get("pc")
+    x_ind, info_ind = G.bicgstab_stencil(diag, off, nbr, b, x0, 1e-10, 5000)
+    assert info[0] == info_ind[0] and info[1] == info_ind[1] == 0
+    assert bit_equal(x_ctx, x_ind)

@@ -205,3 +205,39 @@ def test_divergence_of_fixture_fields(lattice128):
     assert abs(normdiv / GOLD["info"]["final_normdiv"] - 1) < 1e-12
     q, vmoy, K, pin, pout = o.compute_permeability()
     assert abs(q / (GOLD["fields"]["Vx"]["outlet_sum"] * 4e-12) - 1) < 1e-14
+
+
+@pytest.mark.parametrize("kind,relax", [(O.PRESSURE_FULL, 1), (O.PRESSURE_FULL, 0), (O.PC_SIMPLE, 0),
+                                        (O.PC_SIMPLEC, 0)])
+def test_pressure_matrix_is_bit_symmetric(kind, relax):
+    """Both rows of a face take the same -1/(a_f h^2) (NSBase.cxx:478-851:
+    each off-diagonal is written from the face's momentum diagonal or row
+    sum), so the pressure matrix equals its transpose bit for bit.  The GPU
+    solver relies on this: it reads only the diagonal and the E, N, T
+    coefficients of the pressure system and takes W, S, B from the
+    neighbours' E, N, T."""
+    import scipy.sparse as sp
+    from helpers import random_fields, small_image
+    img = small_image((18, 15, 14), seed=11)
+    o = O.Oracle(img, dx=1.3, dy=-1, dz=-1, mu=0.7, rho=2.1, p0=5.0, gradp=0.25, alpha=0.8, beta=1.0)
+    f = random_fields({"u": o.nu, "v": o.nv, "w": o.nw, "uh": o.nu, "vh": o.nv, "wh": o.nw, "p": o.np},
+                      seed=11)
+    for k, x in f.items():
+        o.set(k, x)
+    o.set_alpha_beta(0.8, 0.7)
+    o.assemble_momentum()
+    o.assemble_pressure(kind, relax, O.SET_HAT if kind == O.PRESSURE_FULL else O.SET_VEL)
+    cnt, col, val = o.matrix(O.SYS_P)
+    n = len(cnt)
+    rows = np.repeat(np.arange(n), cnt)
+    mask = np.arange(7)[None, :] < cnt[:, None]
+    A = sp.csr_matrix((val[mask], (rows, col[mask])), shape=(n, n))
+    A.sort_indices()
+    At = A.T.tocsr()
+    At.sort_indices()
+    assert np.array_equal(A.indptr, At.indptr) and np.array_equal(A.indices, At.indices)
+    assert np.array_equal(A.data.view(np.uint64) & ~np.uint64(1 << 63) == 0, At.data.view(np.uint64) & ~np.uint64(1 << 63) == 0)
+    nz = A.data != 0
+    assert np.array_equal(A.data[nz].view(np.uint64), At.data[nz].view(np.uint64))
+    # and the fixed point is not trivial: off-diagonals exist in all three axes
+    assert A.nnz > 4 * n
