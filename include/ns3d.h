@@ -221,6 +221,28 @@ ns3d_status ns3d_set_precision(ns3d_ctx* ctx, int mode, double inner_tol);
 ns3d_status ns3d_timing(ns3d_ctx* ctx, int enable);
 ns3d_status ns3d_timing_read(ns3d_ctx* ctx, double out[36]);
 
+/* ---- batched solves (no counterpart in the reference) -------------------
+ * An engine owns one stream and a dispatcher thread.  Contexts attached to
+ * it hand every BiCGSTAB solve (FP64, and the FP32 inner solves of the mixed
+ * form) to the engine, which runs the solves of all attached contexts that
+ * are waiting together: one launch per kernel phase over a table of tiles of
+ * all of them, solves joining and leaving at chunk boundaries (every `chunk`
+ * iterations).  The calling thread blocks until its solve has stopped; each
+ * attached context must be driven by its own host thread for solves to
+ * overlap.  Results are those of the context's own path (same kernels and
+ * tiles, same summation order), alone or in a batch.  The engine must
+ * outlive the contexts attached to it, or be detached from them first. */
+typedef struct ns3d_engine ns3d_engine;
+ns3d_status ns3d_engine_create(int device, int chunk, ns3d_engine** out);
+void ns3d_engine_destroy(ns3d_engine* eng);
+/* eng NULL detaches. */
+ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
+/* out[32]: chunks, launches, solves done, max active solves, sum of rows x
+ * iterations; then per precision q (0 FP64, 1 FP32) and kernel k (A, B, C),
+ * out[5 + 9q + 3k] = sampled milliseconds, sampled launches, algorithmic
+ * bytes of those launches. */
+ns3d_status ns3d_engine_stats(ns3d_engine* eng, double out[32]);
+
 /* Stored entries (diagonal + existing neighbours) of the u, v, w, p systems. */
 ns3d_status ns3d_matrix_nnz(ns3d_ctx* ctx, int64_t out[4]);
 

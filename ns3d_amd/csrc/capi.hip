@@ -5,12 +5,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <stdexcept>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/ns3d.h"
+#include "engine.h"
 #include "kernels.h"
 
 using namespace ns3d;
@@ -50,6 +52,8 @@ struct SysMem {
   float* invdiag32 = nullptr;
   uint64_t* pk = nullptr;    // packed neighbour distances (layout != ML_INDEXED)
   int layout = ML_INDEXED;   // how the solver reads the matrix (MatLayout)
+  Tile* tiles = nullptr;     // the solver's tiles of this system (device)
+  int ntiles = 0;
   SysView view() const { return SysView{n, d0, grid, cell, nbr, coef, invdiag, rowsum}; }
   SMat<double> mat64() const { return SMat<double>{n, layout, nbr, pk, coef, invdiag}; }
   SMat<float> mat32() const { return SMat<float>{n, layout, nbr, pk, coef32, invdiag32}; }
@@ -90,7 +94,9 @@ struct ns3d_ctx {
   double* rres = nullptr;   // FP64 residual of the defect correction
   int precision = NS3D_PRECISION_FP64;
   double inner_tol = 1e-4;
-  double* partials = nullptr;
+  double* partials = nullptr;        // [kPartialSlots][kMaxTilesPerSolve]
+  void* ddesc = nullptr;             // one SolveDesc (device) for the context's own solves
+  void* hdesc = nullptr;             // pinned staging for it
   double* result = nullptr;          // reduction results (device)
   BState* hstate = nullptr;          // pinned copy of the solver state
   double* hresult = nullptr;         // pinned copy of results
@@ -100,6 +106,7 @@ struct ns3d_ctx {
   Timing tm;
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> tev;  // start / stop of A, B, C of the sampled iteration
+  ns3d_engine* engine = nullptr;  // batched solve engine (null: solve on the context's stream)
 
   Geo geo() const { return Geo{nx, ny, nz, pg, ug, vg, wg}; }
 
@@ -140,6 +147,9 @@ ns3d_status guarded(F&& f) {
   } catch (const std::bad_alloc&) {
     g_err = "host allocation failed";
     return NS3D_ERR_OOM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return NS3D_ERR_HIP;
   }
 }
 
@@ -191,10 +201,32 @@ double kernel_bytes(int64_t n, int64_t nnz, int k, int vb) {
 // final solver state in c->hstate.  hint: iterations of the previous solve of
 // this system (first chunk size), updated on return.
 template <typename T>
-void bicg_loop(ns3d_ctx* c, const SysMem& S, SMat<T> M, T* x, const T* b, SWork<T> w, double tol, int64_t maxit,
+void bicg_loop(ns3d_ctx* c, int sid, SMat<T> M, T* x, const T* b, SWork<T> w, double tol, int64_t maxit,
                int64_t& hint, int kind) {
   hipStream_t s = c->stream;
-  HCHK(launch_bicg_init<T>(M, x, b, w, tol, (long long)maxit, s));
+  const SysMem& S = c->sys[sid];
+  if (c->engine) {
+    // batched: the engine runs this solve together with the other contexts'
+    EngineRequest req;
+    req.prec = sizeof(T) == 4 ? 1 : 0;
+    const SolveDesc<T> d{M, w, x, b, tol, (long long)maxit, c->partials, kMaxTilesPerSolve, 0, 0, 0};
+    if constexpr (sizeof(T) == 4) req.d32 = d;
+    else req.d64 = d;
+    req.n = S.n;
+    req.nnz = S.nnz;
+    req.kind = kind;
+    HCHK(hipEventRecord(c->ev[0], s));
+    req.ready = c->ev[0];
+    c->engine->solve(req);
+    *c->hstate = req.result;
+    if (c->hstate->status != BS_ZERO_RHS) hint = c->hstate->it_c > 0 ? c->hstate->it_c : 4;
+    return;
+  }
+  SolveDesc<T>* hd = (SolveDesc<T>*)c->hdesc;
+  *hd = SolveDesc<T>{M, w, x, b, tol, (long long)maxit, c->partials, kMaxTilesPerSolve, 0, S.ntiles, 0};
+  HCHK(hipMemcpyAsync(c->ddesc, hd, sizeof(SolveDesc<T>), hipMemcpyHostToDevice, s));
+  const Batch<T> B{(const SolveDesc<T>*)c->ddesc, S.tiles, 1, S.ntiles};
+  HCHK(launch_bicg_init<T>(B, s));
   HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));
   int64_t chunk = hint > 4 ? (hint * 3) / 4 : 4;
@@ -202,7 +234,7 @@ void bicg_loop(ns3d_ctx* c, const SysMem& S, SMat<T> M, T* x, const T* b, SWork<
   const int64_t cap = 4 * (maxit > 0 ? maxit : 1) + 64;
   const int prec = sizeof(T) == 4 ? 1 : 0;
   while (c->hstate->status == BS_RUNNING || c->hstate->status == BS_RESTART) {
-    if (c->hstate->status == BS_RESTART) HCHK(launch_bicg_restart<T>(M, x, b, w, s));
+    if (c->hstate->status == BS_RESTART) HCHK(launch_bicg_restart<T>(B, s));
     if (chunk > maxit + 1) chunk = maxit + 1;
     const long long done_before = c->hstate->it_c;
     // with timing on, the kernels of one iteration in the middle of the
@@ -211,9 +243,9 @@ void bicg_loop(ns3d_ctx* c, const SysMem& S, SMat<T> M, T* x, const T* b, SWork<
     const int64_t qt = c->tm.on ? chunk / 2 : -1;
     for (int64_t q = 0; q < chunk; ++q) {
       hipEvent_t* e = (q == qt) ? c->tev.data() : nullptr;
-      HCHK(launch_bicg_A<T>(M, w, s, e));
-      HCHK(launch_bicg_B<T>(M, w, s, e ? e + 2 : nullptr));
-      HCHK(launch_bicg_C<T>(M, x, w, s, e ? e + 4 : nullptr));
+      HCHK(launch_bicg_A<T>(B, s, e));
+      HCHK(launch_bicg_B<T>(B, s, e ? e + 2 : nullptr));
+      HCHK(launch_bicg_C<T>(B, s, e ? e + 4 : nullptr));
     }
     HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
@@ -239,7 +271,7 @@ void bicgstab64(ns3d_ctx* c, int sid, double* x, const double* b, double tol, in
     info->iterations = 0; info->info = 0; info->restarts = 0; info->error = 0; info->work = 0;
     return;
   }
-  bicg_loop<double>(c, S, S.mat64(), x, b, c->bw, tol, maxit, S.last_iters, sid == 3 ? 1 : 0);
+  bicg_loop<double>(c, sid, S.mat64(), x, b, c->bw, tol, maxit, S.last_iters, sid == 3 ? 1 : 0);
   const BState& st = *c->hstate;
   if (st.status == BS_ZERO_RHS) {
     HCHK(launch_fill(x, S.n, 0.0, c->stream));
@@ -296,7 +328,7 @@ void bicgstab_mixed(ns3d_ctx* c, int sid, double* x, const double* b, double tol
     const double rel = std::sqrt(rr / bb);
     const double tin = std::max(c->inner_tol, 0.5 * tol / rel);
     HCHK(launch_fill_f32(c->d32, S.n, s));
-    bicg_loop<float>(c, S, M32, c->d32, c->r32, c->bw32, tin, maxit - total, S.last_iters32, sid == 3 ? 1 : 0);
+    bicg_loop<float>(c, sid, M32, c->d32, c->r32, c->bw32, tin, maxit - total, S.last_iters32, sid == 3 ? 1 : 0);
     const BState& st = *c->hstate;
     const int64_t k = (st.status == BS_ZERO_RHS) ? 0 : st.it_c;
     info->restarts += (int32_t)st.restarts;
@@ -327,6 +359,25 @@ void refresh_f32(ns3d_ctx* c, int sid) {
     S.invdiag32 = c->alloc<float>(S.n);
   }
   HCHK(launch_coef_to_f32(S.coef, S.invdiag, S.n, S.coef32, S.invdiag32, c->stream));
+}
+
+// The solver's tile table of system sid (static for the context).
+void build_tiles(ns3d_ctx* c, int sid) {
+  SysMem& S = c->sys[sid];
+  std::vector<Tile> t;
+  int t0 = 0, t1 = 0;
+  make_tiles(0, S.n, t, t0, t1);
+  S.ntiles = t1;
+  S.tiles = c->alloc<Tile>(S.ntiles);
+  HCHK(hipMemcpyAsync(S.tiles, t.data(), t.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
+  HCHK(hipStreamSynchronize(c->stream));
+}
+
+// Descriptor buffers of the context's own solves.
+void alloc_solver_common(ns3d_ctx* c) {
+  const size_t dsz = std::max(sizeof(SolveDesc<double>), sizeof(SolveDesc<float>));
+  c->ddesc = c->alloc<char>((int64_t)dsz);
+  HCHK(hipHostMalloc(&c->hdesc, dsz, hipHostMallocDefault));
 }
 
 void alloc_system(ns3d_ctx* c, int sid, int n, int d0, int32_t* grid, bool momentum) {
@@ -462,8 +513,9 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
     w.v[0] = c->alloc<double>(nmax);
     w.v[1] = c->alloc<double>(nmax);
     w.t = c->alloc<double>(nmax);
-    c->partials = c->alloc<double>(8 * kMaxRedBlocks);
-    w.partials = c->partials;
+    c->partials = c->alloc<double>((int64_t)kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks));
+    alloc_solver_common(c);
+    for (int q = 0; q < 4; ++q) build_tiles(c, q);
     w.st = c->alloc<BState>(1);
     HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), s));
     c->result = c->alloc<double>(16);
@@ -490,6 +542,7 @@ void ns3d_destroy(ns3d_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->hstate) (void)hipHostFree(c->hstate);
   if (c->hresult) (void)hipHostFree(c->hresult);
+  if (c->hdesc) (void)hipHostFree(c->hdesc);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->tev)
@@ -781,8 +834,9 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
     w.p[0] = c->alloc<double>(n); w.p[1] = c->alloc<double>(n);
     w.v[0] = c->alloc<double>(n); w.v[1] = c->alloc<double>(n);
     w.t = c->alloc<double>(n);
-    c->partials = c->alloc<double>(8 * kMaxRedBlocks);
-    w.partials = c->partials;
+    c->partials = c->alloc<double>((int64_t)kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks));
+    alloc_solver_common(c);
+    build_tiles(c, 3);
     w.st = c->alloc<BState>(1);
     HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), s));
     HCHK(hipHostMalloc((void**)&c->hstate, sizeof(BState), hipHostMallocDefault));
@@ -838,7 +892,6 @@ ns3d_status ns3d_set_precision(ns3d_ctx* c, int mode, double inner_tol) {
       w.v[0] = c->alloc<float>(nmax);
       w.v[1] = c->alloc<float>(nmax);
       w.t = c->alloc<float>(nmax);
-      w.partials = c->partials;
       w.st = c->bw.st;
       c->d32 = c->alloc<float>(nmax);
       c->r32 = c->alloc<float>(nmax);
@@ -854,6 +907,34 @@ ns3d_status ns3d_set_precision(ns3d_ctx* c, int mode, double inner_tol) {
 ns3d_status ns3d_matrix_nnz(ns3d_ctx* c, int64_t out[4]) {
   if (!c || !out) return fail_invalid("ns3d_matrix_nnz: bad arguments");
   for (int q = 0; q < 4; ++q) out[q] = c->sys[q].nnz;
+  return NS3D_OK;
+}
+
+ns3d_status ns3d_engine_create(int device, int chunk, ns3d_engine** out) {
+  if (!out) return fail_invalid("ns3d_engine_create: null output");
+  *out = nullptr;
+  return guarded([&] {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw Invalid{"no GPU visible"};
+    if (device < 0) HCHK(hipGetDevice(&device));
+    if (device >= ndev) throw Invalid{"ns3d_engine_create: device index out of range"};
+    if (chunk < 1 || chunk > 256) throw Invalid{"ns3d_engine_create: chunk must be in [1, 256]"};
+    *out = new ns3d_engine(device, chunk);
+  });
+}
+
+void ns3d_engine_destroy(ns3d_engine* e) { delete e; }
+
+ns3d_status ns3d_attach_engine(ns3d_ctx* c, ns3d_engine* e) {
+  if (!c) return fail_invalid("ns3d_attach_engine: null context");
+  if (e && e->device != c->device) return fail_invalid("ns3d_attach_engine: engine and context on different devices");
+  c->engine = e;
+  return NS3D_OK;
+}
+
+ns3d_status ns3d_engine_stats(ns3d_engine* e, double out[32]) {
+  if (!e || !out) return fail_invalid("ns3d_engine_stats: bad arguments");
+  e->stats(out, 32);
   return NS3D_OK;
 }
 

@@ -173,19 +173,33 @@ __device__ __forceinline__ void finalize_partials(const double* partials, int nb
   }
 }
 
+// Block sums of NV values stored at base[q * qstride + idx] (thread 0).
+template <int NV, typename P>
+__device__ __forceinline__ void block_sums_at(double (&v)[NV], P base, int64_t qstride, int64_t idx) {
+  __shared__ double sh[NV][kWaves];
+  double bv[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) bv[q] = block_reduce<RSUM>(v[q], sh[q]);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) base[q * qstride + idx] = bv[q];
+  }
+}
+
 // In every block of a kernel: all threads get the NV sums of nb partials per
-// value (slot q at partials[q * nb + b]).  The order of the additions is
+// value (slot q at partials[q * qstride + b], qstride = nb by default).  The order of the additions is
 // fixed and the same in every block, so every block computes the same
 // bits; this replaces a separate finalize launch between two row kernels.
-template <int NV>
-__device__ __forceinline__ void sum_partials_all(const double* __restrict__ partials, int nb, double (&out)[NV]) {
+template <int NV, typename P>
+__device__ __forceinline__ void sum_partials_all(P partials, int nb, double (&out)[NV], int64_t qstride = -1) {
+  if (qstride < 0) qstride = nb;
   __shared__ double sh[NV][kWaves];
   double a[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) a[q] = 0.0;
   for (int b = threadIdx.x; b < nb; b += kBlock) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) a[q] += partials[(int64_t)q * nb + b];
+    for (int q = 0; q < NV; ++q) a[q] += partials[q * qstride + b];
   }
 #pragma unroll
   for (int q = 0; q < NV; ++q) a[q] = wave_reduce<RSUM>(a[q]);

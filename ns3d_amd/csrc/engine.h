@@ -1,0 +1,92 @@
+// engine.h -- the batched solve engine: one dispatcher thread per engine
+// runs the BiCGSTAB solves submitted by any number of contexts (each driven
+// by its own host thread) together, one launch per kernel phase over all of
+// them (continuous batching: solves join and leave at chunk boundaries).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "kernels.h"
+
+namespace ns3d {
+
+// One solve handed to the engine.  The caller fills prec, the descriptor of
+// that precision (its partials buffer of kPartialSlots x kMaxTilesPerSolve
+// doubles included; t0/t1 are set by the engine), n, nnz, kind (0 momentum,
+// 1 pressure, for the statistics) and ready (an event recorded on the
+// caller's stream after the inputs were written; may be null), then blocks
+// in ns3d_engine::solve until done.
+struct EngineRequest {
+  int prec = 0;  // 0 FP64, 1 FP32
+  SolveDesc<double> d64{};
+  SolveDesc<float> d32{};
+  int64_t n = 0, nnz = 0;
+  int kind = 0;
+  hipEvent_t ready = nullptr;
+  BState result{};
+  bool done = false;
+};
+
+}  // namespace ns3d
+
+struct ns3d_engine {
+  explicit ns3d_engine(int device, int chunk);
+  ~ns3d_engine();
+  // Blocks until the solve has stopped (result holds the final state);
+  // throws std::runtime_error if the engine failed.
+  void solve(ns3d::EngineRequest& req);
+  void stats(double* out, int count);
+
+  int device = 0;
+  int chunk = 8;
+
+ private:
+  struct Group {
+    std::vector<ns3d::EngineRequest*> active;
+    void* d_desc = nullptr;
+    ns3d::Tile* d_tiles = nullptr;
+    void* d_idesc = nullptr;
+    ns3d::Tile* d_itiles = nullptr;
+    void* h_desc = nullptr;    // pinned staging
+    ns3d::Tile* h_tiles = nullptr;
+    void* h_idesc = nullptr;
+    ns3d::Tile* h_itiles = nullptr;
+    int ntiles = 0;
+    bool dirty = true;
+  };
+  void run();
+  void admit(std::vector<ns3d::EngineRequest*>& reqs);
+  void rebuild(int g);
+  template <typename T>
+  ns3d::Batch<T> batch(int g) const;
+  void fail(const std::string& msg);
+  bool any_active() const { return !grp[0].active.empty() || !grp[1].active.empty(); }
+
+  hipStream_t stream = nullptr;
+  std::thread thr;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<ns3d::EngineRequest*> pending;
+  bool stopping = false;
+  std::string fatal;
+  Group grp[2];
+  ns3d::BState** d_stptr = nullptr;   // state pointers of all active solves
+  ns3d::BState** h_stptr = nullptr;
+  ns3d::BState* d_states = nullptr;   // gathered copies
+  ns3d::BState* h_states = nullptr;
+  std::vector<ns3d::EngineRequest*> order;  // solve order of d_stptr
+  hipEvent_t tev[12] = {};
+  std::vector<void*> allocs;
+  // statistics
+  double n_chunks = 0, n_launch = 0, n_done = 0, max_active = 0;
+  double ms[2][3] = {}, launches[2][3] = {}, bytes[2][3] = {};
+  double rows_x_iters = 0;
+};

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "device.h"
 
 namespace ns3d {
@@ -87,9 +89,6 @@ struct SMat {
   const T* invdiag;     // [n]
 };
 // BiCGSTAB vectors of storage type T; scalars and reductions are FP64.
-// partials: [kPartialSlots][kMaxRedBlocks] block partials (A: r0.v in slot
-// 0; B: t.s, t.t in 1-2; C: r0.r, r.r in 3-4; init / restart in 5-6).
-constexpr int kPartialSlots = 8;
 template <typename T>
 struct SWork {
   T* r;
@@ -97,24 +96,69 @@ struct SWork {
   T* p[2];
   T* v[2];
   T* t;
-  double* partials;
   BState* st;
 };
 using BWork = SWork<double>;
 
+// One linear solve as the batched kernels see it: the matrix, the vectors,
+// the unknown and right-hand side, the tolerance and iteration cap, the
+// range [t0, t1) of its tiles in the launch's tile table, and its own block
+// partials: tile t writes partials[slot * pstride + (t - t0)] (kPartialSlots
+// slots: A: r0.v in slot 0; B: t.s, t.t in 1-2; C: r0.r, r.r in 3-4; init /
+// restart in 5-6).  Partials belong to the solve, not to the table, so they
+// stay valid when the table is rebuilt between two kernels of an iteration.
 template <typename T>
-hipError_t launch_bicg_init(SMat<T> S, const T* x, const T* b, SWork<T> w, double tol, long long maxit,
-                            hipStream_t s);
+struct SolveDesc {
+  SMat<T> S;
+  SWork<T> w;
+  T* x;
+  const T* b;
+  double tol;
+  long long maxit;
+  double* partials;
+  int pstride;
+  int t0, t1;
+  int pad;
+};
+// A tile: rows [begin, end) of one solve's system, processed by one block.
+struct Tile {
+  int solve;
+  int pad;
+  long long begin, end;
+};
+// The operands of one launch over a set of solves of storage type T.
+constexpr int kPartialSlots = 8;
 template <typename T>
-hipError_t launch_bicg_restart(SMat<T> S, const T* x, const T* b, SWork<T> w, hipStream_t s);
-// ev (may be null): two events per kernel, {start, stop} taken from the
-// kernel's own dispatch (hipExtLaunchKernelGGL).
+struct Batch {
+  const SolveDesc<T>* d;
+  const Tile* tiles;
+  int nsolves, ntiles;
+};
+// Tiles of one solve of n rows: between 1 and kMaxTilesPerSolve tiles of
+// a multiple of kBlock rows, at least kTileRows rows each where n allows.
+constexpr int kMaxTilesPerSolve = 1024;
+constexpr int kTileRows = 1536;
+int tiles_for(int64_t n);
+// Appends the tiles of solve `solve` (n rows) to out; returns its [t0, t1).
+void make_tiles(int solve, int64_t n, std::vector<Tile>& out, int& t0, int& t1);
+// Blocks of a launch over ntiles tiles (a multiple of the XCD count).
+int tile_grid(int ntiles);
+
+// init: r = b - A x, r0 = r, p = v = 0, |r|^2 and |b|^2 -> BState (status
+// RUNNING, DONE or ZERO_RHS), for every solve of the batch.
 template <typename T>
-hipError_t launch_bicg_A(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t* ev = nullptr);
+hipError_t launch_bicg_init(Batch<T> B, hipStream_t s);
+// restart (solves whose status is RESTART; others exit at once).
 template <typename T>
-hipError_t launch_bicg_B(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t* ev = nullptr);
+hipError_t launch_bicg_restart(Batch<T> B, hipStream_t s);
+// The three row kernels of one iteration.  ev (may be null): {start, stop}
+// taken from the kernel's own dispatch (hipExtLaunchKernelGGL).
 template <typename T>
-hipError_t launch_bicg_C(SMat<T> S, T* x, SWork<T> w, hipStream_t s, hipEvent_t* ev = nullptr);
+hipError_t launch_bicg_A(Batch<T> B, hipStream_t s, hipEvent_t* ev = nullptr);
+template <typename T>
+hipError_t launch_bicg_B(Batch<T> B, hipStream_t s, hipEvent_t* ev = nullptr);
+template <typename T>
+hipError_t launch_bicg_C(Batch<T> B, hipStream_t s, hipEvent_t* ev = nullptr);
 hipError_t launch_invdiag(SysView S, hipStream_t s);
 // Packs nbr[6][n] into pk[n]; *bad (zeroed by the caller) becomes nonzero if
 // a distance does not fit or a neighbour is not on its expected side.

@@ -33,6 +33,9 @@
 // instantiation computes exactly the expressions above.
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace ns3d {
@@ -43,8 +46,6 @@ constexpr double kEps = 2.220446049250313e-16;  // NumTraits<double>::epsilon()
 constexpr int kSlotA = 0, kSlotB = 1, kSlotC = 3, kSlotI = 5;
 
 
-__device__ __forceinline__ double* slot(double* partials, int q) { return partials + (int64_t)q * kMaxRedBlocks; }
-__device__ __forceinline__ bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
 // The seven points of a row in SpMV order: W, S, B, the row itself, T, N,
 // E -- ascending column order for monotone numbering (i-1 < j-1 < k-1 <
@@ -56,8 +57,8 @@ struct RowPts {
   T c[7];
 };
 
-template <typename T, int L>
-__device__ __forceinline__ void row_pts(const SMat<T>& S, int row, int64_t n, RowPts<T>& P) {
+template <typename T, int L, typename M>
+__device__ __forceinline__ void row_pts(const M& S, int row, int64_t n, RowPts<T>& P) {
   int nW, nS, nB, nT, nN, nE;
   bool hW = true, hS = true, hB = true;
   if constexpr (L == ML_INDEXED) {
@@ -99,8 +100,8 @@ __device__ __forceinline__ double spmv_pts(const RowPts<T>& P, F&& y) {
   return t;
 }
 
-template <typename T, int L, typename F>
-__device__ __forceinline__ double stencil_row(const SMat<T>& S, int row, int64_t n, F&& x) {
+template <typename T, int L, typename M, typename F>
+__device__ __forceinline__ double stencil_row(const M& S, int row, int64_t n, F&& x) {
   RowPts<T> P;
   row_pts<T, L>(S, row, n, P);
   return spmv_pts(P, [&](int k) { return x(P.q[k]); });
@@ -114,9 +115,8 @@ struct AIn {
   T r[7], p[7], v[7], m[7];
   T r0;
 };
-template <typename T, int L>
-__device__ __forceinline__ void a_fetch(const SMat<T>& S, const SWork<T>& w, const T* __restrict__ pold,
-                                        const T* __restrict__ vold, int row, int64_t n, AIn<T>& f) {
+template <typename T, int L, typename M, typename W, typename P>
+__device__ __forceinline__ void a_fetch(const M& S, const W& w, P pold, P vold, int row, int64_t n, AIn<T>& f) {
   row_pts<T, L>(S, row, n, f.P);
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
@@ -135,9 +135,8 @@ struct BIn {
   RowPts<T> P;
   T r[7], v[7], m[7];
 };
-template <typename T, int L>
-__device__ __forceinline__ void b_fetch(const SMat<T>& S, const SWork<T>& w, const T* __restrict__ vn, int row,
-                                        int64_t n, BIn<T>& f) {
+template <typename T, int L, typename M, typename W, typename P>
+__device__ __forceinline__ void b_fetch(const M& S, const W& w, P vn, int row, int64_t n, BIn<T>& f) {
   row_pts<T, L>(S, row, n, f.P);
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
@@ -153,9 +152,8 @@ template <typename T>
 struct CIn {
   T m, p, r, v, t, x, r0;
 };
-template <typename T>
-__device__ __forceinline__ void c_fetch(const T* __restrict__ invdiag, const T* __restrict__ x, const SWork<T>& w,
-                                        int par, int64_t row, CIn<T>& f) {
+template <typename T, typename P, typename X, typename W>
+__device__ __forceinline__ void c_fetch(P invdiag, X x, const W& w, int par, int64_t row, CIn<T>& f) {
   f.m = invdiag[row];
   f.p = w.p[par][row];
   f.r = w.r[row];
@@ -165,37 +163,141 @@ __device__ __forceinline__ void c_fetch(const T* __restrict__ invdiag, const T* 
   f.r0 = w.r0[row];
 }
 
-// r = b - A x, r0 = r; zero p, v of the first buffer; partials |r|^2, |b|^2.
-template <typename T, int L>
-__global__ void __launch_bounds__(kBlock)
-bicg_init_kernel(SMat<T> S, const T* __restrict__ x, const T* __restrict__ b, SWork<T> w) {
-  const int64_t n = S.n;
-  const RowIter rr = block_rows(n);
-  double v2[2] = {0.0, 0.0};
-  for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += rr.step) {
-    const double ax = stencil_row<T, L>(S, (int)row, n, [&](int q) { return (double)x[q]; });
-    const double bv = b[row];
-    const double r = bv - ax;
-    w.r[row] = (T)r;
-    w.r0[row] = (T)r;
-    w.p[0][row] = (T)0;
-    w.v[0][row] = (T)0;
-    const double rs = (double)(T)r;
-    v2[0] += rs * rs;
-    v2[1] += bv * bv;
-  }
-  const int ops[2] = {RSUM, RSUM};
-  block_partials<2>(v2, ops, slot(w.partials, kSlotI));
+// Tile of this block (logical tiles of one XCD are contiguous), -1 past the
+// end of the table (the grid is padded to a multiple of the XCD count).
+__device__ __forceinline__ int block_tile(int ntiles) {
+  const int t = logical_block();
+  return t < ntiles ? t : -1;
 }
 
+// Reads a block-uniform struct and moves it to scalar registers.  The
+// kernels store to global memory, so the compiler cannot prove the table
+// read-only and would keep the fields in vector registers.
+template <typename S>
+__device__ __forceinline__ S uniform_load(const S* p) {
+  static_assert(sizeof(S) % 4 == 0, "uniform_load: size must be a multiple of 4");
+  constexpr int kWords = (int)(sizeof(S) / 4);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+  uint32_t w[kWords];
+#pragma unroll
+  for (int q = 0; q < kWords; ++q) w[q] = __builtin_amdgcn_readfirstlane(src[q]);
+  S out;
+  __builtin_memcpy(&out, w, sizeof(S));
+  return out;
+}
+
+// Pointers read from a table are generic pointers to the compiler, which
+// then emits flat loads and stores (slower, and more registers).  The
+// kernels work on views whose pointers are typed global (address space 1).
+#define NS3D_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+struct GMat {
+  int n;
+  int layout;
+  const NS3D_GLOBAL int32_t* nbr;
+  const NS3D_GLOBAL uint64_t* pk;
+  const NS3D_GLOBAL T* coef;
+  const NS3D_GLOBAL T* invdiag;
+};
+template <typename T>
+struct GWork {
+  NS3D_GLOBAL T* r;
+  NS3D_GLOBAL T* r0;
+  NS3D_GLOBAL T* p[2];
+  NS3D_GLOBAL T* v[2];
+  NS3D_GLOBAL T* t;
+  NS3D_GLOBAL BState* st;
+};
+template <typename T>
+struct GDesc {
+  GMat<T> S;
+  GWork<T> w;
+  NS3D_GLOBAL T* x;
+  const NS3D_GLOBAL T* b;
+  double tol;
+  long long maxit;
+  NS3D_GLOBAL double* partials;
+  int pstride;
+  int t0, t1;
+  // slot q of this solve's partials (tile-local index)
+  __device__ __forceinline__ NS3D_GLOBAL double* slot(int q) const { return partials + (int64_t)q * pstride; }
+};
+template <typename P>
+__device__ __forceinline__ NS3D_GLOBAL P* gl(P* p) {
+  return (NS3D_GLOBAL P*)p;
+}
+template <typename T>
+__device__ __forceinline__ GDesc<T> load_desc(const SolveDesc<T>* p) {
+  const SolveDesc<T> D = uniform_load(p);
+  GDesc<T> G;
+  G.S = GMat<T>{D.S.n, D.S.layout, gl(D.S.nbr), gl(D.S.pk), gl(D.S.coef), gl(D.S.invdiag)};
+  G.w.r = gl(D.w.r);
+  G.w.r0 = gl(D.w.r0);
+  G.w.p[0] = gl(D.w.p[0]);
+  G.w.p[1] = gl(D.w.p[1]);
+  G.w.v[0] = gl(D.w.v[0]);
+  G.w.v[1] = gl(D.w.v[1]);
+  G.w.t = gl(D.w.t);
+  G.w.st = gl(D.w.st);
+  G.x = gl(D.x);
+  G.b = gl(D.b);
+  G.tol = D.tol;
+  G.maxit = D.maxit;
+  G.partials = gl(D.partials);
+  G.pstride = D.pstride;
+  G.t0 = D.t0;
+  G.t1 = D.t1;
+  return G;
+}
+
+// Calls body.template operator()<L>() with the solve's layout as a
+// compile-time constant (the branch is uniform over the block).
+template <typename F>
+__device__ __forceinline__ void by_layout(int layout, F&& body) {
+  if (layout == ML_PACKED_SYM) body(std::integral_constant<int, ML_PACKED_SYM>{});
+  else if (layout == ML_PACKED) body(std::integral_constant<int, ML_PACKED>{});
+  else body(std::integral_constant<int, ML_INDEXED>{});
+}
+
+// r = b - A x, r0 = r; zero p, v of the first buffer; partials |r|^2, |b|^2.
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-bicg_init_fin(double* partials, BState* st, int nb, double tol, long long maxit) {
-  const int ops[2] = {RSUM, RSUM};
+bicg_init_kernel(Batch<T> B) {
+  const int t = block_tile(B.ntiles);
+  if (t < 0) return;
+  const Tile tl = uniform_load(B.tiles + t);
+  const GDesc<T> D = load_desc(B.d + tl.solve);
+  double v2[2] = {0.0, 0.0};
+  by_layout(D.S.layout, [&](auto LC) {
+    constexpr int L = decltype(LC)::value;
+    const int64_t n = D.S.n;
+    for (int64_t row = tl.begin + threadIdx.x; row < tl.end; row += kBlock) {
+      const double ax = stencil_row<T, L>(D.S, (int)row, n, [&](int q) { return (double)D.x[q]; });
+      const double bv = D.b[row];
+      const double r = bv - ax;
+      D.w.r[row] = (T)r;
+      D.w.r0[row] = (T)r;
+      D.w.p[0][row] = (T)0;
+      D.w.v[0][row] = (T)0;
+      const double rs = (double)(T)r;
+      v2[0] += rs * rs;
+      v2[1] += bv * bv;
+    }
+  });
+  block_sums_at<2>(v2, D.slot(kSlotI), D.pstride, t - D.t0);
+}
+
+// One block per solve.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+bicg_init_fin(Batch<T> B) {
+  const GDesc<T> D = load_desc(B.d + blockIdx.x);
   double out[2];
-  finalize_partials<2>(slot(partials, kSlotI), nb, ops, out);
+  sum_partials_all<2>(D.slot(kSlotI), D.t1 - D.t0, out, D.pstride);
   if (threadIdx.x != 0) return;
-  st->tol = tol;
-  st->maxit = maxit;
+  NS3D_GLOBAL BState* st = D.w.st;
+  st->tol = D.tol;
+  st->maxit = D.maxit;
   st->r0_sqnorm = out[0];
   st->rhs_sqnorm = out[1];
   st->it_a = 0;
@@ -207,41 +309,49 @@ bicg_init_fin(double* partials, BState* st, int nb, double tol, long long maxit)
     st->status = BS_ZERO_RHS;
     return;
   }
-  st->tol2 = tol * tol * out[1];
+  st->tol2 = D.tol * D.tol * out[1];
   // the slot of iteration -1 holds Eigen's initial rho = alpha = w = 1
   st->rho[1] = 1;
   st->alpha[1] = 1;
   st->w[1] = 1;
   st->rho_fresh = out[0];  // rho = r0.r = |r0|^2 (no restart possible at i = 0)
   st->fresh_at = 0;
-  st->status = (out[0] > st->tol2 && 0 < maxit) ? BS_RUNNING : BS_DONE;
+  st->status = (out[0] > st->tol2 && 0 < D.maxit) ? BS_RUNNING : BS_DONE;
 }
 
-template <typename T, int L>
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-bicg_restart_kernel(SMat<T> S, const T* __restrict__ x, const T* __restrict__ b, SWork<T> w) {
-  if (w.st->status != BS_RESTART) return;
-  const int64_t n = S.n;
-  const RowIter rr = block_rows(n);
+bicg_restart_kernel(Batch<T> B) {
+  const int t = block_tile(B.ntiles);
+  if (t < 0) return;
+  const Tile tl = uniform_load(B.tiles + t);
+  const GDesc<T> D = load_desc(B.d + tl.solve);
+  if (D.w.st->status != BS_RESTART) return;
   double v1[1] = {0.0};
-  for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += rr.step) {
-    const double ax = stencil_row<T, L>(S, (int)row, n, [&](int q) { return (double)x[q]; });
-    const double r = (double)b[row] - ax;
-    w.r[row] = (T)r;
-    w.r0[row] = (T)r;
-    const double rs = (double)(T)r;
-    v1[0] += rs * rs;
-  }
-  const int ops[1] = {RSUM};
-  block_partials<1>(v1, ops, slot(w.partials, kSlotI));
+  by_layout(D.S.layout, [&](auto LC) {
+    constexpr int L = decltype(LC)::value;
+    const int64_t n = D.S.n;
+    for (int64_t row = tl.begin + threadIdx.x; row < tl.end; row += kBlock) {
+      const double ax = stencil_row<T, L>(D.S, (int)row, n, [&](int q) { return (double)D.x[q]; });
+      const double r = (double)D.b[row] - ax;
+      D.w.r[row] = (T)r;
+      D.w.r0[row] = (T)r;
+      const double rs = (double)(T)r;
+      v1[0] += rs * rs;
+    }
+  });
+  block_sums_at<1>(v1, D.slot(kSlotI), D.pstride, t - D.t0);
 }
 
+// One block per solve.
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-bicg_restart_fin(double* partials, BState* st, int nb) {
+bicg_restart_fin(Batch<T> B) {
+  const GDesc<T> D = load_desc(B.d + blockIdx.x);
+  NS3D_GLOBAL BState* st = D.w.st;
   if (st->status != BS_RESTART) return;
-  const int ops[1] = {RSUM};
   double out[1];
-  finalize_partials<1>(slot(partials, kSlotI), nb, ops, out);
+  sum_partials_all<1>(D.slot(kSlotI), D.t1 - D.t0, out, D.pstride);
   if (threadIdx.x != 0) return;
   // the iteration that found the restart is re-run with the new residual;
   // rho_old stays the pre-restart rho (slot of the previous iteration)
@@ -262,154 +372,178 @@ bicg_restart_fin(double* partials, BState* st, int nb) {
 #define NS3D_SOLVE_PIPE 0
 #endif
 
-template <typename T, int L>
+// The block leader (first thread of the solve's first tile) records the
+// scalars for the host.
+__device__ __forceinline__ bool leader_of(int t, int t0) { return t == t0 && threadIdx.x == 0; }
+
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-bicg_A_kernel(SMat<T> S, SWork<T> w) {
-  BState* st = w.st;
+bicg_A_kernel(Batch<T> B) {
+  const int t = block_tile(B.ntiles);
+  if (t < 0) return;
+  const Tile tl = uniform_load(B.tiles + t);
+  const GDesc<T> D = load_desc(B.d + tl.solve);
+  NS3D_GLOBAL BState* st = D.w.st;
   if (st->status != BS_RUNNING) return;
+  const bool lead = leader_of(t, D.t0);
   const long long it = st->it_c;
   const int sl = (int)(it & 1);
   const int par = sl;
-  const T* __restrict__ pold = w.p[par];
-  const T* __restrict__ vold = w.v[par];
-  T* __restrict__ pnew = w.p[par ^ 1];
-  T* __restrict__ vnew = w.v[par ^ 1];
-  const int64_t n = S.n;
-  const RowIter rr = block_rows(n);
-  int64_t row = rr.begin + threadIdx.x;
-  AIn<T> cur;
-  if (row < rr.end) a_fetch<T, L>(S, w, pold, vold, (int)row, n, cur);
-  double rho;
-  if (st->fresh_at == it) {
-    rho = st->rho_fresh;
-  } else {
-    double red[2];
-    sum_partials_all<2>(slot(w.partials, kSlotC), gridDim.x, red);
-    const double r0r = red[0], rr2 = red[1];
-    if (!(rr2 > st->tol2 && it - st->ibase < st->maxit)) {
-      if (leader()) { st->rr = rr2; st->status = BS_DONE; }
-      return;
-    }
-    if (fabs(r0r) < kEps * kEps * st->r0_sqnorm) {
-      if (leader()) { st->rr = rr2; st->status = BS_RESTART; }
-      return;
-    }
-    rho = r0r;
-    if (leader()) st->rr = rr2;
-  }
-  const double beta = (rho / st->rho[sl ^ 1]) * (st->alpha[sl ^ 1] / st->w[sl ^ 1]);
-  const double om = st->w[sl ^ 1];
-  if (leader()) {
-    st->rho[sl] = rho;
-    st->it_a = it;
-  }
+  const NS3D_GLOBAL T* __restrict__ pold = D.w.p[par];
+  const NS3D_GLOBAL T* __restrict__ vold = D.w.v[par];
+  NS3D_GLOBAL T* __restrict__ pnew = D.w.p[par ^ 1];
+  NS3D_GLOBAL T* __restrict__ vnew = D.w.v[par ^ 1];
   double acc = 0.0;
-  while (row < rr.end) {
-    const int64_t nxt = row + rr.step;
-    AIn<T> nx;
-    if (NS3D_SOLVE_PIPE && nxt < rr.end) a_fetch<T, L>(S, w, pold, vold, (int)nxt, n, nx);
-    double pn[7];
+  bool stop = false;
+  by_layout(D.S.layout, [&](auto LC) {
+    constexpr int L = decltype(LC)::value;
+    const int64_t n = D.S.n;
+    int64_t row = tl.begin + threadIdx.x;
+    AIn<T> cur;
+    if (row < tl.end) a_fetch<T, L>(D.S, D.w, pold, vold, (int)row, n, cur);
+    double rho;
+    if (st->fresh_at == it) {
+      rho = st->rho_fresh;
+    } else {
+      double red[2];
+      sum_partials_all<2>(D.slot(kSlotC), D.t1 - D.t0, red, D.pstride);
+      const double r0r = red[0], rr2 = red[1];
+      if (!(rr2 > st->tol2 && it - st->ibase < st->maxit)) {
+        if (lead) { st->rr = rr2; st->status = BS_DONE; }
+        stop = true;
+        return;
+      }
+      if (fabs(r0r) < kEps * kEps * st->r0_sqnorm) {
+        if (lead) { st->rr = rr2; st->status = BS_RESTART; }
+        stop = true;
+        return;
+      }
+      rho = r0r;
+      if (lead) st->rr = rr2;
+    }
+    const double beta = (rho / st->rho[sl ^ 1]) * (st->alpha[sl ^ 1] / st->w[sl ^ 1]);
+    const double om = st->w[sl ^ 1];
+    if (lead) {
+      st->rho[sl] = rho;
+      st->it_a = it;
+    }
+    while (row < tl.end) {
+      const int64_t nxt = row + kBlock;
+      AIn<T> nx;
+      if (NS3D_SOLVE_PIPE && nxt < tl.end) a_fetch<T, L>(D.S, D.w, pold, vold, (int)nxt, n, nx);
+      double pn[7];
 #pragma unroll
-    for (int k = 0; k < 7; ++k)
-      pn[k] = (double)(T)((double)cur.r[k] + beta * ((double)cur.p[k] - om * (double)cur.v[k]));
-    const double vn = spmv_pts(cur.P, [&](int k) { return (double)cur.m[k] * pn[k]; });
-    pnew[row] = (T)pn[3];
-    vnew[row] = (T)vn;
-    acc += (double)cur.r0 * (double)(T)vn;
-    if (!NS3D_SOLVE_PIPE && nxt < rr.end) a_fetch<T, L>(S, w, pold, vold, (int)nxt, n, nx);
-    cur = nx;
-    row = nxt;
-  }
+      for (int k = 0; k < 7; ++k)
+        pn[k] = (double)(T)((double)cur.r[k] + beta * ((double)cur.p[k] - om * (double)cur.v[k]));
+      const double vn = spmv_pts(cur.P, [&](int k) { return (double)cur.m[k] * pn[k]; });
+      pnew[row] = (T)pn[3];
+      vnew[row] = (T)vn;
+      acc += (double)cur.r0 * (double)(T)vn;
+      if (!NS3D_SOLVE_PIPE && nxt < tl.end) a_fetch<T, L>(D.S, D.w, pold, vold, (int)nxt, n, nx);
+      cur = nx;
+      row = nxt;
+    }
+  });
+  if (stop) return;
   double v1[1] = {acc};
-  const int ops[1] = {RSUM};
-  block_partials<1>(v1, ops, slot(w.partials, kSlotA));
-}
-
-template <typename T, int L>
-__global__ void __launch_bounds__(kBlock)
-bicg_B_kernel(SMat<T> S, SWork<T> w) {
-  BState* st = w.st;
-  if (st->status != BS_RUNNING) return;
-  const long long it = st->it_a;
-  const int sl = (int)(it & 1);
-  const int par = sl ^ 1;
-  const T* __restrict__ vn = w.v[par];
-  const int64_t n = S.n;
-  const RowIter rr = block_rows(n);
-  int64_t row = rr.begin + threadIdx.x;
-  BIn<T> cur;
-  if (row < rr.end) b_fetch<T, L>(S, w, vn, (int)row, n, cur);
-  double red[1];
-  sum_partials_all<1>(slot(w.partials, kSlotA), gridDim.x, red);
-  const double alpha = st->rho[sl] / red[0];
-  if (leader()) {
-    st->r0v = red[0];
-    st->alpha[sl] = alpha;
-  }
-  double a0 = 0.0, a1 = 0.0;
-  while (row < rr.end) {
-    const int64_t nxt = row + rr.step;
-    BIn<T> nx;
-    if (NS3D_SOLVE_PIPE && nxt < rr.end) b_fetch<T, L>(S, w, vn, (int)nxt, n, nx);
-    double sv[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) sv[k] = (double)(T)((double)cur.r[k] - alpha * (double)cur.v[k]);
-    const double tv = (double)(T)spmv_pts(cur.P, [&](int k) { return (double)cur.m[k] * sv[k]; });
-    w.t[row] = (T)tv;
-    a0 += tv * sv[3];
-    a1 += tv * tv;
-    if (!NS3D_SOLVE_PIPE && nxt < rr.end) b_fetch<T, L>(S, w, vn, (int)nxt, n, nx);
-    cur = nx;
-    row = nxt;
-  }
-  double v2[2] = {a0, a1};
-  const int ops[2] = {RSUM, RSUM};
-  block_partials<2>(v2, ops, slot(w.partials, kSlotB));
+  block_sums_at<1>(v1, D.slot(kSlotA), D.pstride, t - D.t0);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-bicg_C_kernel(int n_, T* __restrict__ x, const T* __restrict__ invdiag, SWork<T> w) {
-  BState* st = w.st;
+bicg_B_kernel(Batch<T> B) {
+  const int t = block_tile(B.ntiles);
+  if (t < 0) return;
+  const Tile tl = uniform_load(B.tiles + t);
+  const GDesc<T> D = load_desc(B.d + tl.solve);
+  NS3D_GLOBAL BState* st = D.w.st;
   if (st->status != BS_RUNNING) return;
+  const bool lead = leader_of(t, D.t0);
   const long long it = st->it_a;
   const int sl = (int)(it & 1);
   const int par = sl ^ 1;
-  const int64_t n = n_;
-  const RowIter rr = block_rows(n);
-  int64_t row = rr.begin + threadIdx.x;
+  const NS3D_GLOBAL T* __restrict__ vn = D.w.v[par];
+  double a0 = 0.0, a1 = 0.0;
+  by_layout(D.S.layout, [&](auto LC) {
+    constexpr int L = decltype(LC)::value;
+    const int64_t n = D.S.n;
+    int64_t row = tl.begin + threadIdx.x;
+    BIn<T> cur;
+    if (row < tl.end) b_fetch<T, L>(D.S, D.w, vn, (int)row, n, cur);
+    double red[1];
+    sum_partials_all<1>(D.slot(kSlotA), D.t1 - D.t0, red, D.pstride);
+    const double alpha = st->rho[sl] / red[0];
+    if (lead) {
+      st->r0v = red[0];
+      st->alpha[sl] = alpha;
+    }
+    while (row < tl.end) {
+      const int64_t nxt = row + kBlock;
+      BIn<T> nx;
+      if (NS3D_SOLVE_PIPE && nxt < tl.end) b_fetch<T, L>(D.S, D.w, vn, (int)nxt, n, nx);
+      double sv[7];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) sv[k] = (double)(T)((double)cur.r[k] - alpha * (double)cur.v[k]);
+      const double tv = (double)(T)spmv_pts(cur.P, [&](int k) { return (double)cur.m[k] * sv[k]; });
+      D.w.t[row] = (T)tv;
+      a0 += tv * sv[3];
+      a1 += tv * tv;
+      if (!NS3D_SOLVE_PIPE && nxt < tl.end) b_fetch<T, L>(D.S, D.w, vn, (int)nxt, n, nx);
+      cur = nx;
+      row = nxt;
+    }
+  });
+  double v2[2] = {a0, a1};
+  block_sums_at<2>(v2, D.slot(kSlotB), D.pstride, t - D.t0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+bicg_C_kernel(Batch<T> B) {
+  const int t = block_tile(B.ntiles);
+  if (t < 0) return;
+  const Tile tl = uniform_load(B.tiles + t);
+  const GDesc<T> D = load_desc(B.d + tl.solve);
+  NS3D_GLOBAL BState* st = D.w.st;
+  if (st->status != BS_RUNNING) return;
+  const bool lead = leader_of(t, D.t0);
+  const long long it = st->it_a;
+  const int sl = (int)(it & 1);
+  const int par = sl ^ 1;
+  NS3D_GLOBAL T* __restrict__ x = D.x;
+  const NS3D_GLOBAL T* __restrict__ invdiag = D.S.invdiag;
+  int64_t row = tl.begin + threadIdx.x;
   CIn<T> cur;
-  if (row < rr.end) c_fetch(invdiag, x, w, par, row, cur);
+  if (row < tl.end) c_fetch<T>(invdiag, x, D.w, par, row, cur);
   double red[2];
-  sum_partials_all<2>(slot(w.partials, kSlotB), gridDim.x, red);
+  sum_partials_all<2>(D.slot(kSlotB), D.t1 - D.t0, red, D.pstride);
   const double om = (red[1] > 0.0) ? red[0] / red[1] : 0.0;
   const double alpha = st->alpha[sl];
-  if (leader()) {
+  if (lead) {
     st->ts = red[0];
     st->tt = red[1];
     st->w[sl] = om;
     st->it_c = it + 1;
   }
   double a0 = 0.0, a1 = 0.0;
-  while (row < rr.end) {
-    const int64_t nxt = row + rr.step;
+  while (row < tl.end) {
+    const int64_t nxt = row + kBlock;
     CIn<T> nx;
-    if (nxt < rr.end) c_fetch(invdiag, x, w, par, nxt, nx);
+    if (nxt < tl.end) c_fetch<T>(invdiag, x, D.w, par, nxt, nx);
     const double md = cur.m;
     const double y = md * (double)cur.p;
     const double s = (double)(T)((double)cur.r - alpha * (double)cur.v);
     const double z = md * s;
     x[row] = (T)((double)cur.x + (alpha * y + om * z));
     const double rn = (double)(T)(s - om * (double)cur.t);
-    w.r[row] = (T)rn;
+    D.w.r[row] = (T)rn;
     a0 += (double)cur.r0 * rn;
     a1 += rn * rn;
     cur = nx;
     row = nxt;
   }
   double v2[2] = {a0, a1};
-  const int ops[2] = {RSUM, RSUM};
-  block_partials<2>(v2, ops, slot(w.partials, kSlotC));
+  block_sums_at<2>(v2, D.slot(kSlotC), D.pstride, t - D.t0);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -498,11 +632,6 @@ __global__ void __launch_bounds__(kBlock) fill_f32_kernel(float* __restrict__ x,
 
 }  // namespace
 
-// Kernel instance for a layout; the layout is a template parameter so the
-// row loop carries no per-row branches on it.
-#define NS3D_BY_LAYOUT(L, KERNEL, T)                                                                       \
-  ((L) == ML_PACKED_SYM ? KERNEL<T, ML_PACKED_SYM> : (L) == ML_PACKED ? KERNEL<T, ML_PACKED> : KERNEL<T, ML_INDEXED>)
-
 // Launch with optional start / stop events taken from the dispatch itself.
 template <typename K, typename... Args>
 hipError_t launch_timed(K kernel, int nb, hipStream_t s, hipEvent_t* ev, Args... args) {
@@ -511,45 +640,62 @@ hipError_t launch_timed(K kernel, int nb, hipStream_t s, hipEvent_t* ev, Args...
   return hipGetLastError();
 }
 
+int tiles_for(int64_t n) {
+  int64_t nt = (n + kTileRows - 1) / kTileRows;
+  if (nt > kMaxTilesPerSolve) nt = kMaxTilesPerSolve;
+  if (nt < 1) nt = 1;
+  return (int)nt;
+}
+
+void make_tiles(int solve, int64_t n, std::vector<Tile>& out, int& t0, int& t1) {
+  const int nt = tiles_for(n);
+  const int64_t per = ((n + nt - 1) / nt + kBlock - 1) / kBlock * kBlock;
+  t0 = (int)out.size();
+  for (int k = 0; k < nt; ++k) {
+    const int64_t b = std::min<int64_t>((int64_t)k * per, n);
+    const int64_t e = std::min<int64_t>(b + per, n);
+    out.push_back(Tile{solve, 0, b, e});
+  }
+  t1 = (int)out.size();
+}
+
+int tile_grid(int ntiles) { return std::max(kXcd, (ntiles + kXcd - 1) / kXcd * kXcd); }
+
 template <typename T>
-hipError_t launch_bicg_init(SMat<T> S, const T* x, const T* b, SWork<T> w, double tol, long long maxit,
-                            hipStream_t s) {
-  const int nb = reduce_blocks(S.n);
-  hipLaunchKernelGGL(NS3D_BY_LAYOUT(S.layout, bicg_init_kernel, T), dim3(nb), dim3(kBlock), 0, s, S, x, b, w);
-  hipLaunchKernelGGL(bicg_init_fin, dim3(1), dim3(kBlock), 0, s, w.partials, w.st, nb, tol, maxit);
+hipError_t launch_bicg_init(Batch<T> B, hipStream_t s) {
+  hipLaunchKernelGGL(bicg_init_kernel<T>, dim3(tile_grid(B.ntiles)), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(bicg_init_fin<T>, dim3(B.nsolves), dim3(kBlock), 0, s, B);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_bicg_restart(SMat<T> S, const T* x, const T* b, SWork<T> w, hipStream_t s) {
-  const int nb = reduce_blocks(S.n);
-  hipLaunchKernelGGL(NS3D_BY_LAYOUT(S.layout, bicg_restart_kernel, T), dim3(nb), dim3(kBlock), 0, s, S, x, b, w);
-  hipLaunchKernelGGL(bicg_restart_fin, dim3(1), dim3(kBlock), 0, s, w.partials, w.st, nb);
+hipError_t launch_bicg_restart(Batch<T> B, hipStream_t s) {
+  hipLaunchKernelGGL(bicg_restart_kernel<T>, dim3(tile_grid(B.ntiles)), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(bicg_restart_fin<T>, dim3(B.nsolves), dim3(kBlock), 0, s, B);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_bicg_A(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t* ev) {
-  return launch_timed(NS3D_BY_LAYOUT(S.layout, bicg_A_kernel, T), reduce_blocks(S.n), s, ev, S, w);
+hipError_t launch_bicg_A(Batch<T> B, hipStream_t s, hipEvent_t* ev) {
+  return launch_timed(bicg_A_kernel<T>, tile_grid(B.ntiles), s, ev, B);
 }
 
 template <typename T>
-hipError_t launch_bicg_B(SMat<T> S, SWork<T> w, hipStream_t s, hipEvent_t* ev) {
-  return launch_timed(NS3D_BY_LAYOUT(S.layout, bicg_B_kernel, T), reduce_blocks(S.n), s, ev, S, w);
+hipError_t launch_bicg_B(Batch<T> B, hipStream_t s, hipEvent_t* ev) {
+  return launch_timed(bicg_B_kernel<T>, tile_grid(B.ntiles), s, ev, B);
 }
 
 template <typename T>
-hipError_t launch_bicg_C(SMat<T> S, T* x, SWork<T> w, hipStream_t s, hipEvent_t* ev) {
-  return launch_timed(bicg_C_kernel<T>, reduce_blocks(S.n), s, ev, S.n, x, S.invdiag, w);
+hipError_t launch_bicg_C(Batch<T> B, hipStream_t s, hipEvent_t* ev) {
+  return launch_timed(bicg_C_kernel<T>, tile_grid(B.ntiles), s, ev, B);
 }
 
-#define NS3D_INSTANTIATE(T)                                                                                 \
-  template hipError_t launch_bicg_init<T>(SMat<T>, const T*, const T*, SWork<T>, double, long long,       \
-                                          hipStream_t);                                                     \
-  template hipError_t launch_bicg_restart<T>(SMat<T>, const T*, const T*, SWork<T>, hipStream_t);           \
-  template hipError_t launch_bicg_A<T>(SMat<T>, SWork<T>, hipStream_t, hipEvent_t*);                       \
-  template hipError_t launch_bicg_B<T>(SMat<T>, SWork<T>, hipStream_t, hipEvent_t*);                       \
-  template hipError_t launch_bicg_C<T>(SMat<T>, T*, SWork<T>, hipStream_t, hipEvent_t*);
+#define NS3D_INSTANTIATE(T)                                                         \
+  template hipError_t launch_bicg_init<T>(Batch<T>, hipStream_t);                  \
+  template hipError_t launch_bicg_restart<T>(Batch<T>, hipStream_t);               \
+  template hipError_t launch_bicg_A<T>(Batch<T>, hipStream_t, hipEvent_t*);        \
+  template hipError_t launch_bicg_B<T>(Batch<T>, hipStream_t, hipEvent_t*);        \
+  template hipError_t launch_bicg_C<T>(Batch<T>, hipStream_t, hipEvent_t*);
 NS3D_INSTANTIATE(double)
 NS3D_INSTANTIATE(float)
 #undef NS3D_INSTANTIATE

@@ -33,6 +33,7 @@ EXPORTS = [
     "ns3d_compute_pressure_correction", "ns3d_compute_pressure_field", "ns3d_correct_velocity",
     "ns3d_correct_pressure", "ns3d_compute_div", "ns3d_compute_permeability", "ns3d_get_matrix",
     "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_set_precision", "ns3d_sync",
+    "ns3d_engine_create", "ns3d_engine_destroy", "ns3d_attach_engine", "ns3d_engine_stats",
 ]
 
 
@@ -70,9 +71,14 @@ def lib():
         L.ns3d_device_count.restype = i32
         for name in EXPORTS:
             f = getattr(L, name)
-            if name not in ("ns3d_last_error", "ns3d_device_count", "ns3d_destroy"):
+            if name not in ("ns3d_last_error", "ns3d_device_count", "ns3d_destroy", "ns3d_engine_destroy"):
                 f.restype = i32
         L.ns3d_destroy.restype = None
+        L.ns3d_engine_destroy.restype = None
+        L.ns3d_engine_create.argtypes = [i32, i32, P(vp)]
+        L.ns3d_engine_destroy.argtypes = [vp]
+        L.ns3d_attach_engine.argtypes = [vp, vp]
+        L.ns3d_engine_stats.argtypes = [vp, P(dbl)]
         L.ns3d_prepare_image.argtypes = [P(C.c_uint8), i32, i32, i32, i32, i32, P(C.c_uint8), P(i32)]
         L.ns3d_create.argtypes = [P(C.c_uint8), i32, i32, i32, i32, P(vp)]
         L.ns3d_destroy.argtypes = [vp]
@@ -286,6 +292,47 @@ class NavierStokesGPU:
         out = np.zeros(4, np.int64)
         _check(lib().ns3d_matrix_nnz(self.h, _p(out, C.c_int64)))
         return [int(x) for x in out]
+
+
+class Engine:
+    """Batched solve engine: contexts attached to it (each driven by its own
+    thread) have their BiCGSTAB solves run together, one launch per kernel
+    phase over all waiting solves."""
+
+    STAT_NAMES = ["chunks", "launches", "solves", "max_active", "rows_x_iters"]
+
+    def __init__(self, device: int = 0, chunk: int = 8):
+        h = C.c_void_p()
+        _check(lib().ns3d_engine_create(device, chunk, C.byref(h)))
+        self.h = h
+
+    def attach(self, ctx):
+        _check(lib().ns3d_attach_engine(ctx, self.h))
+
+    @staticmethod
+    def detach(ctx):
+        _check(lib().ns3d_attach_engine(ctx, None))
+
+    def stats(self):
+        out = np.zeros(32)
+        _check(lib().ns3d_engine_stats(self.h, _p(out, C.c_double)))
+        d = dict(zip(self.STAT_NAMES, out[:5].tolist()))
+        for q, prec in enumerate(("f64", "f32")):
+            for k, name in enumerate("ABC"):
+                o = 5 + 9 * q + 3 * k
+                d[f"{prec}_{name}"] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
+        return d
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ns3d_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def bicgstab_stencil(diag, off, nbr, b, x0, tol, maxit):

@@ -132,6 +132,15 @@ class Solver:
     def timing(self, enable=True):
         _gpu._check(_gpu.lib().ns3d_timing(self.context(), int(enable)))
 
+    def attach(self, engine):
+        """Hands this solver's BiCGSTAB solves to a batched engine (None
+        detaches); run() must then be called from its own thread for solves
+        of several solvers to overlap."""
+        if engine is None:
+            _gpu.Engine.detach(self.context())
+        else:
+            engine.attach(self.context())
+
     def timing_read(self):
         out = np.zeros(36)
         _gpu._check(_gpu.lib().ns3d_timing_read(self.context(), out.ctypes.data_as(C.POINTER(C.c_double))))
@@ -166,6 +175,50 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+def solve_batch(images, params, engine=None, chunk=8):
+    """Solves several images concurrently: one thread per image runs its
+    driver, their BiCGSTAB solves batched on one engine.  params: one dict
+    per image (or one dict for all).  Returns [(rows, summary)] in order;
+    summaries carry the final fields."""
+    import threading
+    if isinstance(params, dict):
+        params = [params] * len(images)
+    own = engine is None
+    eng = _gpu.Engine(device=params[0].get("device", 0) if params else 0, chunk=chunk) if own else engine
+    solvers = [Solver(img, **p) for img, p in zip(images, params)]
+    out = [None] * len(solvers)
+    errs = []
+
+    def work(i):
+        try:
+            rows, summ = solvers[i].run()
+            summ["fields"] = solvers[i].fields()
+            out[i] = (rows, summ)
+        except Exception as e:  # reported after the join
+            errs.append((i, e))
+
+    try:
+        for s in solvers:
+            s.attach(eng)
+        threads = [threading.Thread(target=work, args=(i,)) for i in range(len(solvers))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+    finally:
+        for s in solvers:
+            s.close()
+        if own:
+            stats = eng.stats()
+            eng.close()
+    if errs:
+        raise errs[0][1]
+    if own:
+        for r in out:
+            r[1]["engine"] = stats
+    return out
 
 
 def solve(img, **params):
