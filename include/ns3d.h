@@ -243,6 +243,12 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
  * bytes of those launches. */
 ns3d_status ns3d_engine_stats(ns3d_engine* eng, double out[32]);
 
+/* Upper bound of the device bytes a context of an nx x ny x nz image with
+ * nfluid fluid voxels allocates (mixed != 0: with the mixed form's buffers),
+ * for admission control; ns3d_ctx_bytes gives what a context did request. */
+ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, double* bytes);
+ns3d_status ns3d_ctx_bytes(ns3d_ctx* ctx, double* bytes);
+
 /* Stored entries (diagonal + existing neighbours) of the u, v, w, p systems. */
 ns3d_status ns3d_matrix_nnz(ns3d_ctx* ctx, int64_t out[4]);
 

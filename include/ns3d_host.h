@@ -52,6 +52,9 @@ typedef struct {
                                    p solves, corrections, diagnostics, other */
   int64_t solver_work[4];       /* summed executed BiCGSTAB iterations (a zero
                                    right-hand side counts 0, not maxit) */
+  int32_t fallback;             /* served requests: 1 if the mixed-precision run
+                                   did not converge and this is the FP64 rerun */
+  int32_t pad;
 } ns3d_run_result;
 
 typedef struct ns3d_solver ns3d_solver;
@@ -74,6 +77,35 @@ void ns3d_solver_destroy(ns3d_solver* s);
 /* The GPU context behind a solver (for ns3d_timing / ns3d_matrix_nnz). */
 struct ns3d_ctx* ns3d_solver_context(ns3d_solver* s);
 const char* ns3d_host_last_error(void);
+
+/* ---- serving (no counterpart in the reference) ---------------------------
+ * A server takes permeability requests as they arrive.  max_slots worker
+ * threads run requests concurrently; a queued request starts (in arrival
+ * order) when a slot is free and its projected device footprint
+ * (ns3d_footprint) fits in what budget_bytes has left.  All running requests
+ * share one solve engine (ns3d_engine_create with `chunk`), so their
+ * BiCGSTAB solves run batched and a new request joins at the next chunk
+ * boundary.  fallback != 0: a mixed-precision request that does not
+ * converge is re-run in FP64 (result field `fallback`).  The image is copied
+ * at submission; outdir/name strings too.  times[4] of poll/wait: submit,
+ * start and end time (seconds since the server was created) and the
+ * projected footprint in bytes. */
+typedef struct ns3d_server ns3d_server;
+int ns3d_server_create(int device, double budget_bytes, int max_slots, int chunk, int fallback,
+                       ns3d_server** out);
+/* 0 queued; 1 bad arguments; 2 error; 3 the request alone exceeds the budget. */
+int ns3d_server_submit(ns3d_server* s, const ns3d_run_params* p, const uint8_t* img, int nx, int ny, int nz,
+                       int64_t* ticket);
+/* 0 done (res, times filled); 1 queued or running; -1 unknown ticket; -2
+ * the request failed (message in ns3d_server_last_error, times filled). */
+int ns3d_server_poll(ns3d_server* s, int64_t ticket, ns3d_run_result* res, double times[4]);
+int ns3d_server_wait(ns3d_server* s, int64_t ticket, ns3d_run_result* res, double times[4]);
+/* out[16]: submitted, queued, running, done, failed, fallbacks, bytes in use,
+ * peak bytes in use, peak running, budget, seconds since creation. */
+int ns3d_server_stats(ns3d_server* s, double out[16]);
+/* Finishes every queued and running request, then frees the server. */
+void ns3d_server_destroy(ns3d_server* s);
+const char* ns3d_server_last_error(void);
 
 /* Reads a .ini file with the reference's grammar (ReadInput,
  * navierstokes3D.cxx:266-463); the image path is copied to image (size cap).

@@ -103,6 +103,7 @@ struct ns3d_ctx {
   ns3d_params prm{};
   bool have_params = false;
   std::vector<void*> allocs;
+  double alloc_bytes = 0;   // device bytes requested by this context
   Timing tm;
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> tev;  // start / stop of A, B, C of the sampled iteration
@@ -117,6 +118,7 @@ struct ns3d_ctx {
     hipError_t e = hipMalloc(&ptr, bytes);
     if (e != hipSuccess) throw HipFail{e, "hipMalloc"};
     allocs.push_back(ptr);
+    alloc_bytes += (double)bytes;
     return (T*)ptr;
   }
 };
@@ -935,6 +937,30 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* c, ns3d_engine* e) {
 ns3d_status ns3d_engine_stats(ns3d_engine* e, double out[32]) {
   if (!e || !out) return fail_invalid("ns3d_engine_stats: bad arguments");
   e->stats(out, 32);
+  return NS3D_OK;
+}
+
+ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, double* bytes) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || nfluid < 0 || !bytes) return fail_invalid("ns3d_footprint: bad arguments");
+  const double N = (double)nx * ny * nz, plane = (double)ny * nz, nf = (double)nfluid;
+  // rows: p, v, w <= fluid cells; u <= fluid cells + one outlet plane
+  const double nu = nf + plane, nmax = nu, rows_all = 3 * nf + nu;
+  double b = N + 12 * N + 4 * (N + plane);            // image, index grids
+  b += 2 * 16 * std::ceil(N / kScanCells) + 64;        // scan block counts / offsets
+  b += rows_all * (4 + 24 + 56 + 8 + 8) + 3 * nmax * 8;  // cell, nbr, coef, invdiag, pk; rowsum (u, v, w)
+  b += 4 * kMaxTilesPerSolve * sizeof(Tile);           // tile tables
+  b += 3 * 8 * (3 * nf + plane) + 24 * nf;             // velocity sets, p, pc, b
+  b += (2 + 7) * 8 * nmax;                             // momentum rhs, saved field, BiCGSTAB vectors
+  b += 8.0 * kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks) + 4096;  // partials, state, descriptors
+  if (mixed) b += rows_all * (28 + 4) + nmax * (7 * 4 + 2 * 4 + 8);  // FP32 matrices, inner vectors, residual
+  b += 64.0 * 1024 * 48;                               // allocation rounding (48 allocations at 64 KiB)
+  *bytes = b;
+  return NS3D_OK;
+}
+
+ns3d_status ns3d_ctx_bytes(ns3d_ctx* c, double* bytes) {
+  if (!c || !bytes) return fail_invalid("ns3d_ctx_bytes: bad arguments");
+  *bytes = c->alloc_bytes;
   return NS3D_OK;
 }
 

@@ -34,6 +34,7 @@ EXPORTS = [
     "ns3d_correct_pressure", "ns3d_compute_div", "ns3d_compute_permeability", "ns3d_get_matrix",
     "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_set_precision", "ns3d_sync",
     "ns3d_engine_create", "ns3d_engine_destroy", "ns3d_attach_engine", "ns3d_engine_stats",
+    "ns3d_footprint", "ns3d_ctx_bytes",
 ]
 
 
@@ -79,6 +80,8 @@ def lib():
         L.ns3d_engine_destroy.argtypes = [vp]
         L.ns3d_attach_engine.argtypes = [vp, vp]
         L.ns3d_engine_stats.argtypes = [vp, P(dbl)]
+        L.ns3d_footprint.argtypes = [i32, i32, i32, i64, i32, P(dbl)]
+        L.ns3d_ctx_bytes.argtypes = [vp, P(dbl)]
         L.ns3d_prepare_image.argtypes = [P(C.c_uint8), i32, i32, i32, i32, i32, P(C.c_uint8), P(i32)]
         L.ns3d_create.argtypes = [P(C.c_uint8), i32, i32, i32, i32, P(vp)]
         L.ns3d_destroy.argtypes = [vp]
@@ -285,6 +288,11 @@ class NavierStokesGPU:
                     res[(prec, kind, name)] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
         return res
 
+    def device_bytes(self):
+        out = C.c_double()
+        _check(lib().ns3d_ctx_bytes(self.h, C.byref(out)))
+        return out.value
+
     def set_precision(self, mixed: bool, inner_tol: float = 1e-4):
         _check(lib().ns3d_set_precision(self.h, int(bool(mixed)), inner_tol))
 
@@ -333,6 +341,13 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+def footprint(shape, nfluid, mixed=False):
+    """Projected device bytes of a context (upper bound)."""
+    out = C.c_double()
+    _check(lib().ns3d_footprint(*[int(x) for x in shape], int(nfluid), int(bool(mixed)), C.byref(out)))
+    return out.value
 
 
 def bicgstab_stencil(diag, off, nbr, b, x0, tol, maxit):

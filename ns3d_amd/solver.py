@@ -17,7 +17,8 @@ METHODS = {"SIMPLEC": 0, "SIMPLER": 1, "SIMPLERR": 2, "IDEAL": 3}
 
 EXPORTS = ["ns3d_default_run_params", "ns3d_solver_create", "ns3d_solver_info", "ns3d_solver_run",
            "ns3d_solver_fields", "ns3d_solver_destroy", "ns3d_solver_context", "ns3d_host_last_error",
-           "ns3d_read_ini"]
+           "ns3d_read_ini", "ns3d_server_create", "ns3d_server_submit", "ns3d_server_poll", "ns3d_server_wait",
+           "ns3d_server_stats", "ns3d_server_destroy", "ns3d_server_last_error"]
 
 
 class RunParams(C.Structure):
@@ -38,7 +39,8 @@ class RunResult(C.Structure):
                 ("v_moy", C.c_double), ("RMR", C.c_double), ("normdiv", C.c_double), ("reldiv", C.c_double),
                 ("maxdiv", C.c_double), ("seconds", C.c_double), ("np", C.c_int64), ("nu", C.c_int64),
                 ("nv", C.c_int64), ("nw", C.c_int64), ("solver_iterations", C.c_int64 * 4),
-                ("phase_seconds", C.c_double * 8), ("solver_work", C.c_int64 * 4)]
+                ("phase_seconds", C.c_double * 8), ("solver_work", C.c_int64 * 4), ("fallback", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 PHASES = ["momentum_assembly", "velocity_solves", "pseudo_velocities", "pressure_assembly",
@@ -72,6 +74,19 @@ def lib():
         L.ns3d_host_last_error.restype = C.c_char_p
         L.ns3d_read_ini.argtypes = [C.c_char_p, P(RunParams), C.c_char_p, C.c_int]
         L.ns3d_read_ini.restype = C.c_int
+        vp = C.c_void_p
+        L.ns3d_server_create.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, P(vp)]
+        L.ns3d_server_create.restype = C.c_int
+        L.ns3d_server_submit.argtypes = [vp, P(RunParams), P(C.c_uint8), C.c_int, C.c_int, C.c_int, P(C.c_int64)]
+        L.ns3d_server_submit.restype = C.c_int
+        for f in (L.ns3d_server_poll, L.ns3d_server_wait):
+            f.argtypes = [vp, C.c_int64, P(RunResult), P(C.c_double)]
+            f.restype = C.c_int
+        L.ns3d_server_stats.argtypes = [vp, P(C.c_double)]
+        L.ns3d_server_stats.restype = C.c_int
+        L.ns3d_server_destroy.argtypes = [vp]
+        L.ns3d_server_destroy.restype = None
+        L.ns3d_server_last_error.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -119,7 +134,7 @@ class Solver:
         if rc != 0:
             raise _gpu.NS3DError(lib().ns3d_host_last_error().decode())
         summary = {f: getattr(res, f) for f, _ in RunResult._fields_
-                   if f not in ("solver_iterations", "phase_seconds", "solver_work")}
+                   if f not in ("solver_iterations", "phase_seconds", "solver_work", "pad")}
         summary["solver_iterations"] = list(res.solver_iterations)
         summary["solver_work"] = list(res.solver_work)
         summary["phases"] = dict(zip(PHASES, list(res.phase_seconds)))
@@ -168,6 +183,81 @@ class Solver:
     def close(self):
         if getattr(self, "h", None):
             lib().ns3d_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _summary(res):
+    summary = {f: getattr(res, f) for f, _ in RunResult._fields_
+               if f not in ("solver_iterations", "phase_seconds", "solver_work", "pad")}
+    summary["solver_iterations"] = list(res.solver_iterations)
+    summary["solver_work"] = list(res.solver_work)
+    summary["phases"] = dict(zip(PHASES, list(res.phase_seconds)))
+    return summary
+
+
+class Server:
+    """Permeability requests served as they arrive (native server of
+    libns3d_host.so): worker slots admit requests by projected device
+    footprint within budget_bytes, all running requests share one solve
+    engine, and a mixed-precision request that does not converge is re-run in
+    FP64 when fallback is on."""
+
+    STAT_NAMES = ["submitted", "queued", "running", "done", "failed", "fallbacks", "bytes_in_use",
+                  "peak_bytes_in_use", "peak_running", "budget", "seconds"]
+
+    def __init__(self, device=0, budget_bytes=200e9, max_slots=16, chunk=8, fallback=True):
+        h = C.c_void_p()
+        if lib().ns3d_server_create(device, float(budget_bytes), int(max_slots), int(chunk), int(bool(fallback)),
+                                    C.byref(h)) != 0:
+            raise _gpu.NS3DError(lib().ns3d_server_last_error().decode())
+        self.h = h
+        self._keep = {}
+
+    def submit(self, img, **params):
+        """Queues one request; returns its ticket."""
+        p = default_params(**params)
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        t = C.c_int64()
+        rc = lib().ns3d_server_submit(self.h, C.byref(p), img.ctypes.data_as(C.POINTER(C.c_uint8)), *img.shape,
+                                      C.byref(t))
+        if rc != 0:
+            raise _gpu.NS3DError(lib().ns3d_server_last_error().decode())
+        return int(t.value)
+
+    def _fetch(self, fn, ticket):
+        res = RunResult()
+        times = (C.c_double * 4)()
+        rc = fn(self.h, ticket, C.byref(res), times)
+        if rc == 1:
+            return None
+        if rc < 0:
+            raise _gpu.NS3DError(f"request {ticket}: " + lib().ns3d_server_last_error().decode())
+        summ = _summary(res)
+        summ.update(t_submit=times[0], t_start=times[1], t_end=times[2], footprint=times[3])
+        return summ
+
+    def poll(self, ticket):
+        """The request's summary (with t_submit, t_start, t_end, footprint)
+        once done, else None."""
+        return self._fetch(lib().ns3d_server_poll, ticket)
+
+    def wait(self, ticket):
+        return self._fetch(lib().ns3d_server_wait, ticket)
+
+    def stats(self):
+        out = (C.c_double * 16)()
+        lib().ns3d_server_stats(self.h, out)
+        return dict(zip(self.STAT_NAMES, list(out)[:11]))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ns3d_server_destroy(self.h)
             self.h = None
 
     def __del__(self):
