@@ -1,20 +1,37 @@
 #!/usr/bin/env python3
-"""Permeability solves/s of the SIMPLE-family outer iteration on one MI355X.
+"""Permeability solves/s and latency of the SIMPLE-family outer iteration on
+one MI355X: the served, batched mixed-precision form against the plain FP64
+GPU path on the same request mix.
 
-Workload (one step = one complete permeability solve): a 128^3 seeded
-random-sequential-addition pack of non-overlapping spheres of radius 10 voxels
-at porosity 0.75, flow along x, unit pressure drop, mu = rho = 1, voxel size 1,
-SIMPLEC to an outer tolerance of 1e-6 on the relative mass residual, inner
-tolerances 1e-10 (velocity, pressure) and 1e-6 (pressure correction).
+Headline workload ("mix", one step = the whole mix served once): a seeded
+request mix (ns3d_amd.mix: per image, edges per axis from {32, 48, 64, 96},
+porosity uniform in [0.35, 0.90], generators split among monodisperse RSA
+sphere packs, polydisperse Boolean spheres and random fibres; images whose
+fluid does not percolate along x are redrawn), SIMPLEC, flow along x, unit
+pressure drop, mu = rho = 1, voxel size 1, outer tolerance 1e-6 on the
+relative mass residual (at most 1000 outer iterations), inner tolerances 1e-10
+(velocity, pressure) and 1e-6 (pressure correction).  Forms, all on the same
+requests:
+  plain_fp64    one request at a time on its own context (the drop-in path),
+                contexts created (images uploaded, index scan done) before the
+                timed region;
+  served_fp64   the native server: all requests submitted at t = 0, worker
+                slots share one solve engine (batched BiCGSTAB), FP64;
+  served_mixed  the same with the mixed form (FP64 defect correction around
+                FP32-storage BiCGSTAB), FP64 fallback on non-convergence --
+                the headline value, timed over `steps` passes;
+  served_mixed_poisson  the same requests arriving as a Poisson stream at a
+                fraction of the measured burst throughput, for latency under
+                load.
+Served forms include each request's upload, index scan and teardown inside
+the timed region (that is the service); plain_fp64 does not.  Every served
+permeability is compared with the plain FP64 one (|dK|/K).
 
-Two forms are timed on the same image: the plain FP64 GPU path and the mixed
-form (FP64 defect correction around FP32-storage BiCGSTAB sweeps).  The
-headline value is the mixed form's solves/s when its permeability agrees with
-the FP64 run's to |dK|/K <= 1e-4, else the FP64 form's.
+Second section ("single"): one 128^3 RSA pack (r = 10, porosity 0.75), plain
+FP64 and mixed, with per-kernel timings from dispatch events -- the roofline
+of the dominant FP64 kernel, and the CPU baseline's image.
 
-The images are generated and uploaded (index scan included) before the timed
-region; the timed region is K back-to-back solves on device-resident inputs.
-Output: one JSON line (see the keys below); progress goes to stderr.
+Output: one JSON line; progress on stderr.
 """
 from __future__ import annotations
 
@@ -31,67 +48,154 @@ sys.path.insert(0, ROOT)
 
 METRIC = "permeability solves/s + p50/p99 latency, mixed vs FP64 GPU at |Δk|/k≤1e-4"
 PEAK_HBM_GBS = 8000.0
+TOLS = dict(vsolvertol=1e-10, psolvertol=1e-10, pcsolvertol=1e-6, maxitsolver=5000)
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def workload(args):
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
+
+
+# ---------------------------------------------------------------- the mix
+def mix_params(img, args, precision):
+    nx = img.shape[0]
+    return dict(dx=1.0, mu=1.0, rho=1.0, p0=1.0, gradp=1.0 / nx, alpha=0.8, beta=1.0, direction=1, border=0,
+                method="SIMPLEC", simpletol=args.simpletol, maxitSIMPLE=args.mix_maxit, N1=4, N2=4, device=0,
+                precision=precision, inner_tol=args.inner_tol, **TOLS)
+
+
+def served_pass(reqs, args, precision, label, arrivals=None):
+    """All requests through one native server; arrivals: submission times
+    (seconds from the start) or None for a burst at t = 0."""
+    from ns3d_amd import solver
+    srv = solver.Server(device=0, budget_bytes=args.budget_gb * 1e9, max_slots=args.slots, chunk=args.chunk,
+                        fallback=True)
+    try:
+        t0 = time.perf_counter()
+        tickets = []
+        for q, (img, _) in enumerate(reqs):
+            if arrivals is not None:
+                dt = arrivals[q] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+            tickets.append(srv.submit(img, **mix_params(img, args, precision)))
+        out = [srv.wait(t) for t in tickets]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        est = srv.engine_stats()
+    finally:
+        srv.close()
+    lat = [r["t_end"] - r["t_submit"] for r in out]
+    log(f"  [{label}] {len(reqs)} requests in {wall:.2f} s ({len(reqs) / wall:.3f} solves/s), latency p50 "
+        f"{pct(lat, 50):.2f} s p99 {pct(lat, 99):.2f} s, peak running {st['peak_running']:.0f}, "
+        f"fallbacks {st['fallbacks']:.0f}")
+    return dict(wall=wall, results=out, latency=lat, stats=st, engine=est)
+
+
+def plain_pass(reqs, args):
+    """The drop-in path: contexts created up front, then one solve at a time."""
+    from ns3d_amd import solver
+    solvers = [solver.Solver(img, **mix_params(img, args, 0)) for img, _ in reqs]
+    out, lat = [], []
+    t0 = time.perf_counter()
+    for s in solvers:
+        t1 = time.perf_counter()
+        out.append(s.run()[1])
+        lat.append(time.perf_counter() - t1)
+    wall = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    log(f"  [plain_fp64] {len(reqs)} requests in {wall:.2f} s ({len(reqs) / wall:.3f} solves/s)")
+    return dict(wall=wall, results=out, latency=lat)
+
+
+def engine_roofline(est, prec="f32", k="A"):
+    ms, launches, nbytes = est[f"{prec}_{k}"]
+    if not launches or ms <= 0:
+        return None
+    ach = nbytes / (ms * 1e-3) / 1e9
+    return dict(avg_launch_us=round(1e3 * ms / launches, 2), sampled_launches=int(launches),
+                bytes_per_launch=round(nbytes / launches), achieved=round(ach, 1), frac=round(ach / PEAK_HBM_GBS, 4))
+
+
+def mix_section(args):
+    from ns3d_amd import mix
+    log(f"generating the request mix ({args.requests} requests, seed {args.mix_seed})")
+    reqs = mix.request_mix(args.requests, args.mix_seed, tuple(int(e) for e in args.edges.split(",")))
+    fluid = sum(d["fluid"] for _, d in reqs)
+    for w in range(args.warmup):
+        log(f"warmup pass {w} (served mixed)")
+        served_pass(reqs, args, 1, "warmup")
+    log("plain FP64 path")
+    plain = plain_pass(reqs, args)
+    log("served FP64")
+    s64 = served_pass(reqs, args, 0, "served_fp64")
+    log(f"served mixed: {args.steps} timed pass(es)")
+    smx = [served_pass(reqs, args, 1, f"served_mixed step {q}") for q in range(args.steps)]
+    rate = args.poisson_frac * args.requests * args.steps / sum(p["wall"] for p in smx)
+    rng = np.random.default_rng(args.mix_seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs))) - 0.0
+    arrivals[0] = 0.0
+    log(f"served mixed, Poisson arrivals at {rate:.3f} requests/s")
+    spo = served_pass(reqs, args, 1, "served_mixed_poisson", arrivals=arrivals)
+    k64 = np.array([r["K"] for r in plain["results"]])
+    dk = lambda res: [abs(r["K"] / k - 1) if k != 0 else float("inf") for r, k in zip(res, k64)]  # noqa: E731
+    dk_mixed = dk(smx[-1]["results"])
+    dk_s64 = dk(s64["results"])
+    n = len(reqs)
+    mixed_wall = sum(p["wall"] for p in smx) / len(smx)
+    lat_m = [x for p in smx for x in p["latency"]]
+
+    def form(wall, lat, extra=None):
+        d = dict(solves_per_s=round(n / wall, 4), ms_per_pass=round(1e3 * wall, 1),
+                 latency_s=dict(p50=round(pct(lat, 50), 3), p99=round(pct(lat, 99), 3)))
+        if extra:
+            d.update(extra)
+        return d
+
+    forms = {
+        "plain_fp64": form(plain["wall"], plain["latency"], dict(
+            converged=int(sum(r["converged"] for r in plain["results"])),
+            outer_iterations=int(sum(r["iterations"] for r in plain["results"])))),
+        "served_fp64": form(s64["wall"], s64["latency"], dict(max_dK_over_K=max(dk_s64),
+                                                              peak_running=s64["stats"]["peak_running"])),
+        "served_mixed": form(mixed_wall, lat_m, dict(
+            max_dK_over_K=max(dk_mixed), median_dK_over_K=float(np.median(dk_mixed)),
+            fallbacks=int(sum(r["fallback"] for r in smx[-1]["results"])),
+            converged=int(sum(r["converged"] for r in smx[-1]["results"])),
+            peak_running=smx[-1]["stats"]["peak_running"],
+            peak_device_bytes=smx[-1]["stats"]["peak_bytes_in_use"],
+            engine_A_f32=engine_roofline(smx[-1]["engine"], "f32", "A"))),
+        "served_mixed_poisson": form(spo["wall"], spo["latency"], dict(arrival_rate=round(rate, 4))),
+    }
+    desc = dict(workload=f"request mix: {n} seeded images (edges {args.edges} per axis, porosity U[0.35,0.90], "
+                         f"RSA / Boolean / fibres), SIMPLEC, flow x, simpletol {args.simpletol:g}, at most "
+                         f"{args.mix_maxit} outer iterations",
+                requests=n, fluid_cells=int(fluid), mix_seed=args.mix_seed, slots=args.slots, chunk=args.chunk,
+                images=[dict(shape=d["shape"], generator=d["generator"], porosity=d["porosity"]) for _, d in reqs])
+    return dict(forms=forms, desc=desc, reqs=reqs, plain=plain, mixed_wall=mixed_wall, lat=lat_m,
+                max_dk=max(dk_mixed), engine=smx[-1]["engine"])
+
+
+# ---------------------------------------------------------------- single image
+def single_workload(args):
     from ns3d_amd import generators
     n = args.size
     img, solid, overlap, nsph = generators.rsa_spheres((n, n, n), args.radius, 1.0 - args.porosity, args.seed)
-    params = dict(dx=1.0, mu=1.0, rho=1.0, p0=1.0, gradp=1.0 / n, alpha=0.8, beta=1.0, vsolvertol=1e-10,
-                  psolvertol=1e-10, pcsolvertol=1e-6, maxitsolver=5000, direction=1, border=0)
-    desc = dict(workload=f"{n}^3 RSA sphere pack r={args.radius:g} porosity {1 - solid:.4f} (target "
-                         f"{args.porosity}), {args.method}, flow x, simpletol {args.simpletol:g}",
+    params = dict(dx=1.0, mu=1.0, rho=1.0, p0=1.0, gradp=1.0 / n, alpha=0.8, beta=1.0, direction=1, border=0,
+                  **TOLS)
+    desc = dict(workload=f"{n}^3 RSA sphere pack r={args.radius:g} porosity {1 - solid:.4f}, SIMPLEC, flow x",
                 image=dict(shape=[n, n, n], radius=args.radius, seed=args.seed, porosity=round(1 - solid, 6),
-                           rsa_overlap_voxels=overlap, spheres=nsph),
-                method=args.method, simpletol=args.simpletol, inner_tol=[1e-10, 1e-10, 1e-6])
+                           rsa_overlap_voxels=overlap, spheres=nsph))
     return img, params, desc
 
 
-def cpu_baseline(img, params, args, gpu_work):
-    """The oracle (CPU restatement, OpenMP SpMV/sweeps, serial vector ops) on
-    the same image: a bounded sample of outer iterations, extrapolated to a
-    whole solve by the FP64 GPU run's executed BiCGSTAB iteration count (the
-    work of one iteration is the same on both sides; zero right-hand sides,
-    which Eigen reports as maxit iterations, execute none and count 0)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib
-    o = oracle_lib.Oracle(img, **{k: v for k, v in params.items() if k not in ("direction",)})
-    t0 = time.perf_counter()
-    rows, summ = o.solve(args.method, args.simpletol, args.cpu_iters, 4, 4)
-    dt = time.perf_counter() - t0
-    cpu_work = summ["work"]
-    per_iter = dt / max(cpu_work, 1)
-    est = per_iter * sum(gpu_work)
-    threads = oracle_lib.lib().orc_num_threads()
-    return dict(value=1.0 / est if est > 0 else None, unit="solves/s", cores=int(threads), kind="port",
-                sample=f"{args.cpu_iters} outer iterations of the same {args.method} solve on the same image "
-                       f"({cpu_work} executed BiCGSTAB iterations, {dt:.1f} s), extrapolated to the FP64 GPU "
-                       f"run's {sum(gpu_work)} executed BiCGSTAB iterations per solve",
-                sample_seconds=round(dt, 3))
-
-
-def measured_traffic(args, kernel):
-    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
-    kernel) recorded with rocprofv3 on this workload, from
-    profiles/hbm_traffic_128rsa.json; None for another workload."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_128rsa.json")
-    default = (args.size, args.radius, args.porosity, args.seed) == (128, 10.0, 0.75, 20261203)
-    if not default or not os.path.exists(path):
-        return None, None
-    k = json.load(open(path))["kernels"].get(kernel)
-    if not k:
-        return None, None
-    return k["fetch_bytes"] + k["write_bytes"], "profiles/hbm_traffic_128rsa.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
-
-
 def kernel_table(solvers):
-    """Per-kernel averages over the timed solves from the in-library HIP
-    events taken from the dispatches of one sampled BiCGSTAB iteration per
-    queued chunk of iterations, on the solver's own stream."""
+    """Per-kernel averages from the in-library events taken from the
+    dispatches of one sampled BiCGSTAB iteration per queued chunk."""
     tot = {}
     for s in solvers:
         for key, (ms, launches, nbytes) in s.timing_read().items():
@@ -106,40 +210,127 @@ def kernel_table(solvers):
     return tot, kernels
 
 
-def run_form(img, run_kw, args, precision, label):
-    """Warmup, then K timed back-to-back solves on pre-created solvers."""
+def single_form(img, params, args, precision, label, steps, warmup):
     from ns3d_amd import gpu, solver
-    kw = dict(run_kw, precision=precision, inner_tol=args.inner_tol)
-    log(f"[{label}] warmup: {args.warmup} solve(s)")
-    for w in range(args.warmup):
+    kw = dict(params, method="SIMPLEC", simpletol=args.simpletol, maxitSIMPLE=args.maxit, N1=4, N2=4, device=0,
+              precision=precision, inner_tol=args.inner_tol, verbose=2 if args.progress else 0)
+    for w in range(warmup):
         s = solver.Solver(img, **kw)
-        rows, summ = s.run()
-        log(f"  warmup {w}: {summ['iterations']} outer its, K={summ['K']:.10e}, {summ['seconds']:.2f} s")
+        summ = s.run()[1]
+        log(f"  [{label}] warmup: {summ['iterations']} outer its, {summ['seconds']:.2f} s")
         s.close()
-    log(f"[{label}] uploading {args.steps} image(s)")
-    solvers = [solver.Solver(img, **kw) for _ in range(args.steps)]
+    solvers = [solver.Solver(img, **kw) for _ in range(steps)]
     if not args.no_timing:
         for s in solvers:
             s.timing(True)
     for s in solvers:
         gpu._check(gpu.lib().ns3d_sync(s.context()))
-    lat, summs = [], []
-    t_start = time.perf_counter()
-    for q, s in enumerate(solvers):
-        t0 = time.perf_counter()
-        rows, summ = s.run()
-        lat.append(time.perf_counter() - t0)
-        summs.append(summ)
-        log(f"  step {q}: {summ['iterations']} outer its, converged={summ['converged']}, K={summ['K']:.10e}, "
-            f"{lat[-1]:.2f} s")
-    wall = time.perf_counter() - t_start
+    summs, lat = [], []
+    t0 = time.perf_counter()
+    for s in solvers:
+        t1 = time.perf_counter()
+        summs.append(s.run()[1])
+        lat.append(time.perf_counter() - t1)
+        log(f"  [{label}] {summs[-1]['iterations']} outer its, K={summs[-1]['K']:.10e}, {lat[-1]:.2f} s")
+    wall = time.perf_counter() - t0
     tot, kernels = ({}, {}) if args.no_timing else kernel_table(solvers)
     info = dict(nnz=solvers[0].nnz(), counts=(solvers[0].np, solvers[0].nu, solvers[0].nv, solvers[0].nw))
     for s in solvers:
         s.close()
-    lat_sorted = sorted(lat)
-    return dict(value=args.steps / wall, wall=wall, p50=float(np.percentile(lat_sorted, 50)),
-                p99=float(np.percentile(lat_sorted, 99)), summs=summs, tot=tot, kernels=kernels, **info)
+    return dict(value=steps / wall, wall=wall, lat=lat, summs=summs, tot=tot, kernels=kernels, **info)
+
+
+def measured_traffic(args, kernel):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
+    kernel) recorded with rocprofv3 on the 128^3 image, from
+    profiles/hbm_traffic_128rsa.json; None for another image."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_128rsa.json")
+    default = (args.size, args.radius, args.porosity, args.seed) == (128, 10.0, 0.75, 20261203)
+    if not default or not os.path.exists(path):
+        return None, None
+    k = json.load(open(path))["kernels"].get(kernel)
+    if not k:
+        return None, None
+    return k["fetch_bytes"] + k["write_bytes"], "profiles/hbm_traffic_128rsa.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
+
+
+def mix_traffic(args):
+    """HBM bytes per launch of the batched A kernel in the served mixed pass
+    (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy kernel, averaged over
+    its working launches), from profiles/hbm_traffic_mix.json; None for
+    another mix."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_mix.json")
+    if not os.path.exists(path):
+        return None, None
+    d = json.load(open(path))
+    if (d.get("requests"), d.get("mix_seed"), d.get("edges")) != (args.requests, args.mix_seed, args.edges):
+        return None, None
+    k = d["kernels"].get("bicg_A_kernel<float>")
+    if not k:
+        return None, None
+    return k["fetch_bytes"] + k["write_bytes"], "profiles/hbm_traffic_mix.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
+
+
+def single_section(args):
+    img, params, desc = single_workload(args)
+    log(f"single image: {desc['workload']}")
+    f64 = single_form(img, params, args, 0, "single fp64", args.single_steps, 1)
+    mx = single_form(img, params, args, 1, "single mixed", args.single_steps, 0)
+    roof = None
+    if ("f64", "pressure", "A") in f64["tot"]:
+        ms, launches, nbytes = f64["tot"][("f64", "pressure", "A")]
+        if launches:
+            ach = nbytes / (ms * 1e-3) / 1e9
+            traffic, tsrc = measured_traffic(args, "bicg_A_kernel<double, 2>")
+            roof = dict(bound="hbm", achieved=round(ach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                        frac=round(ach / PEAK_HBM_GBS, 4), traffic=traffic,
+                        kernel="bicg_A_kernel<double> (pressure system, packed symmetric storage): p update + "
+                               "7-point SpMV + r0.v, 128^3 image, plain FP64 path",
+                        bytes_per_launch=round(nbytes / launches), avg_launch_us=round(1e3 * ms / launches, 3),
+                        traffic_source=tsrc)
+    kernels = dict(f64["kernels"])
+    kernels.update(mx["kernels"])
+    s0 = f64["summs"][0]
+    out = dict(desc, n_p=f64["counts"][0], nnz=dict(zip("uvwp", f64["nnz"])),
+               fp64=dict(solves_per_s=round(f64["value"], 5), ms_per_solve=round(1e3 * f64["wall"] / len(f64["lat"]), 1),
+                         K=s0["K"], outer_iterations=s0["iterations"],
+                         bicgstab_iterations=dict(zip("uvwp", s0["solver_work"])),
+                         phases_s={k: round(v, 4) for k, v in s0["phases"].items()}),
+               mixed=dict(solves_per_s=round(mx["value"], 5), ms_per_solve=round(1e3 * mx["wall"] / len(mx["lat"]), 1),
+                          K=mx["summs"][0]["K"], dK_over_K_vs_fp64=abs(mx["summs"][0]["K"] / s0["K"] - 1),
+                          outer_iterations=mx["summs"][0]["iterations"]),
+               fluid_cell_updates_per_s_fp64=round(f64["counts"][0] * s0["iterations"] / (f64["wall"] / len(f64["lat"])), 1))
+    return dict(out=out, roof=roof, kernels=kernels, img=img, params=params, work=s0["solver_work"])
+
+
+# ---------------------------------------------------------------- CPU baseline
+def cpu_baseline_mix(mixres, args):
+    """The oracle (CPU restatement, OpenMP SpMV and sweeps) on the largest
+    request of the mix for a bounded number of outer iterations; its seconds per row x
+    executed BiCGSTAB iteration, times the plain FP64 GPU run's rows x
+    executed iterations summed over the mix, gives the CPU time of the whole
+    mix."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    reqs, plain = mixres["reqs"], mixres["plain"]["results"]
+    q = max(range(len(reqs)), key=lambda k: reqs[k][1]["fluid"])
+    img = reqs[q][0]
+    p = mix_params(img, args, 0)
+    o = oracle_lib.Oracle(img, **{k: p[k] for k in ("dx", "mu", "rho", "p0", "gradp", "alpha", "beta",
+                                                       "vsolvertol", "psolvertol", "pcsolvertol", "maxitsolver")})
+    t0 = time.perf_counter()
+    summ = o.solve("SIMPLEC", args.simpletol, args.cpu_iters, 4, 4)[1]
+    dt = time.perf_counter() - t0
+    rate = dt / max(1.0, float(o.np) * summ["work"])
+    units = sum(r["np"] * w for r in plain for w in [sum(r["solver_work"])])
+    est = rate * units
+    threads = oracle_lib.lib().orc_num_threads()
+    return dict(value=round(len(reqs) / est, 6) if est > 0 else None, unit="solves/s", cores=int(threads),
+                kind="port",
+                sample=f"{args.cpu_iters} outer iterations of request {q} ({reqs[q][1]['shape']}, "
+                       f"{summ['work']} executed BiCGSTAB iterations, {dt:.1f} s), scaled per row x executed "
+                       f"iteration to the plain FP64 GPU run's {units:.3e} rows x iterations over the mix",
+                sample_seconds=round(dt, 3))
 
 
 def copy_calibration(mib):
@@ -159,21 +350,31 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1, help="timed served-mixed passes over the mix")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed served-mixed passes over the mix")
+    ap.add_argument("--section", default="all", choices=["all", "mix", "single"])
+    ap.add_argument("--requests", type=int, default=24)
+    ap.add_argument("--mix-seed", type=int, default=20261203)
+    ap.add_argument("--edges", default="32,48,64,96")
+    ap.add_argument("--slots", type=int, default=32)
+    ap.add_argument("--chunk", type=int, default=8)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--poisson-frac", type=float, default=0.5)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--radius", type=float, default=10.0)
     ap.add_argument("--porosity", type=float, default=0.75)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--method", default="SIMPLEC")
+    ap.add_argument("--single-steps", type=int, default=1)
     ap.add_argument("--simpletol", type=float, default=1e-6)
-    ap.add_argument("--maxit", type=int, default=3000)
+    ap.add_argument("--maxit", type=int, default=3000, help="outer-iteration cap, single image")
+    ap.add_argument("--mix-maxit", type=int, default=1000,
+                    help="outer-iteration cap of the mix (its requests converge in <= 440; one stalls at "
+                         "RMR ~4e-6 in every precision and runs to the cap)")
     ap.add_argument("--inner-tol", type=float, default=1e-4, help="mixed form: FP32 sweep tolerance")
-    ap.add_argument("--forms", default="fp64,mixed")
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events (profiler runs)")
-    ap.add_argument("--progress", action="store_true", help="one stderr line per outer iteration")
+    ap.add_argument("--progress", action="store_true", help="one stderr line per outer iteration (single)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scales HBM byte counters under a profiler)")
     args = ap.parse_args()
@@ -184,75 +385,59 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part of "
               f"this build", file=sys.stderr)
         sys.exit(2)
-
     from ns3d_amd import gpu
     if gpu.device_count() < 1:
         raise SystemExit("no GPU visible")
-    log("generating workload")
-    img, params, desc = workload(args)
-    run_kw = dict(params, method=args.method, simpletol=args.simpletol, maxitSIMPLE=args.maxit, N1=4, N2=4,
-                  device=0, verbose=2 if args.progress else 0)
-    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
-    res = {}
-    for f in forms:
-        if f not in ("fp64", "mixed"):
-            raise SystemExit(f"unknown form {f}")
-        res[f] = run_form(img, run_kw, args, 1 if f == "mixed" else 0, f)
 
-    base = res.get("fp64") or res[forms[0]]
-    np_, nu, nv, nw = base["counts"]
-    nnz = base["nnz"]
-    s0 = base["summs"][0]
-    roof = None
-    if not args.no_timing and ("f64", "pressure", "A") in base["tot"]:
-        ms, launches, nbytes = base["tot"][("f64", "pressure", "A")]
-        if launches:
-            achieved = nbytes / (ms * 1e-3) / 1e9
-            traffic, tsrc = measured_traffic(args, "bicg_A_kernel<double, 2>")
-            roof = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                        frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic,
-                        kernel="bicg_A_kernel<double, 2> (pressure system): p update + 7-point SpMV + r0.v",
-                        bytes_per_launch=round(nbytes / launches), avg_launch_us=round(1e3 * ms / launches, 3),
-                        traffic_source=tsrc)
-
-    forms_out = {}
-    for f, r in res.items():
-        sm = r["summs"][0]
-        forms_out[f] = dict(solves_per_s=round(r["value"], 5), ms_per_solve=round(1e3 * r["wall"] / args.steps, 2),
-                            latency_s=dict(p50=round(r["p50"], 4), p99=round(r["p99"], 4)), K=sm["K"],
-                            outer_iterations=sm["iterations"], converged=sm["converged"],
-                            bicgstab_iterations=dict(zip("uvwp", sm["solver_work"])),
-                            phases_s={k: round(v, 4) for k, v in sm["phases"].items()})
-    headline, form_name, dtype = base, "fp64", "f64"
-    if "mixed" in res and "fp64" in res:
-        dk = abs(res["mixed"]["summs"][0]["K"] / res["fp64"]["summs"][0]["K"] - 1)
-        forms_out["mixed"]["dK_over_K_vs_fp64"] = dk
-        forms_out["mixed"]["speedup_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 4)
-        if dk <= 1e-4 and res["mixed"]["summs"][0]["converged"]:
-            headline, form_name, dtype = res["mixed"], "mixed", "f64 arithmetic, f32 inner-solve storage"
-    elif forms == ["mixed"]:
-        headline, form_name, dtype = res["mixed"], "mixed", "f64 arithmetic, f32 inner-solve storage"
+    mixres = mix_section(args) if args.section in ("all", "mix") else None
+    single = single_section(args) if args.section in ("all", "single") else None
 
     cpu = None
-    if not args.no_cpu:
-        log(f"CPU baseline: {args.cpu_iters} outer iterations of the oracle")
-        cpu = cpu_baseline(img, params, args, s0["solver_work"])
-        log(f"  CPU {cpu['value']:.5f} solves/s ({cpu['sample']})")
-    kernels = {}
-    for r in res.values():
-        kernels.update(r["kernels"])
-    line = {
-        "metric": METRIC, "value": round(headline["value"], 5), "unit": "solves/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * headline["wall"] / args.steps, 2),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
-        "form": form_name,
-        "config": dict(desc, n_p=np_, n_u=nu, n_v=nv, n_w=nw, nnz=dict(zip("uvwp", nnz))),
-        "latency_s": dict(p50=round(headline["p50"], 4), p99=round(headline["p99"], 4)),
-        "forms": forms_out,
-        "roofline": roof,
-        "kernels": kernels,
-        "cpu_baseline": cpu,
-    }
+    if not args.no_cpu and mixres is not None:
+        log(f"CPU baseline: oracle, {args.cpu_iters} outer iterations of one request")
+        cpu = cpu_baseline_mix(mixres, args)
+        log(f"  CPU {cpu['value']} solves/s ({cpu['sample']})")
+
+    if mixres is not None:
+        f = mixres["forms"]
+        value = f["served_mixed"]["solves_per_s"]
+        line = {
+            "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(1e3 * mixres["mixed_wall"], 1), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64 arithmetic, f32 inner-solve storage",
+            "data": "synthetic", "form": "served_mixed",
+            "config": mixres["desc"],
+            "latency_s": f["served_mixed"]["latency_s"],
+            "speedup_vs_plain_fp64": round(value / f["plain_fp64"]["solves_per_s"], 3),
+            "max_dK_over_K_vs_fp64": mixres["max_dk"],
+            "forms": f,
+        }
+    else:
+        s = single["out"]
+        line = {"metric": METRIC, "value": s["mixed"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+                "steps": args.single_steps, "warmup": 1, "ms_per_step": s["mixed"]["ms_per_solve"],
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f64 arithmetic, f32 inner-solve storage", "data": "synthetic", "form": "single_mixed",
+                "config": {"workload": s["workload"]}}
+    if mixres is not None:
+        # dominant kernel of the headline form: the batched A kernel (FP32
+        # storage) over all requests in flight, sampled by the engine
+        r = engine_roofline(mixres["engine"], "f32", "A")
+        traffic, tsrc = mix_traffic(args)
+        line["roofline"] = None if r is None else dict(
+            bound="hbm", achieved=r["achieved"], peak=PEAK_HBM_GBS, unit="GB/s", frac=r["frac"], traffic=traffic,
+            kernel="bicg_A_kernel<float> batched over the requests in flight (served mixed form): p update + "
+                   "7-point SpMV + r0.v", bytes_per_launch=r["bytes_per_launch"], avg_launch_us=r["avg_launch_us"],
+            sampled_launches=r["sampled_launches"], traffic_source=tsrc)
+        line["served_mixed_engine"] = {k: v for k, v in mixres["engine"].items()}
+    if single is not None:
+        if mixres is None:
+            line["roofline"] = single["roof"]
+        else:
+            line["roofline_single_fp64"] = single["roof"]
+        line["single_image"] = single["out"]
+        line["kernels"] = single["kernels"]
+    line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 
 

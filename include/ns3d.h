@@ -205,7 +205,10 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
  * vectors, FP64 arithmetic in registers): r = b - A x in FP64, inner solve of
  * A d = r to max(inner_tol, tol |b| / 2|r|), x += d, until |b - A x| <= tol |b|.
  * Reported iterations are summed inner iterations; error is the true FP64
- * relative residual.  This form is not part of the reference. */
+ * relative residual.  When a correction step reduces |b - A x| by less than
+ * half (FP32 has reached its accuracy on that system), the solve finishes
+ * with the plain FP64 BiCGSTAB from the current x.  This form is not part of
+ * the reference. */
 ns3d_status ns3d_set_precision(ns3d_ctx* ctx, int mode, double inner_tol);
 
 /* Device time of the fused BiCGSTAB kernels (enable with ns3d_timing(ctx,

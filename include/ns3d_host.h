@@ -73,6 +73,12 @@ int ns3d_solver_info(ns3d_solver* s, int dims[3], int64_t counts[4]);
 int ns3d_solver_run(ns3d_solver* s, double* rows, int max_rows, int* nrows, ns3d_run_result* res);
 /* Final fields on the node sets (lengths from ns3d_solver_info). */
 int ns3d_solver_fields(ns3d_solver* s, double* p, double* u, double* v, double* w);
+/* Warm start: initial p, u, v, w on the node sets (lengths from
+ * ns3d_solver_info) for the next run, in place of the linear pressure
+ * profile and zero velocities (NULL keeps that default for the field) --
+ * the in-memory form of the pressurefield / vxfield / vyfield / vzfield
+ * inputs (NSBase.cxx:2575-2690). */
+int ns3d_solver_warm_start(ns3d_solver* s, const double* p, const double* u, const double* v, const double* w);
 void ns3d_solver_destroy(ns3d_solver* s);
 /* The GPU context behind a solver (for ns3d_timing / ns3d_matrix_nnz). */
 struct ns3d_ctx* ns3d_solver_context(ns3d_solver* s);
@@ -103,6 +109,8 @@ int ns3d_server_wait(ns3d_server* s, int64_t ticket, ns3d_run_result* res, doubl
 /* out[16]: submitted, queued, running, done, failed, fallbacks, bytes in use,
  * peak bytes in use, peak running, budget, seconds since creation. */
 int ns3d_server_stats(ns3d_server* s, double out[16]);
+/* The server's solve engine (for ns3d_engine_stats). */
+struct ns3d_engine* ns3d_server_engine(ns3d_server* s);
 /* Finishes every queued and running request, then frees the server. */
 void ns3d_server_destroy(ns3d_server* s);
 const char* ns3d_server_last_error(void);

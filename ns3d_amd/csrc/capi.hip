@@ -52,8 +52,6 @@ struct SysMem {
   float* invdiag32 = nullptr;
   uint64_t* pk = nullptr;    // packed neighbour distances (layout != ML_INDEXED)
   int layout = ML_INDEXED;   // how the solver reads the matrix (MatLayout)
-  Tile* tiles = nullptr;     // the solver's tiles of this system (device)
-  int ntiles = 0;
   SysView view() const { return SysView{n, d0, grid, cell, nbr, coef, invdiag, rowsum}; }
   SMat<double> mat64() const { return SMat<double>{n, layout, nbr, pk, coef, invdiag}; }
   SMat<float> mat32() const { return SMat<float>{n, layout, nbr, pk, coef32, invdiag32}; }
@@ -95,8 +93,6 @@ struct ns3d_ctx {
   int precision = NS3D_PRECISION_FP64;
   double inner_tol = 1e-4;
   double* partials = nullptr;        // [kPartialSlots][kMaxTilesPerSolve]
-  void* ddesc = nullptr;             // one SolveDesc (device) for the context's own solves
-  void* hdesc = nullptr;             // pinned staging for it
   double* result = nullptr;          // reduction results (device)
   BState* hstate = nullptr;          // pinned copy of the solver state
   double* hresult = nullptr;         // pinned copy of results
@@ -104,6 +100,7 @@ struct ns3d_ctx {
   bool have_params = false;
   std::vector<void*> allocs;
   double alloc_bytes = 0;   // device bytes requested by this context
+  int64_t fp64_finishes = 0;  // mixed form: solves finished in FP64 after FP32 stagnated
   Timing tm;
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> tev;  // start / stop of A, B, C of the sampled iteration
@@ -224,11 +221,9 @@ void bicg_loop(ns3d_ctx* c, int sid, SMat<T> M, T* x, const T* b, SWork<T> w, do
     if (c->hstate->status != BS_ZERO_RHS) hint = c->hstate->it_c > 0 ? c->hstate->it_c : 4;
     return;
   }
-  SolveDesc<T>* hd = (SolveDesc<T>*)c->hdesc;
-  *hd = SolveDesc<T>{M, w, x, b, tol, (long long)maxit, c->partials, kMaxTilesPerSolve, 0, S.ntiles, 0};
-  HCHK(hipMemcpyAsync(c->ddesc, hd, sizeof(SolveDesc<T>), hipMemcpyHostToDevice, s));
-  const Batch<T> B{(const SolveDesc<T>*)c->ddesc, S.tiles, 1, S.ntiles};
-  HCHK(launch_bicg_init<T>(B, s));
+  // one solve: its descriptor travels by value in the kernel arguments
+  const SolveDesc<T> d{M, w, x, b, tol, (long long)maxit, c->partials, kMaxTilesPerSolve, 0, 0, 0};
+  HCHK(launch_bicg_init<T>(single_batch(d, 0, 0), s));
   HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));
   int64_t chunk = hint > 4 ? (hint * 3) / 4 : 4;
@@ -236,15 +231,16 @@ void bicg_loop(ns3d_ctx* c, int sid, SMat<T> M, T* x, const T* b, SWork<T> w, do
   const int64_t cap = 4 * (maxit > 0 ? maxit : 1) + 64;
   const int prec = sizeof(T) == 4 ? 1 : 0;
   while (c->hstate->status == BS_RUNNING || c->hstate->status == BS_RESTART) {
-    if (c->hstate->status == BS_RESTART) HCHK(launch_bicg_restart<T>(B, s));
-    if (chunk > maxit + 1) chunk = maxit + 1;
     const long long done_before = c->hstate->it_c;
+    if (c->hstate->status == BS_RESTART) HCHK(launch_bicg_restart<T>(single_batch(d, done_before, 0), s));
+    if (chunk > maxit + 1) chunk = maxit + 1;
     // with timing on, the kernels of one iteration in the middle of the
     // chunk carry start/stop events, read back after the chunk's sync if
     // that iteration did work
     const int64_t qt = c->tm.on ? chunk / 2 : -1;
     for (int64_t q = 0; q < chunk; ++q) {
       hipEvent_t* e = (q == qt) ? c->tev.data() : nullptr;
+      const Batch<T> B = single_batch(d, done_before, (int)q);
       HCHK(launch_bicg_A<T>(B, s, e));
       HCHK(launch_bicg_B<T>(B, s, e ? e + 2 : nullptr));
       HCHK(launch_bicg_C<T>(B, s, e ? e + 4 : nullptr));
@@ -326,6 +322,7 @@ void bicgstab_mixed(ns3d_ctx* c, int sid, double* x, const double* b, double tol
   const double tol2 = tol * tol * bb;
   int64_t total = 0;
   int outer = 0;
+  bool stagnated = false;
   while (rr > tol2 && total < maxit && outer < 100) {
     const double rel = std::sqrt(rr / bb);
     const double tin = std::max(c->inner_tol, 0.5 * tol / rel);
@@ -336,13 +333,32 @@ void bicgstab_mixed(ns3d_ctx* c, int sid, double* x, const double* b, double tol
     info->restarts += (int32_t)st.restarts;
     total += k;
     ++outer;
-    if (k == 0) break;  // no progress possible in FP32
+    if (k == 0) {  // the FP32 sweep cannot act on this residual
+      stagnated = true;
+      break;
+    }
     HCHK(launch_add_f32(x, c->d32, S.n, s));
+    const double rr_old = rr;
     residual(rr, bb);
+    if (rr > 0.25 * rr_old) {  // |r| fell by less than 2x: FP32 has reached its accuracy here
+      stagnated = true;
+      break;
+    }
+  }
+  double err = std::sqrt(rr / bb);
+  if (stagnated && rr > tol2 && total < maxit) {
+    // finish this solve in FP64 from the current iterate (same tolerance,
+    // the remaining iteration budget)
+    bicg_loop<double>(c, sid, M64, x, b, c->bw, tol, maxit - total, S.last_iters, sid == 3 ? 1 : 0);
+    const BState& st = *c->hstate;
+    total += st.it_c;
+    info->restarts += (int32_t)st.restarts;
+    err = std::sqrt(st.rr / st.rhs_sqnorm);
+    c->fp64_finishes += 1;
   }
   info->iterations = total;
   info->work = total;
-  info->error = std::sqrt(rr / bb);
+  info->error = err;
   info->info = (info->error <= tol) ? 0 : 2;
 }
 
@@ -361,25 +377,6 @@ void refresh_f32(ns3d_ctx* c, int sid) {
     S.invdiag32 = c->alloc<float>(S.n);
   }
   HCHK(launch_coef_to_f32(S.coef, S.invdiag, S.n, S.coef32, S.invdiag32, c->stream));
-}
-
-// The solver's tile table of system sid (static for the context).
-void build_tiles(ns3d_ctx* c, int sid) {
-  SysMem& S = c->sys[sid];
-  std::vector<Tile> t;
-  int t0 = 0, t1 = 0;
-  make_tiles(0, S.n, t, t0, t1);
-  S.ntiles = t1;
-  S.tiles = c->alloc<Tile>(S.ntiles);
-  HCHK(hipMemcpyAsync(S.tiles, t.data(), t.size() * sizeof(Tile), hipMemcpyHostToDevice, c->stream));
-  HCHK(hipStreamSynchronize(c->stream));
-}
-
-// Descriptor buffers of the context's own solves.
-void alloc_solver_common(ns3d_ctx* c) {
-  const size_t dsz = std::max(sizeof(SolveDesc<double>), sizeof(SolveDesc<float>));
-  c->ddesc = c->alloc<char>((int64_t)dsz);
-  HCHK(hipHostMalloc(&c->hdesc, dsz, hipHostMallocDefault));
 }
 
 void alloc_system(ns3d_ctx* c, int sid, int n, int d0, int32_t* grid, bool momentum) {
@@ -516,8 +513,6 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
     w.v[1] = c->alloc<double>(nmax);
     w.t = c->alloc<double>(nmax);
     c->partials = c->alloc<double>((int64_t)kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks));
-    alloc_solver_common(c);
-    for (int q = 0; q < 4; ++q) build_tiles(c, q);
     w.st = c->alloc<BState>(1);
     HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), s));
     c->result = c->alloc<double>(16);
@@ -544,7 +539,6 @@ void ns3d_destroy(ns3d_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->hstate) (void)hipHostFree(c->hstate);
   if (c->hresult) (void)hipHostFree(c->hresult);
-  if (c->hdesc) (void)hipHostFree(c->hdesc);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->tev)
@@ -837,8 +831,6 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
     w.v[0] = c->alloc<double>(n); w.v[1] = c->alloc<double>(n);
     w.t = c->alloc<double>(n);
     c->partials = c->alloc<double>((int64_t)kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks));
-    alloc_solver_common(c);
-    build_tiles(c, 3);
     w.st = c->alloc<BState>(1);
     HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), s));
     HCHK(hipHostMalloc((void**)&c->hstate, sizeof(BState), hipHostMallocDefault));

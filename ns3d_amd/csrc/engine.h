@@ -33,6 +33,7 @@ struct EngineRequest {
   hipEvent_t ready = nullptr;
   BState result{};
   bool done = false;
+  long long it_c = 0;  // engine-private: executed iterations at the last poll
 };
 
 }  // namespace ns3d
@@ -59,14 +60,17 @@ struct ns3d_engine {
     ns3d::Tile* h_tiles = nullptr;
     void* h_idesc = nullptr;
     ns3d::Tile* h_itiles = nullptr;
+    long long* d_it0 = nullptr;  // per active solve: executed iterations when the chunk is queued
+    long long* h_it0 = nullptr;
     int ntiles = 0;
     bool dirty = true;
   };
   void run();
   void admit(std::vector<ns3d::EngineRequest*>& reqs);
   void rebuild(int g);
+  void upload_it0(int g);
   template <typename T>
-  ns3d::Batch<T> batch(int g) const;
+  ns3d::Batch<T> batch(int g, int q) const;
   void fail(const std::string& msg);
   bool any_active() const { return !grp[0].active.empty() || !grp[1].active.empty(); }
 

@@ -73,6 +73,8 @@ ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) 
     g.d_idesc = dalloc(dsz * kEngMaxSolves);
     g.d_tiles = (Tile*)dalloc(sizeof(Tile) * kEngMaxTiles);
     g.d_itiles = (Tile*)dalloc(sizeof(Tile) * kEngMaxTiles);
+    g.d_it0 = (long long*)dalloc(sizeof(long long) * kEngMaxSolves);
+    ECHK(hipHostMalloc((void**)&g.h_it0, sizeof(long long) * kEngMaxSolves, hipHostMallocDefault));
     ECHK(hipHostMalloc(&g.h_desc, dsz * kEngMaxSolves, hipHostMallocDefault));
     ECHK(hipHostMalloc((void**)&g.h_tiles, sizeof(Tile) * kEngMaxTiles, hipHostMallocDefault));
     ECHK(hipHostMalloc(&g.h_idesc, dsz * kEngMaxSolves, hipHostMallocDefault));
@@ -99,6 +101,7 @@ ns3d_engine::~ns3d_engine() {
     if (g.h_tiles) (void)hipHostFree(g.h_tiles);
     if (g.h_idesc) (void)hipHostFree(g.h_idesc);
     if (g.h_itiles) (void)hipHostFree(g.h_itiles);
+    if (g.h_it0) (void)hipHostFree(g.h_it0);
   }
   if (h_stptr) (void)hipHostFree(h_stptr);
   if (h_states) (void)hipHostFree(h_states);
@@ -139,9 +142,16 @@ void ns3d_engine::fail(const std::string& msg) {
 }
 
 template <typename T>
-Batch<T> ns3d_engine::batch(int g) const {
+Batch<T> ns3d_engine::batch(int g, int q) const {
   const Group& G = grp[g];
-  return Batch<T>{(const SolveDesc<T>*)G.d_desc, G.d_tiles, (int)G.active.size(), G.ntiles};
+  Batch<T> B{};
+  B.d = (const SolveDesc<T>*)G.d_desc;
+  B.tiles = G.d_tiles;
+  B.it0 = G.d_it0;
+  B.nsolves = (int)G.active.size();
+  B.ntiles = G.ntiles;
+  B.q = q;
+  return B;
 }
 
 void ns3d_engine::admit(std::vector<EngineRequest*>& reqs) {
@@ -164,15 +174,31 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs) {
       std::memcpy(G.h_itiles, tiles.data(), tiles.size() * sizeof(Tile));
       ECHK(hipMemcpyAsync(G.d_idesc, hd, ns * sizeof(SolveDesc<T>), hipMemcpyHostToDevice, stream));
       ECHK(hipMemcpyAsync(G.d_itiles, G.h_itiles, tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
-      const Batch<T> B{(const SolveDesc<T>*)G.d_idesc, G.d_itiles, ns, (int)tiles.size()};
+      Batch<T> B{};
+      B.d = (const SolveDesc<T>*)G.d_idesc;
+      B.tiles = G.d_itiles;
+      B.nsolves = ns;
+      B.ntiles = (int)tiles.size();
       ECHK(launch_bicg_init<T>(B, stream));
     };
     if (g == 0) init(double{});
     else init(float{});
     for (EngineRequest* r : reqs)
-      if (r->prec == g) G.active.push_back(r);
+      if (r->prec == g) {
+        r->it_c = 0;
+        G.active.push_back(r);
+      }
     if (ns) G.dirty = true;
   }
+}
+
+// Executed iterations of the group's active solves, in table order, for the
+// row kernels of the next chunk (launch q of the chunk is iteration it0 + q).
+void ns3d_engine::upload_it0(int g) {
+  Group& G = grp[g];
+  if (G.active.empty()) return;
+  for (size_t i = 0; i < G.active.size(); ++i) G.h_it0[i] = G.active[i]->it_c;
+  ECHK(hipMemcpyAsync(G.d_it0, G.h_it0, G.active.size() * sizeof(long long), hipMemcpyHostToDevice, stream));
 }
 
 void ns3d_engine::rebuild(int g) {
@@ -222,11 +248,14 @@ void ns3d_engine::run() {
         }
       }
       if (!admitted.empty()) admit(admitted);
+      // tables, iteration bases (each pinned staging buffer is written once
+      // per round, after the previous round's sync), restarts
       for (int g = 0; g < 2; ++g) {
         if (grp[g].dirty) rebuild(g);
+        upload_it0(g);
         if (restart[g] && !grp[g].active.empty()) {
-          if (g == 0) ECHK(launch_bicg_restart<double>(batch<double>(0), stream));
-          else ECHK(launch_bicg_restart<float>(batch<float>(1), stream));
+          if (g == 0) ECHK(launch_bicg_restart<double>(batch<double>(0, 0), stream));
+          else ECHK(launch_bicg_restart<float>(batch<float>(1, 0), stream));
         }
         restart[g] = false;
       }
@@ -242,12 +271,12 @@ void ns3d_engine::run() {
       const int qt = chunk / 2;
       for (int q = 0; q < chunk; ++q) {
         const bool ev = (q == qt);
-        if (!grp[0].active.empty()) ECHK(launch_bicg_A<double>(batch<double>(0), stream, ev ? tev : nullptr));
-        if (!grp[1].active.empty()) ECHK(launch_bicg_A<float>(batch<float>(1), stream, ev ? tev + 6 : nullptr));
-        if (!grp[0].active.empty()) ECHK(launch_bicg_B<double>(batch<double>(0), stream, ev ? tev + 2 : nullptr));
-        if (!grp[1].active.empty()) ECHK(launch_bicg_B<float>(batch<float>(1), stream, ev ? tev + 8 : nullptr));
-        if (!grp[0].active.empty()) ECHK(launch_bicg_C<double>(batch<double>(0), stream, ev ? tev + 4 : nullptr));
-        if (!grp[1].active.empty()) ECHK(launch_bicg_C<float>(batch<float>(1), stream, ev ? tev + 10 : nullptr));
+        if (!grp[0].active.empty()) ECHK(launch_bicg_A<double>(batch<double>(0, q), stream, ev ? tev : nullptr));
+        if (!grp[1].active.empty()) ECHK(launch_bicg_A<float>(batch<float>(1, q), stream, ev ? tev + 6 : nullptr));
+        if (!grp[0].active.empty()) ECHK(launch_bicg_B<double>(batch<double>(0, q), stream, ev ? tev + 2 : nullptr));
+        if (!grp[1].active.empty()) ECHK(launch_bicg_B<float>(batch<float>(1, q), stream, ev ? tev + 8 : nullptr));
+        if (!grp[0].active.empty()) ECHK(launch_bicg_C<double>(batch<double>(0, q), stream, ev ? tev + 4 : nullptr));
+        if (!grp[1].active.empty()) ECHK(launch_bicg_C<float>(batch<float>(1, q), stream, ev ? tev + 10 : nullptr));
       }
       n_launch += (double)chunk * 3 * ((grp[0].active.empty() ? 0 : 1) + (grp[1].active.empty() ? 0 : 1));
       n_chunks += 1;
@@ -279,6 +308,7 @@ void ns3d_engine::run() {
         for (int i = 0; i < nact; ++i) {
           EngineRequest* r = order[i];
           const BState& st = h_states[i];
+          r->it_c = st.it_c;
           if (st.status == BS_RUNNING) continue;
           if (st.status == BS_RESTART) {
             restart[r->prec] = true;

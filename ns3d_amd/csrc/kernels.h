@@ -127,13 +127,28 @@ struct Tile {
   long long begin, end;
 };
 // The operands of one launch over a set of solves of storage type T.
+// Either a table (d, tiles, it0: the executed iterations of each solve when
+// the chunk was queued) or one solve passed by value (single = 1: `one`,
+// tiles of `per` rows, one_it0).  q is the launch's index in its chunk, so a
+// row kernel knows its iteration (it0 + q) -- and with it the p/v buffer and
+// the scalar slot -- without a dependent load; it exits if the solve has
+// stopped or its own count disagrees.
 constexpr int kPartialSlots = 8;
 template <typename T>
 struct Batch {
   const SolveDesc<T>* d;
   const Tile* tiles;
+  const long long* it0;
   int nsolves, ntiles;
+  int q;
+  int single;
+  long long one_it0;
+  long long per;
+  SolveDesc<T> one;
 };
+// Batch of the one solve `d` (tiles as make_tiles builds them).
+template <typename T>
+Batch<T> single_batch(const SolveDesc<T>& d, long long it0, int q);
 // Tiles of one solve of n rows: between 1 and kMaxTilesPerSolve tiles of
 // a multiple of kBlock rows, at least kTileRows rows each where n allows.
 constexpr int kMaxTilesPerSolve = 1024;

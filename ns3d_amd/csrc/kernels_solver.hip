@@ -152,12 +152,12 @@ template <typename T>
 struct CIn {
   T m, p, r, v, t, x, r0;
 };
-template <typename T, typename P, typename X, typename W>
-__device__ __forceinline__ void c_fetch(P invdiag, X x, const W& w, int par, int64_t row, CIn<T>& f) {
+template <typename T, typename P, typename X, typename W, typename V>
+__device__ __forceinline__ void c_fetch(P invdiag, X x, const W& w, V pn, V vn, int64_t row, CIn<T>& f) {
   f.m = invdiag[row];
-  f.p = w.p[par][row];
+  f.p = pn[row];
   f.r = w.r[row];
-  f.v = w.v[par][row];
+  f.v = vn[row];
   f.t = w.t[row];
   f.x = x[row];
   f.r0 = w.r0[row];
@@ -168,22 +168,6 @@ __device__ __forceinline__ void c_fetch(P invdiag, X x, const W& w, int par, int
 __device__ __forceinline__ int block_tile(int ntiles) {
   const int t = logical_block();
   return t < ntiles ? t : -1;
-}
-
-// Reads a block-uniform struct and moves it to scalar registers.  The
-// kernels store to global memory, so the compiler cannot prove the table
-// read-only and would keep the fields in vector registers.
-template <typename S>
-__device__ __forceinline__ S uniform_load(const S* p) {
-  static_assert(sizeof(S) % 4 == 0, "uniform_load: size must be a multiple of 4");
-  constexpr int kWords = (int)(sizeof(S) / 4);
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
-  uint32_t w[kWords];
-#pragma unroll
-  for (int q = 0; q < kWords; ++q) w[q] = __builtin_amdgcn_readfirstlane(src[q]);
-  S out;
-  __builtin_memcpy(&out, w, sizeof(S));
-  return out;
 }
 
 // Pointers read from a table are generic pointers to the compiler, which
@@ -227,8 +211,7 @@ __device__ __forceinline__ NS3D_GLOBAL P* gl(P* p) {
   return (NS3D_GLOBAL P*)p;
 }
 template <typename T>
-__device__ __forceinline__ GDesc<T> load_desc(const SolveDesc<T>* p) {
-  const SolveDesc<T> D = uniform_load(p);
+__device__ __forceinline__ GDesc<T> to_gdesc(const SolveDesc<T>& D) {
   GDesc<T> G;
   G.S = GMat<T>{D.S.n, D.S.layout, gl(D.S.nbr), gl(D.S.pk), gl(D.S.coef), gl(D.S.invdiag)};
   G.w.r = gl(D.w.r);
@@ -249,6 +232,32 @@ __device__ __forceinline__ GDesc<T> load_desc(const SolveDesc<T>* p) {
   G.t1 = D.t1;
   return G;
 }
+template <typename T>
+__device__ __forceinline__ GDesc<T> load_desc(const SolveDesc<T>* p) {
+  return to_gdesc(*p);
+}
+
+// The solve and rows of tile t of a launch, and the iteration the launch
+// stands for.  One solve passed by value needs no table loads.
+template <typename T>
+__device__ __forceinline__ void tile_of(const Batch<T>& B, int t, Tile& tl, GDesc<T>& D, long long& it) {
+  if (B.single) {
+    D = to_gdesc(B.one);
+    const long long n = D.S.n, b = (long long)t * B.per;
+    tl.solve = 0;
+    tl.begin = b < n ? b : n;
+    tl.end = b + B.per < n ? b + B.per : n;
+    it = B.one_it0 + B.q;
+  } else {
+    tl = B.tiles[t];
+    D = load_desc(B.d + tl.solve);
+    it = (B.it0 ? B.it0[tl.solve] : 0) + B.q;
+  }
+}
+template <typename T>
+__device__ __forceinline__ GDesc<T> solve_of(const Batch<T>& B, int s) {
+  return B.single ? to_gdesc(B.one) : load_desc(B.d + s);
+}
 
 // Calls body.template operator()<L>() with the solve's layout as a
 // compile-time constant (the branch is uniform over the block).
@@ -265,8 +274,10 @@ __global__ void __launch_bounds__(kBlock)
 bicg_init_kernel(Batch<T> B) {
   const int t = block_tile(B.ntiles);
   if (t < 0) return;
-  const Tile tl = uniform_load(B.tiles + t);
-  const GDesc<T> D = load_desc(B.d + tl.solve);
+  Tile tl;
+  GDesc<T> D;
+  long long it;
+  tile_of(B, t, tl, D, it);
   double v2[2] = {0.0, 0.0};
   by_layout(D.S.layout, [&](auto LC) {
     constexpr int L = decltype(LC)::value;
@@ -291,7 +302,7 @@ bicg_init_kernel(Batch<T> B) {
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 bicg_init_fin(Batch<T> B) {
-  const GDesc<T> D = load_desc(B.d + blockIdx.x);
+  const GDesc<T> D = solve_of(B, blockIdx.x);
   double out[2];
   sum_partials_all<2>(D.slot(kSlotI), D.t1 - D.t0, out, D.pstride);
   if (threadIdx.x != 0) return;
@@ -324,8 +335,10 @@ __global__ void __launch_bounds__(kBlock)
 bicg_restart_kernel(Batch<T> B) {
   const int t = block_tile(B.ntiles);
   if (t < 0) return;
-  const Tile tl = uniform_load(B.tiles + t);
-  const GDesc<T> D = load_desc(B.d + tl.solve);
+  Tile tl;
+  GDesc<T> D;
+  long long it;
+  tile_of(B, t, tl, D, it);
   if (D.w.st->status != BS_RESTART) return;
   double v1[1] = {0.0};
   by_layout(D.S.layout, [&](auto LC) {
@@ -347,7 +360,7 @@ bicg_restart_kernel(Batch<T> B) {
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 bicg_restart_fin(Batch<T> B) {
-  const GDesc<T> D = load_desc(B.d + blockIdx.x);
+  const GDesc<T> D = solve_of(B, blockIdx.x);
   NS3D_GLOBAL BState* st = D.w.st;
   if (st->status != BS_RESTART) return;
   double out[1];
@@ -381,32 +394,43 @@ __global__ void __launch_bounds__(kBlock)
 bicg_A_kernel(Batch<T> B) {
   const int t = block_tile(B.ntiles);
   if (t < 0) return;
-  const Tile tl = uniform_load(B.tiles + t);
-  const GDesc<T> D = load_desc(B.d + tl.solve);
+  Tile tl;
+  GDesc<T> D;
+  long long it;
+  tile_of(B, t, tl, D, it);
   NS3D_GLOBAL BState* st = D.w.st;
-  if (st->status != BS_RUNNING) return;
   const bool lead = leader_of(t, D.t0);
-  const long long it = st->it_c;
   const int sl = (int)(it & 1);
   const int par = sl;
-  const NS3D_GLOBAL T* __restrict__ pold = D.w.p[par];
-  const NS3D_GLOBAL T* __restrict__ vold = D.w.v[par];
-  NS3D_GLOBAL T* __restrict__ pnew = D.w.p[par ^ 1];
-  NS3D_GLOBAL T* __restrict__ vnew = D.w.v[par ^ 1];
+  // (selects, not D.w.p[par]: a run-time index into a local array would
+  // put the descriptor on the stack)
+  const NS3D_GLOBAL T* __restrict__ pold = par ? D.w.p[1] : D.w.p[0];
+  const NS3D_GLOBAL T* __restrict__ vold = par ? D.w.v[1] : D.w.v[0];
+  NS3D_GLOBAL T* __restrict__ pnew = par ? D.w.p[0] : D.w.p[1];
+  NS3D_GLOBAL T* __restrict__ vnew = par ? D.w.v[0] : D.w.v[1];
   double acc = 0.0;
   bool stop = false;
   by_layout(D.S.layout, [&](auto LC) {
     constexpr int L = decltype(LC)::value;
     const int64_t n = D.S.n;
     int64_t row = tl.begin + threadIdx.x;
+    // first row, state and partials in flight together (rows of a stopped
+    // solve are fetched for nothing and dropped)
     AIn<T> cur;
     if (row < tl.end) a_fetch<T, L>(D.S, D.w, pold, vold, (int)row, n, cur);
+    const int status = st->status;
+    const long long itc = st->it_c, fresh_at = st->fresh_at;
+    const double rho_prev = st->rho[sl ^ 1], alpha_prev = st->alpha[sl ^ 1], w_prev = st->w[sl ^ 1];
+    double red[2];
+    sum_partials_all<2>(D.slot(kSlotC), D.t1 - D.t0, red, D.pstride);
+    if (status != BS_RUNNING || itc != it) {
+      stop = true;
+      return;
+    }
     double rho;
-    if (st->fresh_at == it) {
+    if (fresh_at == it) {
       rho = st->rho_fresh;
     } else {
-      double red[2];
-      sum_partials_all<2>(D.slot(kSlotC), D.t1 - D.t0, red, D.pstride);
       const double r0r = red[0], rr2 = red[1];
       if (!(rr2 > st->tol2 && it - st->ibase < st->maxit)) {
         if (lead) { st->rr = rr2; st->status = BS_DONE; }
@@ -421,8 +445,8 @@ bicg_A_kernel(Batch<T> B) {
       rho = r0r;
       if (lead) st->rr = rr2;
     }
-    const double beta = (rho / st->rho[sl ^ 1]) * (st->alpha[sl ^ 1] / st->w[sl ^ 1]);
-    const double om = st->w[sl ^ 1];
+    const double beta = (rho / rho_prev) * (alpha_prev / w_prev);
+    const double om = w_prev;
     if (lead) {
       st->rho[sl] = rho;
       st->it_a = it;
@@ -454,25 +478,33 @@ __global__ void __launch_bounds__(kBlock)
 bicg_B_kernel(Batch<T> B) {
   const int t = block_tile(B.ntiles);
   if (t < 0) return;
-  const Tile tl = uniform_load(B.tiles + t);
-  const GDesc<T> D = load_desc(B.d + tl.solve);
+  Tile tl;
+  GDesc<T> D;
+  long long it;
+  tile_of(B, t, tl, D, it);
   NS3D_GLOBAL BState* st = D.w.st;
-  if (st->status != BS_RUNNING) return;
   const bool lead = leader_of(t, D.t0);
-  const long long it = st->it_a;
   const int sl = (int)(it & 1);
   const int par = sl ^ 1;
-  const NS3D_GLOBAL T* __restrict__ vn = D.w.v[par];
+  const NS3D_GLOBAL T* __restrict__ vn = par ? D.w.v[1] : D.w.v[0];
   double a0 = 0.0, a1 = 0.0;
+  bool stop = false;
   by_layout(D.S.layout, [&](auto LC) {
     constexpr int L = decltype(LC)::value;
     const int64_t n = D.S.n;
     int64_t row = tl.begin + threadIdx.x;
     BIn<T> cur;
     if (row < tl.end) b_fetch<T, L>(D.S, D.w, vn, (int)row, n, cur);
+    const int status = st->status;
+    const long long ita = st->it_a;
+    const double rho = st->rho[sl];
     double red[1];
     sum_partials_all<1>(D.slot(kSlotA), D.t1 - D.t0, red, D.pstride);
-    const double alpha = st->rho[sl] / red[0];
+    if (status != BS_RUNNING || ita != it) {
+      stop = true;
+      return;
+    }
+    const double alpha = rho / red[0];
     if (lead) {
       st->r0v = red[0];
       st->alpha[sl] = alpha;
@@ -493,6 +525,7 @@ bicg_B_kernel(Batch<T> B) {
       row = nxt;
     }
   });
+  if (stop) return;
   double v2[2] = {a0, a1};
   block_sums_at<2>(v2, D.slot(kSlotB), D.pstride, t - D.t0);
 }
@@ -502,23 +535,28 @@ __global__ void __launch_bounds__(kBlock)
 bicg_C_kernel(Batch<T> B) {
   const int t = block_tile(B.ntiles);
   if (t < 0) return;
-  const Tile tl = uniform_load(B.tiles + t);
-  const GDesc<T> D = load_desc(B.d + tl.solve);
+  Tile tl;
+  GDesc<T> D;
+  long long it;
+  tile_of(B, t, tl, D, it);
   NS3D_GLOBAL BState* st = D.w.st;
-  if (st->status != BS_RUNNING) return;
   const bool lead = leader_of(t, D.t0);
-  const long long it = st->it_a;
   const int sl = (int)(it & 1);
   const int par = sl ^ 1;
   NS3D_GLOBAL T* __restrict__ x = D.x;
   const NS3D_GLOBAL T* __restrict__ invdiag = D.S.invdiag;
+  const NS3D_GLOBAL T* __restrict__ pn = par ? D.w.p[1] : D.w.p[0];
+  const NS3D_GLOBAL T* __restrict__ vn = par ? D.w.v[1] : D.w.v[0];
   int64_t row = tl.begin + threadIdx.x;
   CIn<T> cur;
-  if (row < tl.end) c_fetch<T>(invdiag, x, D.w, par, row, cur);
+  if (row < tl.end) c_fetch<T>(invdiag, x, D.w, pn, vn, row, cur);
+  const int status = st->status;
+  const long long ita = st->it_a;
+  const double alpha = st->alpha[sl];
   double red[2];
   sum_partials_all<2>(D.slot(kSlotB), D.t1 - D.t0, red, D.pstride);
+  if (status != BS_RUNNING || ita != it) return;
   const double om = (red[1] > 0.0) ? red[0] / red[1] : 0.0;
-  const double alpha = st->alpha[sl];
   if (lead) {
     st->ts = red[0];
     st->tt = red[1];
@@ -529,7 +567,7 @@ bicg_C_kernel(Batch<T> B) {
   while (row < tl.end) {
     const int64_t nxt = row + kBlock;
     CIn<T> nx;
-    if (nxt < tl.end) c_fetch<T>(invdiag, x, D.w, par, nxt, nx);
+    if (nxt < tl.end) c_fetch<T>(invdiag, x, D.w, pn, vn, nxt, nx);
     const double md = cur.m;
     const double y = md * (double)cur.p;
     const double s = (double)(T)((double)cur.r - alpha * (double)cur.v);
@@ -662,6 +700,22 @@ void make_tiles(int solve, int64_t n, std::vector<Tile>& out, int& t0, int& t1) 
 int tile_grid(int ntiles) { return std::max(kXcd, (ntiles + kXcd - 1) / kXcd * kXcd); }
 
 template <typename T>
+Batch<T> single_batch(const SolveDesc<T>& d, long long it0, int q) {
+  Batch<T> B{};
+  const int nt = tiles_for(d.S.n);
+  B.single = 1;
+  B.one = d;
+  B.one.t0 = 0;
+  B.one.t1 = nt;
+  B.nsolves = 1;
+  B.ntiles = nt;
+  B.per = ((d.S.n + nt - 1) / nt + kBlock - 1) / kBlock * kBlock;
+  B.one_it0 = it0;
+  B.q = q;
+  return B;
+}
+
+template <typename T>
 hipError_t launch_bicg_init(Batch<T> B, hipStream_t s) {
   hipLaunchKernelGGL(bicg_init_kernel<T>, dim3(tile_grid(B.ntiles)), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(bicg_init_fin<T>, dim3(B.nsolves), dim3(kBlock), 0, s, B);
@@ -691,6 +745,7 @@ hipError_t launch_bicg_C(Batch<T> B, hipStream_t s, hipEvent_t* ev) {
 }
 
 #define NS3D_INSTANTIATE(T)                                                         \
+  template Batch<T> single_batch<T>(const SolveDesc<T>&, long long, int);          \
   template hipError_t launch_bicg_init<T>(Batch<T>, hipStream_t);                  \
   template hipError_t launch_bicg_restart<T>(Batch<T>, hipStream_t);               \
   template hipError_t launch_bicg_A<T>(Batch<T>, hipStream_t, hipEvent_t*);        \

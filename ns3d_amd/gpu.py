@@ -324,7 +324,11 @@ class Engine:
     def stats(self):
         out = np.zeros(32)
         _check(lib().ns3d_engine_stats(self.h, _p(out, C.c_double)))
-        d = dict(zip(self.STAT_NAMES, out[:5].tolist()))
+        return self.unpack_stats(out)
+
+    @classmethod
+    def unpack_stats(cls, out):
+        d = dict(zip(cls.STAT_NAMES, out[:5].tolist()))
         for q, prec in enumerate(("f64", "f32")):
             for k, name in enumerate("ABC"):
                 o = 5 + 9 * q + 3 * k

@@ -303,7 +303,10 @@ void NavierStokesBase::ComputePermeability(double& Q, double& v_moy, double& K) 
 
 // NSBase.cxx:2645-2690 (Initpvector): field from file or the linear profile.
 void NavierStokesBase::Initpvector() {
-  if (pfieldfilename != "none") {
+  if (!warm_p.empty()) {
+    check(ns3d_set_vector(ctx, NS3D_VEC_P, warm_p.data(), np), "ns3d_set_vector");
+    say("Importing pressure field (warm start)\n");
+  } else if (pfieldfilename != "none") {
     Image im = read_image(pfieldfilename);
     if (im.nx != nx || im.ny != ny || im.nz != nz) throw std::runtime_error("pressure field size mismatch");
     std::vector<int32_t> pidx((size_t)nx * ny * nz);
@@ -328,8 +331,14 @@ void NavierStokesBase::InitVelocity() {
   const std::string* names[3] = {&vxfilename, &vyfilename, &vzfilename};
   const int vecs[3] = {NS3D_VEC_U, NS3D_VEC_V, NS3D_VEC_W};
   const int64_t lens[3] = {nu, nv, nw};
+  const std::vector<double>* warm[3] = {&warm_u, &warm_v, &warm_w};
   std::vector<int32_t> idx[3];
   for (int q = 0; q < 3; ++q) {
+    if (!warm[q]->empty()) {
+      check(ns3d_set_vector(ctx, vecs[q], warm[q]->data(), lens[q]), "ns3d_set_vector");
+      say("Importing velocity field %c (warm start)\n", "xyz"[q]);
+      continue;
+    }
     if (*names[q] == "none") continue;
     if (idx[0].empty()) {
       idx[0].resize((size_t)(nx + 1) * ny * nz);

@@ -58,6 +58,9 @@ class NavierStokesBase {
   SolverReport usolver, vsolver, wsolver, psolver;
   std::string outfilename, info_filename, log_filename, outdir;
   std::string pfieldfilename, vxfilename, vyfilename, vzfilename;
+  // in-memory warm start (node-set vectors; empty: not given), used like the
+  // field files by Initpvector / InitVelocity
+  std::vector<double> warm_p, warm_u, warm_v, warm_w;
   std::ofstream info_file, log_file;
   bool verbose = true;        // reference-style progress on stdout
   bool progress = false;      // one line per outer iteration on stderr

@@ -131,6 +131,20 @@ int ns3d_solver_fields(ns3d_solver* s, double* p, double* u, double* v, double* 
   }
 }
 
+int ns3d_solver_warm_start(ns3d_solver* s, const double* p, const double* u, const double* v, const double* w) {
+  if (!s) return 1;
+  NavierStokesBase& B = *s->base;
+  auto put = [](std::vector<double>& dst, const double* src, int64_t n) {
+    if (src) dst.assign(src, src + n);
+    else dst.clear();
+  };
+  put(B.warm_p, p, B.np);
+  put(B.warm_u, u, B.nu);
+  put(B.warm_v, v, B.nv);
+  put(B.warm_w, w, B.nw);
+  return 0;
+}
+
 void ns3d_solver_destroy(ns3d_solver* s) { delete s; }
 
 ns3d_ctx* ns3d_solver_context(ns3d_solver* s) { return s ? s->base->ctx : nullptr; }

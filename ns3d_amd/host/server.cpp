@@ -71,7 +71,8 @@ struct ns3d_server {
 };
 
 void ns3d_server::run_job(Job& j) {
-  auto attempt = [&](ns3d_run_params p, ns3d_run_result& res) {
+  // fields: in (warm start, may be empty) and out (final fields, when asked)
+  auto attempt = [&](ns3d_run_params p, ns3d_run_result& res, std::vector<double>* warm, std::vector<double>* keep) {
     ns3d_solver* s = nullptr;
     if (ns3d_solver_create(&p, j.img.data(), j.nx, j.ny, j.nz, &s) != 0) throw std::string(ns3d_host_last_error());
     if (ns3d_attach_engine(ns3d_solver_context(s), eng) != NS3D_OK) {
@@ -79,8 +80,18 @@ void ns3d_server::run_job(Job& j) {
       ns3d_solver_destroy(s);
       throw m;
     }
-    const int rc = ns3d_solver_run(s, nullptr, 0, nullptr, &res);
-    const std::string m = rc ? ns3d_host_last_error() : "";
+    if (warm && !warm[0].empty())
+      ns3d_solver_warm_start(s, warm[0].data(), warm[1].data(), warm[2].data(), warm[3].empty() ? nullptr : warm[3].data());
+    int rc = ns3d_solver_run(s, nullptr, 0, nullptr, &res);
+    std::string m = rc ? ns3d_host_last_error() : "";
+    if (!rc && keep) {
+      int dims[3];
+      int64_t cnt[4];
+      ns3d_solver_info(s, dims, cnt);
+      keep[0].resize(cnt[0]); keep[1].resize(cnt[1]); keep[2].resize(cnt[2]); keep[3].resize(cnt[3]);
+      rc = ns3d_solver_fields(s, keep[0].data(), keep[1].data(), keep[2].data(), keep[3].data());
+      if (rc) m = ns3d_host_last_error();
+    }
     ns3d_solver_destroy(s);
     if (rc) throw m;
   };
@@ -88,11 +99,19 @@ void ns3d_server::run_job(Job& j) {
   p.device = device;
   p.outdir = j.outdir.empty() ? nullptr : j.outdir.c_str();
   p.name = j.name.empty() ? nullptr : j.name.c_str();
-  attempt(p, j.res);
-  if (p.precision != NS3D_PRECISION_FP64 && !j.res.converged && fallback) {
+  const bool may_fall_back = p.precision != NS3D_PRECISION_FP64 && fallback;
+  std::vector<double> fields[4];
+  attempt(p, j.res, nullptr, may_fall_back ? fields : nullptr);
+  if (may_fall_back && !j.res.converged) {
+    // continue in FP64 from the mixed run's fields, with a quarter of the
+    // outer-iteration cap: where FP64 can converge it does so quickly from
+    // there, and a request that stalls in any precision is not paid twice
+    const int total = j.res.iterations;
     p.precision = NS3D_PRECISION_FP64;
-    attempt(p, j.res);
+    p.maxitSIMPLE = std::max(10, p.maxitSIMPLE / 4);
+    attempt(p, j.res, fields, nullptr);
     j.res.fallback = 1;
+    j.res.iterations += total;
   }
 }
 
@@ -252,6 +271,8 @@ int ns3d_server_stats(ns3d_server* s, double out[16]) {
   std::memcpy(out, v, sizeof(v));
   return 0;
 }
+
+ns3d_engine* ns3d_server_engine(ns3d_server* s) { return s ? s->eng : nullptr; }
 
 void ns3d_server_destroy(ns3d_server* s) {
   if (!s) return;

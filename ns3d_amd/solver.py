@@ -17,8 +17,8 @@ METHODS = {"SIMPLEC": 0, "SIMPLER": 1, "SIMPLERR": 2, "IDEAL": 3}
 
 EXPORTS = ["ns3d_default_run_params", "ns3d_solver_create", "ns3d_solver_info", "ns3d_solver_run",
            "ns3d_solver_fields", "ns3d_solver_destroy", "ns3d_solver_context", "ns3d_host_last_error",
-           "ns3d_read_ini", "ns3d_server_create", "ns3d_server_submit", "ns3d_server_poll", "ns3d_server_wait",
-           "ns3d_server_stats", "ns3d_server_destroy", "ns3d_server_last_error"]
+           "ns3d_read_ini", "ns3d_solver_warm_start", "ns3d_server_create", "ns3d_server_submit", "ns3d_server_poll", "ns3d_server_wait",
+           "ns3d_server_stats", "ns3d_server_destroy", "ns3d_server_last_error", "ns3d_server_engine"]
 
 
 class RunParams(C.Structure):
@@ -75,6 +75,8 @@ def lib():
         L.ns3d_read_ini.argtypes = [C.c_char_p, P(RunParams), C.c_char_p, C.c_int]
         L.ns3d_read_ini.restype = C.c_int
         vp = C.c_void_p
+        L.ns3d_solver_warm_start.argtypes = [vp] + [P(C.c_double)] * 4
+        L.ns3d_solver_warm_start.restype = C.c_int
         L.ns3d_server_create.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, P(vp)]
         L.ns3d_server_create.restype = C.c_int
         L.ns3d_server_submit.argtypes = [vp, P(RunParams), P(C.c_uint8), C.c_int, C.c_int, C.c_int, P(C.c_int64)]
@@ -87,6 +89,8 @@ def lib():
         L.ns3d_server_destroy.argtypes = [vp]
         L.ns3d_server_destroy.restype = None
         L.ns3d_server_last_error.restype = C.c_char_p
+        L.ns3d_server_engine.argtypes = [vp]
+        L.ns3d_server_engine.restype = vp
         _lib = L
     return _lib
 
@@ -146,6 +150,23 @@ class Solver:
 
     def timing(self, enable=True):
         _gpu._check(_gpu.lib().ns3d_timing(self.context(), int(enable)))
+
+    def warm_start(self, p=None, u=None, v=None, w=None):
+        """Initial fields for the next run (None: the default start)."""
+        arrs = []
+        ptrs = []
+        for x, n in ((p, self.np), (u, self.nu), (v, self.nv), (w, self.nw)):
+            if x is None:
+                ptrs.append(None)
+            else:
+                a = np.ascontiguousarray(x, dtype=np.float64)
+                if a.size != n:
+                    raise ValueError("warm_start: wrong field length")
+                arrs.append(a)
+                ptrs.append(a.ctypes.data_as(C.POINTER(C.c_double)))
+        self._warm = arrs  # kept alive until the run
+        if lib().ns3d_solver_warm_start(self.h, *ptrs) != 0:
+            raise _gpu.NS3DError("ns3d_solver_warm_start failed")
 
     def attach(self, engine):
         """Hands this solver's BiCGSTAB solves to a batched engine (None
@@ -254,6 +275,13 @@ class Server:
         out = (C.c_double * 16)()
         lib().ns3d_server_stats(self.h, out)
         return dict(zip(self.STAT_NAMES, list(out)[:11]))
+
+    def engine_stats(self):
+        """Statistics of the server's solve engine (see gpu.Engine.stats)."""
+        out = np.zeros(32)
+        _gpu._check(_gpu.lib().ns3d_engine_stats(C.c_void_p(lib().ns3d_server_engine(self.h)),
+                                                 out.ctypes.data_as(C.POINTER(C.c_double))))
+        return _gpu.Engine.unpack_stats(out)
 
     def close(self):
         if getattr(self, "h", None):

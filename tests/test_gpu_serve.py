@@ -104,6 +104,27 @@ def test_mixed_request_falls_back_to_fp64():
         st = srv.stats()
     finally:
         srv.close()
-    ref = solver.solve(img, **dict(p, precision=0))[1]
-    assert r["fallback"] == 1 and r["converged"] == 0 and st["fallbacks"] == 1
-    assert r["K"] == ref["K"] and r["iterations"] == ref["iterations"]
+    # FP64 continuation from the mixed run's fields with a quarter of the
+    # outer-iteration cap (at least 10): 3 + 10 outer iterations in all
+    assert r["fallback"] == 1 and st["fallbacks"] == 1
+    assert r["iterations"] == 3 + 10
+
+
+def test_warm_start_from_converged_fields():
+    """A run started from converged fields (the in-memory form of the
+    pressurefield / vxfield inputs) converges in far fewer outer iterations
+    than the cold run (the driver restarts its relaxation history, so not at
+    once) to the same permeability."""
+    img = generators.rsa_spheres((22, 18, 16), 3.0, 0.3, 12)[0]
+    p = dict(PRM, method="SIMPLEC", gradp=1.0 / 22)
+    rows, first = solver.solve(img, **p)
+    f = first["fields"]
+    s = solver.Solver(img, **p)
+    try:
+        s.warm_start(f["p"], f["u"], f["v"], f["w"])
+        rows2, again = s.run()
+    finally:
+        s.close()
+    assert first["converged"] == 1 and again["converged"] == 1
+    assert again["iterations"] < first["iterations"] / 2, (again["iterations"], first["iterations"])
+    assert abs(again["K"] / first["K"] - 1) < 1e-4
