@@ -56,12 +56,12 @@ struct ns3d_engine {
     ns3d::Tile* d_tiles = nullptr;
     void* d_idesc = nullptr;
     ns3d::Tile* d_itiles = nullptr;
-    void* h_desc = nullptr;    // pinned staging
-    ns3d::Tile* h_tiles = nullptr;
-    void* h_idesc = nullptr;
-    ns3d::Tile* h_itiles = nullptr;
+    // host images of the tables, rewritten only between rounds (after the
+    // stream has drained), so no copy in flight ever reads a changed buffer
+    std::vector<char> h_desc, h_idesc;
+    std::vector<ns3d::Tile> h_tiles, h_itiles;
     long long* d_it0 = nullptr;  // per active solve: executed iterations when the chunk is queued
-    long long* h_it0 = nullptr;
+    std::vector<long long> h_it0;
     int ntiles = 0;
     bool dirty = true;
   };
@@ -83,7 +83,7 @@ struct ns3d_engine {
   std::string fatal;
   Group grp[2];
   ns3d::BState** d_stptr = nullptr;   // state pointers of all active solves
-  ns3d::BState** h_stptr = nullptr;
+  std::vector<ns3d::BState*> h_stptr;
   ns3d::BState* d_states = nullptr;   // gathered copies
   ns3d::BState* h_states = nullptr;
   std::vector<ns3d::EngineRequest*> order;  // solve order of d_stptr

@@ -12,8 +12,9 @@
 //      copy), synchronise, retire the solves that stopped and wake their
 //      callers, and queue restart kernels for those that asked for one.
 // Kernels of solves that stopped exit at once, so a chunk never computes
-// past a solve's end.  The tables are only rewritten between rounds, after
-// the stream has drained, so pinned staging is never overwritten in flight.
+// past a solve's end.  The host images of the tables are only rewritten
+// between rounds, after the stream has drained, so no copy in flight reads a
+// buffer that changed.
 #include "engine.h"
 
 #include <hip/hip_ext.h>
@@ -74,15 +75,13 @@ ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) 
     g.d_tiles = (Tile*)dalloc(sizeof(Tile) * kEngMaxTiles);
     g.d_itiles = (Tile*)dalloc(sizeof(Tile) * kEngMaxTiles);
     g.d_it0 = (long long*)dalloc(sizeof(long long) * kEngMaxSolves);
-    ECHK(hipHostMalloc((void**)&g.h_it0, sizeof(long long) * kEngMaxSolves, hipHostMallocDefault));
-    ECHK(hipHostMalloc(&g.h_desc, dsz * kEngMaxSolves, hipHostMallocDefault));
-    ECHK(hipHostMalloc((void**)&g.h_tiles, sizeof(Tile) * kEngMaxTiles, hipHostMallocDefault));
-    ECHK(hipHostMalloc(&g.h_idesc, dsz * kEngMaxSolves, hipHostMallocDefault));
-    ECHK(hipHostMalloc((void**)&g.h_itiles, sizeof(Tile) * kEngMaxTiles, hipHostMallocDefault));
+    g.h_it0.resize(kEngMaxSolves);
+    g.h_desc.resize(dsz * kEngMaxSolves);
+    g.h_idesc.resize(dsz * kEngMaxSolves);
   }
   d_stptr = (BState**)dalloc(sizeof(BState*) * 2 * kEngMaxSolves);
   d_states = (BState*)dalloc(sizeof(BState) * 2 * kEngMaxSolves);
-  ECHK(hipHostMalloc((void**)&h_stptr, sizeof(BState*) * 2 * kEngMaxSolves, hipHostMallocDefault));
+  h_stptr.resize(2 * kEngMaxSolves);
   ECHK(hipHostMalloc((void**)&h_states, sizeof(BState) * 2 * kEngMaxSolves, hipHostMallocDefault));
   for (auto& e : tev) ECHK(hipEventCreate(&e));
   thr = std::thread([this] { run(); });
@@ -96,14 +95,6 @@ ns3d_engine::~ns3d_engine() {
   cv_work.notify_all();
   if (thr.joinable()) thr.join();
   if (stream) (void)hipStreamSynchronize(stream);
-  for (Group& g : grp) {
-    if (g.h_desc) (void)hipHostFree(g.h_desc);
-    if (g.h_tiles) (void)hipHostFree(g.h_tiles);
-    if (g.h_idesc) (void)hipHostFree(g.h_idesc);
-    if (g.h_itiles) (void)hipHostFree(g.h_itiles);
-    if (g.h_it0) (void)hipHostFree(g.h_it0);
-  }
-  if (h_stptr) (void)hipHostFree(h_stptr);
   if (h_states) (void)hipHostFree(h_states);
   for (void* p : allocs) (void)hipFree(p);
   for (auto& e : tev)
@@ -163,7 +154,7 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs) {
     int ns = 0;
     auto init = [&](auto tag) {
       using T = decltype(tag);
-      SolveDesc<T>* hd = (SolveDesc<T>*)G.h_idesc;
+      SolveDesc<T>* hd = (SolveDesc<T>*)G.h_idesc.data();
       for (EngineRequest* r : reqs) {
         if (r->prec != g) continue;
         SolveDesc<T> d = desc_of<T>(r);
@@ -171,9 +162,9 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs) {
         hd[ns++] = d;
       }
       if (ns == 0) return;
-      std::memcpy(G.h_itiles, tiles.data(), tiles.size() * sizeof(Tile));
+      G.h_itiles = tiles;
       ECHK(hipMemcpyAsync(G.d_idesc, hd, ns * sizeof(SolveDesc<T>), hipMemcpyHostToDevice, stream));
-      ECHK(hipMemcpyAsync(G.d_itiles, G.h_itiles, tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+      ECHK(hipMemcpyAsync(G.d_itiles, G.h_itiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
       Batch<T> B{};
       B.d = (const SolveDesc<T>*)G.d_idesc;
       B.tiles = G.d_itiles;
@@ -198,7 +189,7 @@ void ns3d_engine::upload_it0(int g) {
   Group& G = grp[g];
   if (G.active.empty()) return;
   for (size_t i = 0; i < G.active.size(); ++i) G.h_it0[i] = G.active[i]->it_c;
-  ECHK(hipMemcpyAsync(G.d_it0, G.h_it0, G.active.size() * sizeof(long long), hipMemcpyHostToDevice, stream));
+  ECHK(hipMemcpyAsync(G.d_it0, G.h_it0.data(), G.active.size() * sizeof(long long), hipMemcpyHostToDevice, stream));
 }
 
 void ns3d_engine::rebuild(int g) {
@@ -206,7 +197,7 @@ void ns3d_engine::rebuild(int g) {
   std::vector<Tile> tiles;
   auto build = [&](auto tag) {
     using T = decltype(tag);
-    SolveDesc<T>* hd = (SolveDesc<T>*)G.h_desc;
+    SolveDesc<T>* hd = (SolveDesc<T>*)G.h_desc.data();
     for (size_t i = 0; i < G.active.size(); ++i) {
       SolveDesc<T> d = desc_of<T>(G.active[i]);
       make_tiles((int)i, d.S.n, tiles, d.t0, d.t1);
@@ -214,9 +205,9 @@ void ns3d_engine::rebuild(int g) {
     }
     G.ntiles = (int)tiles.size();
     if (G.active.empty()) return;
-    std::memcpy(G.h_tiles, tiles.data(), tiles.size() * sizeof(Tile));
+    G.h_tiles = tiles;
     ECHK(hipMemcpyAsync(G.d_desc, hd, G.active.size() * sizeof(SolveDesc<T>), hipMemcpyHostToDevice, stream));
-    ECHK(hipMemcpyAsync(G.d_tiles, G.h_tiles, tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+    ECHK(hipMemcpyAsync(G.d_tiles, G.h_tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
   };
   if (g == 0) build(double{});
   else build(float{});
@@ -266,7 +257,7 @@ void ns3d_engine::run() {
       const int nact = (int)order.size();
       max_active = std::max(max_active, (double)nact);
       for (int i = 0; i < nact; ++i) h_stptr[i] = order[i]->prec == 0 ? order[i]->d64.w.st : order[i]->d32.w.st;
-      ECHK(hipMemcpyAsync(d_stptr, h_stptr, nact * sizeof(BState*), hipMemcpyHostToDevice, stream));
+      ECHK(hipMemcpyAsync(d_stptr, h_stptr.data(), nact * sizeof(BState*), hipMemcpyHostToDevice, stream));
       // one chunk of iterations; the middle one carries dispatch events
       const int qt = chunk / 2;
       for (int q = 0; q < chunk; ++q) {

@@ -408,25 +408,21 @@ bicg_A_kernel(Batch<T> B) {
   const NS3D_GLOBAL T* __restrict__ vold = par ? D.w.v[1] : D.w.v[0];
   NS3D_GLOBAL T* __restrict__ pnew = par ? D.w.p[0] : D.w.p[1];
   NS3D_GLOBAL T* __restrict__ vnew = par ? D.w.v[0] : D.w.v[1];
+  // launches queued after the solve stopped exit here, before any row load
+  if (st->status != BS_RUNNING || st->it_c != it) return;
   double acc = 0.0;
   bool stop = false;
   by_layout(D.S.layout, [&](auto LC) {
     constexpr int L = decltype(LC)::value;
     const int64_t n = D.S.n;
     int64_t row = tl.begin + threadIdx.x;
-    // first row, state and partials in flight together (rows of a stopped
-    // solve are fetched for nothing and dropped)
+    // first row, scalars and partials in flight together
     AIn<T> cur;
     if (row < tl.end) a_fetch<T, L>(D.S, D.w, pold, vold, (int)row, n, cur);
-    const int status = st->status;
-    const long long itc = st->it_c, fresh_at = st->fresh_at;
+    const long long fresh_at = st->fresh_at;
     const double rho_prev = st->rho[sl ^ 1], alpha_prev = st->alpha[sl ^ 1], w_prev = st->w[sl ^ 1];
     double red[2];
     sum_partials_all<2>(D.slot(kSlotC), D.t1 - D.t0, red, D.pstride);
-    if (status != BS_RUNNING || itc != it) {
-      stop = true;
-      return;
-    }
     double rho;
     if (fresh_at == it) {
       rho = st->rho_fresh;
@@ -487,23 +483,17 @@ bicg_B_kernel(Batch<T> B) {
   const int sl = (int)(it & 1);
   const int par = sl ^ 1;
   const NS3D_GLOBAL T* __restrict__ vn = par ? D.w.v[1] : D.w.v[0];
+  if (st->status != BS_RUNNING || st->it_a != it) return;
   double a0 = 0.0, a1 = 0.0;
-  bool stop = false;
   by_layout(D.S.layout, [&](auto LC) {
     constexpr int L = decltype(LC)::value;
     const int64_t n = D.S.n;
     int64_t row = tl.begin + threadIdx.x;
     BIn<T> cur;
     if (row < tl.end) b_fetch<T, L>(D.S, D.w, vn, (int)row, n, cur);
-    const int status = st->status;
-    const long long ita = st->it_a;
     const double rho = st->rho[sl];
     double red[1];
     sum_partials_all<1>(D.slot(kSlotA), D.t1 - D.t0, red, D.pstride);
-    if (status != BS_RUNNING || ita != it) {
-      stop = true;
-      return;
-    }
     const double alpha = rho / red[0];
     if (lead) {
       st->r0v = red[0];
@@ -525,7 +515,6 @@ bicg_B_kernel(Batch<T> B) {
       row = nxt;
     }
   });
-  if (stop) return;
   double v2[2] = {a0, a1};
   block_sums_at<2>(v2, D.slot(kSlotB), D.pstride, t - D.t0);
 }
@@ -547,15 +536,13 @@ bicg_C_kernel(Batch<T> B) {
   const NS3D_GLOBAL T* __restrict__ invdiag = D.S.invdiag;
   const NS3D_GLOBAL T* __restrict__ pn = par ? D.w.p[1] : D.w.p[0];
   const NS3D_GLOBAL T* __restrict__ vn = par ? D.w.v[1] : D.w.v[0];
+  if (st->status != BS_RUNNING || st->it_a != it) return;
   int64_t row = tl.begin + threadIdx.x;
   CIn<T> cur;
   if (row < tl.end) c_fetch<T>(invdiag, x, D.w, pn, vn, row, cur);
-  const int status = st->status;
-  const long long ita = st->it_a;
   const double alpha = st->alpha[sl];
   double red[2];
   sum_partials_all<2>(D.slot(kSlotB), D.t1 - D.t0, red, D.pstride);
-  if (status != BS_RUNNING || ita != it) return;
   const double om = (red[1] > 0.0) ? red[0] / red[1] : 0.0;
   if (lead) {
     st->ts = red[0];
