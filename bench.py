@@ -18,7 +18,7 @@ requests:
   served_fp64   the native server: all requests submitted at t = 0, worker
                 slots share one solve engine (batched BiCGSTAB), FP64;
   served_mixed  the same with the mixed form (FP64 defect correction around
-                FP32-storage BiCGSTAB), FP64 fallback on non-convergence --
+                FP32-storage BiCGSTAB), FP64 rerun of a failed request --
                 the headline value, timed over `steps` passes;
   served_mixed_poisson  the same requests arriving as a Poisson stream at a
                 fraction of the measured burst throughput, for latency under
@@ -72,7 +72,7 @@ def served_pass(reqs, args, precision, label, arrivals=None):
     (seconds from the start) or None for a burst at t = 0."""
     from ns3d_amd import solver
     srv = solver.Server(device=0, budget_bytes=args.budget_gb * 1e9, max_slots=args.slots, chunk=args.chunk,
-                        fallback=True)
+                        fallback=solver.FALLBACK_FAILED)
     try:
         t0 = time.perf_counter()
         tickets = []

@@ -238,7 +238,9 @@ ns3d_status ns3d_timing_read(ns3d_ctx* ctx, double out[36]);
 typedef struct ns3d_engine ns3d_engine;
 ns3d_status ns3d_engine_create(int device, int chunk, ns3d_engine** out);
 void ns3d_engine_destroy(ns3d_engine* eng);
-/* eng NULL detaches. */
+/* eng NULL detaches.  Attaching also gives the context two more solver
+ * workspaces, so that the three momentum solves of
+ * ns3d_compute_velocity_field go to the engine together. */
 ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
 /* out[32]: chunks, launches, solves done, max active solves, sum of rows x
  * iterations; then per precision q (0 FP64, 1 FP32) and kernel k (A, B, C),
@@ -247,9 +249,10 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
 ns3d_status ns3d_engine_stats(ns3d_engine* eng, double out[32]);
 
 /* Upper bound of the device bytes a context of an nx x ny x nz image with
- * nfluid fluid voxels allocates (mixed != 0: with the mixed form's buffers),
- * for admission control; ns3d_ctx_bytes gives what a context did request. */
-ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, double* bytes);
+ * nfluid fluid voxels allocates (mixed != 0: with the mixed form's buffers;
+ * lanes 1: alone, 3: attached to an engine), for admission control;
+ * ns3d_ctx_bytes gives what a context did request. */
+ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, int lanes, double* bytes);
 ns3d_status ns3d_ctx_bytes(ns3d_ctx* ctx, double* bytes);
 
 /* Stored entries (diagonal + existing neighbours) of the u, v, w, p systems. */

@@ -52,8 +52,8 @@ typedef struct {
                                    p solves, corrections, diagnostics, other */
   int64_t solver_work[4];       /* summed executed BiCGSTAB iterations (a zero
                                    right-hand side counts 0, not maxit) */
-  int32_t fallback;             /* served requests: 1 if the mixed-precision run
-                                   did not converge and this is the FP64 rerun */
+  int32_t fallback;             /* served requests: 1 if an FP64 run replaced or
+                                   continued the mixed-precision one */
   int32_t pad;
 } ns3d_run_result;
 
@@ -91,11 +91,21 @@ const char* ns3d_host_last_error(void);
  * (ns3d_footprint) fits in what budget_bytes has left.  All running requests
  * share one solve engine (ns3d_engine_create with `chunk`), so their
  * BiCGSTAB solves run batched and a new request joins at the next chunk
- * boundary.  fallback != 0: a mixed-precision request that does not
- * converge is re-run in FP64 (result field `fallback`).  The image is copied
+ * boundary.  fallback (NS3D_FALLBACK_*): what happens to a mixed-precision
+ * request whose run fails or ends unconverged (result field `fallback` is 1
+ * when an FP64 run replaced or continued it).  Its linear solves meet the
+ * same FP64 tolerances as the plain path, so a mixed run that stops at its
+ * outer-iteration cap is where the FP64 run would be too; continuing it in
+ * FP64 (mode 2) is for callers who want that second opinion.  The image is copied
  * at submission; outdir/name strings too.  times[4] of poll/wait: submit,
  * start and end time (seconds since the server was created) and the
  * projected footprint in bytes. */
+enum {
+  NS3D_FALLBACK_NONE = 0,        /* the mixed result stands as it is */
+  NS3D_FALLBACK_FAILED = 1,      /* a failed run or non-finite K / residual: rerun in FP64 */
+  NS3D_FALLBACK_UNCONVERGED = 2  /* also: unconverged at the cap -> continue in FP64 from its fields
+                                    with max(10, maxitSIMPLE / 4) more outer iterations */
+};
 typedef struct ns3d_server ns3d_server;
 int ns3d_server_create(int device, double budget_bytes, int max_slots, int chunk, int fallback,
                        ns3d_server** out);

@@ -67,6 +67,21 @@ struct Timing {
   double ms[2][2][3] = {}, launches[2][2][3] = {}, bytes[2][2][3] = {};
 };
 
+// Workspace of one linear solve in flight.
+struct Lane {
+  double* vb = nullptr;     // momentum right-hand side
+  double* temp = nullptr;   // saved field for revert/clamp
+  BWork bw{};
+  SWork<float> bw32{};      // mixed form: FP32 inner-solve vectors (state shared with bw)
+  float* d32 = nullptr;     // inner-solve unknown (the correction)
+  float* r32 = nullptr;     // inner-solve right-hand side
+  double* rres = nullptr;   // FP64 residual of the defect correction
+  double* partials = nullptr;  // [kPartialSlots][kMaxTilesPerSolve]
+  double* result = nullptr;    // reduction results (device)
+  BState* hstate = nullptr;    // pinned copy of the solver state
+  double* hresult = nullptr;   // pinned copy of results
+};
+
 }  // namespace
 
 struct ns3d_ctx {
@@ -82,20 +97,14 @@ struct ns3d_ctx {
   double* p = nullptr;
   double* pc = nullptr;
   double* b = nullptr;
-  double* vb = nullptr;     // momentum right-hand side
-  double* temp = nullptr;   // saved field for revert/clamp
-  // solver workspace
-  BWork bw{};
-  SWork<float> bw32{};      // mixed form: FP32 inner-solve vectors
-  float* d32 = nullptr;     // inner-solve unknown (the correction)
-  float* r32 = nullptr;     // inner-solve right-hand side
-  double* rres = nullptr;   // FP64 residual of the defect correction
+  // solver workspaces: lane 0 serves every solve on the context's stream and
+  // the reductions; with an engine attached, lanes 1 and 2 let the three
+  // momentum solves of one ComputeVelocityField run at once
+  Lane ln[3];
+  int nlanes = 0;
+  int64_t lane_rows = 0;
   int precision = NS3D_PRECISION_FP64;
   double inner_tol = 1e-4;
-  double* partials = nullptr;        // [kPartialSlots][kMaxTilesPerSolve]
-  double* result = nullptr;          // reduction results (device)
-  BState* hstate = nullptr;          // pinned copy of the solver state
-  double* hresult = nullptr;         // pinned copy of results
   ns3d_params prm{};
   bool have_params = false;
   std::vector<void*> allocs;
@@ -196,43 +205,46 @@ double kernel_bytes(int64_t n, int64_t nnz, int k, int vb) {
   return mat + streams[k] * vb * (double)n;
 }
 
-// Queued BiCGSTAB iterations with status polling between chunks.  Leaves the
-// final solver state in c->hstate.  hint: iterations of the previous solve of
-// this system (first chunk size), updated on return.
+// One BiCGSTAB call: system, matrix, unknown (with its guess), right-hand
+// side, vectors, tolerance, cap, the lane whose partials and pinned state it
+// uses, and the chunk-size hint of its system.
 template <typename T>
-void bicg_loop(ns3d_ctx* c, int sid, SMat<T> M, T* x, const T* b, SWork<T> w, double tol, int64_t maxit,
-               int64_t& hint, int kind) {
+struct SolveJob {
+  int sid = 0;
+  SMat<T> M{};
+  T* x = nullptr;
+  const T* b = nullptr;
+  SWork<T> w{};
+  double tol = 0;
+  int64_t maxit = 0;
+  Lane* L = nullptr;
+  int64_t* hint = nullptr;
+};
+
+// Queued BiCGSTAB iterations on the context's stream with status polling
+// between chunks.  Leaves the final solver state in the lane's hstate.  The
+// hint (iterations of the previous solve of this system) sizes the first
+// chunk and is updated on return.
+template <typename T>
+void stream_solve(ns3d_ctx* c, const SolveJob<T>& j) {
   hipStream_t s = c->stream;
-  const SysMem& S = c->sys[sid];
-  if (c->engine) {
-    // batched: the engine runs this solve together with the other contexts'
-    EngineRequest req;
-    req.prec = sizeof(T) == 4 ? 1 : 0;
-    const SolveDesc<T> d{M, w, x, b, tol, (long long)maxit, c->partials, kMaxTilesPerSolve, 0, 0, 0};
-    if constexpr (sizeof(T) == 4) req.d32 = d;
-    else req.d64 = d;
-    req.n = S.n;
-    req.nnz = S.nnz;
-    req.kind = kind;
-    HCHK(hipEventRecord(c->ev[0], s));
-    req.ready = c->ev[0];
-    c->engine->solve(req);
-    *c->hstate = req.result;
-    if (c->hstate->status != BS_ZERO_RHS) hint = c->hstate->it_c > 0 ? c->hstate->it_c : 4;
-    return;
-  }
+  const SysMem& S = c->sys[j.sid];
+  const int kind = j.sid == 3 ? 1 : 0;
+  BState* hs = j.L->hstate;
   // one solve: its descriptor travels by value in the kernel arguments
-  const SolveDesc<T> d{M, w, x, b, tol, (long long)maxit, c->partials, kMaxTilesPerSolve, 0, 0, 0};
+  const SolveDesc<T> d{j.M, j.w, j.x, j.b, j.tol, (long long)j.maxit, j.L->partials, kMaxTilesPerSolve, 0, 0, 0};
   HCHK(launch_bicg_init<T>(single_batch(d, 0, 0), s));
-  HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
+  HCHK(hipMemcpyAsync(hs, j.w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));
+  int64_t& hint = *j.hint;
   int64_t chunk = hint > 4 ? (hint * 3) / 4 : 4;
   int64_t launched = 0;
+  const int64_t maxit = j.maxit;
   const int64_t cap = 4 * (maxit > 0 ? maxit : 1) + 64;
   const int prec = sizeof(T) == 4 ? 1 : 0;
-  while (c->hstate->status == BS_RUNNING || c->hstate->status == BS_RESTART) {
-    const long long done_before = c->hstate->it_c;
-    if (c->hstate->status == BS_RESTART) HCHK(launch_bicg_restart<T>(single_batch(d, done_before, 0), s));
+  while (hs->status == BS_RUNNING || hs->status == BS_RESTART) {
+    const long long done_before = hs->it_c;
+    if (hs->status == BS_RESTART) HCHK(launch_bicg_restart<T>(single_batch(d, done_before, 0), s));
     if (chunk > maxit + 1) chunk = maxit + 1;
     // with timing on, the kernels of one iteration in the middle of the
     // chunk carry start/stop events, read back after the chunk's sync if
@@ -245,9 +257,9 @@ void bicg_loop(ns3d_ctx* c, int sid, SMat<T> M, T* x, const T* b, SWork<T> w, do
       HCHK(launch_bicg_B<T>(B, s, e ? e + 2 : nullptr));
       HCHK(launch_bicg_C<T>(B, s, e ? e + 4 : nullptr));
     }
-    HCHK(hipMemcpyAsync(c->hstate, w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
+    HCHK(hipMemcpyAsync(hs, j.w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
-    if (qt >= 0 && c->hstate->it_c - done_before > qt)
+    if (qt >= 0 && hs->it_c - done_before > qt)
       for (int k = 0; k < 3; ++k) {
         float ms = 0.f;
         HCHK(hipEventElapsedTime(&ms, c->tev[2 * k], c->tev[2 * k + 1]));
@@ -259,112 +271,275 @@ void bicg_loop(ns3d_ctx* c, int sid, SMat<T> M, T* x, const T* b, SWork<T> w, do
     if (launched > cap) throw Invalid{"BiCGSTAB did not stop within its iteration cap"};
     chunk = 16;
   }
-  if (c->hstate->status != BS_ZERO_RHS) hint = c->hstate->it_c > 0 ? c->hstate->it_c : 4;
+  if (hs->status != BS_ZERO_RHS) hint = hs->it_c > 0 ? hs->it_c : 4;
 }
 
-// One BiCGSTAB solve on system sid with x as guess, Eigen semantics (FP64).
-void bicgstab64(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int64_t maxit, ns3d_solve_info* info) {
-  SysMem& S = c->sys[sid];
-  if (S.n == 0) {
-    info->iterations = 0; info->info = 0; info->restarts = 0; info->error = 0; info->work = 0;
+// Runs the solves, each leaving its final state in its lane's hstate: with an
+// engine attached all at once (batched with the other contexts' solves),
+// else one after the other on the context's stream.  Each solve's arithmetic
+// is the same either way.
+template <typename T>
+void run_solves(ns3d_ctx* c, SolveJob<T>* jobs, int n) {
+  if (n <= 0) return;
+  if (!c->engine) {
+    for (int i = 0; i < n; ++i) stream_solve<T>(c, jobs[i]);
     return;
   }
-  bicg_loop<double>(c, sid, S.mat64(), x, b, c->bw, tol, maxit, S.last_iters, sid == 3 ? 1 : 0);
-  const BState& st = *c->hstate;
-  if (st.status == BS_ZERO_RHS) {
-    HCHK(launch_fill(x, S.n, 0.0, c->stream));
-    info->iterations = maxit;
-    info->error = tol;
-    info->info = 0;
-    info->restarts = 0;
-    info->work = 0;
-  } else {
-    info->iterations = st.it_c - st.ibase;
-    info->work = st.it_c;
-    info->error = std::sqrt(st.rr / st.rhs_sqnorm);
-    info->info = (info->error <= tol) ? 0 : 2;
-    info->restarts = (int32_t)st.restarts;
+  if (n > 3) throw Invalid{"run_solves: at most three solves at once"};
+  EngineRequest req[3];
+  EngineRequest* rp[3];
+  HCHK(hipEventRecord(c->ev[0], c->stream));
+  for (int i = 0; i < n; ++i) {
+    const SolveJob<T>& j = jobs[i];
+    const SolveDesc<T> d{j.M, j.w, j.x, j.b, j.tol, (long long)j.maxit, j.L->partials, kMaxTilesPerSolve, 0, 0, 0};
+    req[i].prec = sizeof(T) == 4 ? 1 : 0;
+    if constexpr (sizeof(T) == 4) req[i].d32 = d;
+    else req[i].d64 = d;
+    req[i].n = c->sys[j.sid].n;
+    req[i].nnz = c->sys[j.sid].nnz;
+    req[i].kind = j.sid == 3 ? 1 : 0;
+    req[i].ready = c->ev[0];
+    rp[i] = &req[i];
+  }
+  c->engine->solve_many(rp, n);
+  for (int i = 0; i < n; ++i) {
+    *jobs[i].L->hstate = req[i].result;
+    const BState& st = req[i].result;
+    if (st.status != BS_ZERO_RHS) *jobs[i].hint = st.it_c > 0 ? st.it_c : 4;
+  }
+}
+
+// n BiCGSTAB solves (system sid[i], unknown x[i] with its guess, right-hand
+// side b[i], lane L[i]; at most three, on distinct systems and lanes), Eigen
+// semantics, FP64.
+void bicgstab64_many(ns3d_ctx* c, int n, const int* sid, double* const* x, const double* const* b, double tol,
+                     int64_t maxit, ns3d_solve_info* info, Lane* const* L) {
+  SolveJob<double> jobs[3];
+  int idx[3];
+  int m = 0;
+  for (int i = 0; i < n; ++i) {
+    SysMem& S = c->sys[sid[i]];
+    if (S.n == 0) {
+      info[i].iterations = 0; info[i].info = 0; info[i].restarts = 0; info[i].error = 0; info[i].work = 0;
+      continue;
+    }
+    jobs[m] = SolveJob<double>{sid[i], S.mat64(), x[i], b[i], L[i]->bw, tol, maxit, L[i], &S.last_iters};
+    idx[m++] = i;
+  }
+  run_solves<double>(c, jobs, m);
+  for (int q = 0; q < m; ++q) {
+    const int i = idx[q];
+    const BState& st = *L[i]->hstate;
+    if (st.status == BS_ZERO_RHS) {
+      HCHK(launch_fill(x[i], c->sys[sid[i]].n, 0.0, c->stream));
+      info[i].iterations = maxit;
+      info[i].error = tol;
+      info[i].info = 0;
+      info[i].restarts = 0;
+      info[i].work = 0;
+    } else {
+      info[i].iterations = st.it_c - st.ibase;
+      info[i].work = st.it_c;
+      info[i].error = std::sqrt(st.rr / st.rhs_sqnorm);
+      info[i].info = (info[i].error <= tol) ? 0 : 2;
+      info[i].restarts = (int32_t)st.restarts;
+    }
   }
 }
 
 // Mixed form: FP64 defect correction around FP32 inner BiCGSTAB solves.
 //   r = b - A x (FP64); while |r| > tol |b|: solve A32 d = r32 to a relative
 //   tolerance max(inner_tol, tol |b| / (2 |r|)); x += d; r = b - A x.
-// iterations = summed inner iterations; error = |b - A x| / |b| (true FP64
-// residual); a zero right-hand side and an already-converged guess behave as
-// in the plain solve (x = 0 with iterations = maxit, and 0 iterations).
-void bicgstab_mixed(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int64_t maxit,
-                    ns3d_solve_info* info) {
-  SysMem& S = c->sys[sid];
+// A sweep that does nothing, or after which |r| has not halved, means FP32
+// has reached its accuracy on this system: the solve is finished in FP64
+// from the current iterate.  iterations = summed inner iterations; error =
+// |b - A x| / |b| (true FP64 residual); a zero right-hand side and an
+// already-converged guess behave as in the plain solve (x = 0 with
+// iterations = maxit, and 0 iterations).  The n solves advance in lockstep
+// (their sweeps and finishes run together), each with exactly the steps it
+// would take alone.
+void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, const double* const* b, double tol,
+                         int64_t maxit, ns3d_solve_info* info, Lane* const* L) {
   hipStream_t s = c->stream;
-  info->restarts = 0;
-  info->work = 0;
-  if (S.n == 0) {
-    info->iterations = 0; info->info = 0; info->error = 0;
-    return;
-  }
-  const SMat<double> M64 = S.mat64();
-  const SMat<float> M32 = S.mat32();
-  auto residual = [&](double& rr, double& bb) {
-    HCHK(launch_residual64(M64, x, b, c->rres, c->r32, c->partials, c->result, s));
-    HCHK(hipMemcpyAsync(c->hresult, c->result, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    HCHK(hipStreamSynchronize(s));
-    rr = c->hresult[0];
-    bb = c->hresult[1];
+  struct Track {
+    double rr = 0, bb = 0, tol2 = 0, rr_old = 0, err = 0;
+    int64_t total = 0;
+    int outer = 0;
+    bool live = false, stagnated = false;
   };
-  double rr, bb;
-  residual(rr, bb);
-  if (bb == 0) {
-    HCHK(launch_fill(x, S.n, 0.0, s));
-    info->iterations = maxit; info->error = tol; info->info = 0;
-    return;
-  }
-  const double tol2 = tol * tol * bb;
-  int64_t total = 0;
-  int outer = 0;
-  bool stagnated = false;
-  while (rr > tol2 && total < maxit && outer < 100) {
-    const double rel = std::sqrt(rr / bb);
-    const double tin = std::max(c->inner_tol, 0.5 * tol / rel);
-    HCHK(launch_fill_f32(c->d32, S.n, s));
-    bicg_loop<float>(c, sid, M32, c->d32, c->r32, c->bw32, tin, maxit - total, S.last_iters32, sid == 3 ? 1 : 0);
-    const BState& st = *c->hstate;
-    const int64_t k = (st.status == BS_ZERO_RHS) ? 0 : st.it_c;
-    info->restarts += (int32_t)st.restarts;
-    total += k;
-    ++outer;
-    if (k == 0) {  // the FP32 sweep cannot act on this residual
-      stagnated = true;
-      break;
+  Track t[3];
+  bool use[3] = {false, false, false};
+  auto residuals = [&](const bool* which) {
+    bool any = false;
+    for (int i = 0; i < n; ++i) {
+      if (!which[i]) continue;
+      HCHK(launch_residual64(c->sys[sid[i]].mat64(), x[i], b[i], L[i]->rres, L[i]->r32, L[i]->partials,
+                             L[i]->result, s));
+      HCHK(hipMemcpyAsync(L[i]->hresult, L[i]->result, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+      any = true;
     }
-    HCHK(launch_add_f32(x, c->d32, S.n, s));
-    const double rr_old = rr;
-    residual(rr, bb);
-    if (rr > 0.25 * rr_old) {  // |r| fell by less than 2x: FP32 has reached its accuracy here
-      stagnated = true;
-      break;
+    if (!any) return;
+    HCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i)
+      if (which[i]) {
+        t[i].rr = L[i]->hresult[0];
+        t[i].bb = L[i]->hresult[1];
+      }
+  };
+  for (int i = 0; i < n; ++i) {
+    info[i].restarts = 0;
+    info[i].work = 0;
+    use[i] = c->sys[sid[i]].n > 0;
+    if (!use[i]) { info[i].iterations = 0; info[i].info = 0; info[i].error = 0; }
+  }
+  residuals(use);
+  for (int i = 0; i < n; ++i) {
+    if (!use[i]) continue;
+    if (t[i].bb == 0) {
+      HCHK(launch_fill(x[i], c->sys[sid[i]].n, 0.0, s));
+      info[i].iterations = maxit; info[i].error = tol; info[i].info = 0;
+      use[i] = false;
+      continue;
+    }
+    t[i].tol2 = tol * tol * t[i].bb;
+    t[i].live = true;
+  }
+  for (;;) {
+    SolveJob<float> jobs[3];
+    int idx[3];
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+      Track& k = t[i];
+      if (!use[i] || !k.live) continue;
+      if (!(k.rr > k.tol2 && k.total < maxit && k.outer < 100)) {
+        k.live = false;
+        continue;
+      }
+      SysMem& S = c->sys[sid[i]];
+      const double rel = std::sqrt(k.rr / k.bb);
+      const double tin = std::max(c->inner_tol, 0.5 * tol / rel);
+      HCHK(launch_fill_f32(L[i]->d32, S.n, s));
+      jobs[m] = SolveJob<float>{sid[i], S.mat32(), L[i]->d32, L[i]->r32, L[i]->bw32, tin, maxit - k.total, L[i],
+                                &S.last_iters32};
+      idx[m++] = i;
+    }
+    if (m == 0) break;
+    run_solves<float>(c, jobs, m);
+    bool again[3] = {false, false, false};
+    for (int q = 0; q < m; ++q) {
+      const int i = idx[q];
+      Track& k = t[i];
+      const BState& st = *L[i]->hstate;
+      const int64_t its = (st.status == BS_ZERO_RHS) ? 0 : st.it_c;
+      info[i].restarts += (int32_t)st.restarts;
+      k.total += its;
+      ++k.outer;
+      if (its == 0) {  // the FP32 sweep cannot act on this residual
+        k.stagnated = true;
+        k.live = false;
+        continue;
+      }
+      HCHK(launch_add_f32(x[i], L[i]->d32, c->sys[sid[i]].n, s));
+      k.rr_old = k.rr;
+      again[i] = true;
+    }
+    residuals(again);
+    for (int i = 0; i < n; ++i)
+      if (again[i] && t[i].rr > 0.25 * t[i].rr_old) {  // |r| fell by less than 2x
+        t[i].stagnated = true;
+        t[i].live = false;
+      }
+  }
+  // finish in FP64 from the current iterate (same tolerance, the remaining
+  // iteration budget) where FP32 stagnated short of the tolerance
+  SolveJob<double> fin[3];
+  int idx[3];
+  int m = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!use[i]) continue;
+    Track& k = t[i];
+    k.err = std::sqrt(k.rr / k.bb);
+    if (k.stagnated && k.rr > k.tol2 && k.total < maxit) {
+      SysMem& S = c->sys[sid[i]];
+      fin[m] = SolveJob<double>{sid[i], S.mat64(), x[i], b[i], L[i]->bw, tol, maxit - k.total, L[i], &S.last_iters};
+      idx[m++] = i;
     }
   }
-  double err = std::sqrt(rr / bb);
-  if (stagnated && rr > tol2 && total < maxit) {
-    // finish this solve in FP64 from the current iterate (same tolerance,
-    // the remaining iteration budget)
-    bicg_loop<double>(c, sid, M64, x, b, c->bw, tol, maxit - total, S.last_iters, sid == 3 ? 1 : 0);
-    const BState& st = *c->hstate;
-    total += st.it_c;
-    info->restarts += (int32_t)st.restarts;
-    err = std::sqrt(st.rr / st.rhs_sqnorm);
+  run_solves<double>(c, fin, m);
+  for (int q = 0; q < m; ++q) {
+    const int i = idx[q];
+    const BState& st = *L[i]->hstate;
+    t[i].total += st.it_c;
+    info[i].restarts += (int32_t)st.restarts;
+    t[i].err = std::sqrt(st.rr / st.rhs_sqnorm);
     c->fp64_finishes += 1;
   }
-  info->iterations = total;
-  info->work = total;
-  info->error = err;
-  info->info = (info->error <= tol) ? 0 : 2;
+  for (int i = 0; i < n; ++i) {
+    if (!use[i]) continue;
+    info[i].iterations = t[i].total;
+    info[i].work = t[i].total;
+    info[i].error = t[i].err;
+    info[i].info = (info[i].error <= tol) ? 0 : 2;
+  }
 }
 
+bool mixed_ready(const ns3d_ctx* c, const int* sid, int n) {
+  if (c->precision != NS3D_PRECISION_MIXED) return false;
+  for (int i = 0; i < n; ++i)
+    if (c->sys[sid[i]].n > 0 && !c->sys[sid[i]].coef32) return false;
+  return true;
+}
+
+void bicgstab_many(ns3d_ctx* c, int n, const int* sid, double* const* x, const double* const* b, double tol,
+                   int64_t maxit, ns3d_solve_info* info, Lane* const* L) {
+  if (mixed_ready(c, sid, n)) bicgstab_mixed_many(c, n, sid, x, b, tol, maxit, info, L);
+  else bicgstab64_many(c, n, sid, x, b, tol, maxit, info, L);
+}
+
+// One BiCGSTAB solve on system sid with x as guess (lane 0).
 void bicgstab(ns3d_ctx* c, int sid, double* x, const double* b, double tol, int64_t maxit, ns3d_solve_info* info) {
-  if (c->precision == NS3D_PRECISION_MIXED && c->sys[sid].coef32) bicgstab_mixed(c, sid, x, b, tol, maxit, info);
-  else bicgstab64(c, sid, x, b, tol, maxit, info);
+  Lane* L = &c->ln[0];
+  bicgstab_many(c, 1, &sid, &x, &b, tol, maxit, info, &L);
+}
+
+// A lane's FP64 workspace (vectors of lane_rows rows, partials, state,
+// pinned copies), plus the FP32 one when the context is in the mixed form.
+void alloc_lane32(ns3d_ctx* c, Lane& L) {
+  if (L.d32) return;
+  const int64_t nmax = std::max<int64_t>(c->lane_rows, 1);
+  SWork<float>& w = L.bw32;
+  w.r = c->alloc<float>(nmax);
+  w.r0 = c->alloc<float>(nmax);
+  w.p[0] = c->alloc<float>(nmax);
+  w.p[1] = c->alloc<float>(nmax);
+  w.v[0] = c->alloc<float>(nmax);
+  w.v[1] = c->alloc<float>(nmax);
+  w.t = c->alloc<float>(nmax);
+  w.st = L.bw.st;
+  L.d32 = c->alloc<float>(nmax);
+  L.r32 = c->alloc<float>(nmax);
+  L.rres = c->alloc<double>(nmax);
+}
+
+void alloc_lane(ns3d_ctx* c, Lane& L) {
+  const int64_t nmax = std::max<int64_t>(c->lane_rows, 1);
+  L.vb = c->alloc<double>(nmax);
+  L.temp = c->alloc<double>(nmax);
+  BWork& w = L.bw;
+  w.r = c->alloc<double>(nmax);
+  w.r0 = c->alloc<double>(nmax);
+  w.p[0] = c->alloc<double>(nmax);
+  w.p[1] = c->alloc<double>(nmax);
+  w.v[0] = c->alloc<double>(nmax);
+  w.v[1] = c->alloc<double>(nmax);
+  w.t = c->alloc<double>(nmax);
+  L.partials = c->alloc<double>((int64_t)kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks));
+  w.st = c->alloc<BState>(1);
+  HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), c->stream));
+  L.result = c->alloc<double>(16);
+  HCHK(hipHostMalloc((void**)&L.hstate, sizeof(BState), hipHostMallocDefault));
+  HCHK(hipHostMalloc((void**)&L.hresult, 16 * sizeof(double), hipHostMallocDefault));
+  if (c->precision == NS3D_PRECISION_MIXED) alloc_lane32(c, L);
 }
 
 // FP32 copies of a system after its assembly (mixed form only).
@@ -502,22 +677,9 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
     c->b = c->alloc<double>(np);
     int nmax = np;
     for (int q = 0; q < 3; ++q) nmax = std::max(nmax, c->sys[q].n);
-    c->vb = c->alloc<double>(nmax);
-    c->temp = c->alloc<double>(nmax);
-    BWork& w = c->bw;
-    w.r = c->alloc<double>(nmax);
-    w.r0 = c->alloc<double>(nmax);
-    w.p[0] = c->alloc<double>(nmax);
-    w.p[1] = c->alloc<double>(nmax);
-    w.v[0] = c->alloc<double>(nmax);
-    w.v[1] = c->alloc<double>(nmax);
-    w.t = c->alloc<double>(nmax);
-    c->partials = c->alloc<double>((int64_t)kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks));
-    w.st = c->alloc<BState>(1);
-    HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), s));
-    c->result = c->alloc<double>(16);
-    HCHK(hipHostMalloc((void**)&c->hstate, sizeof(BState), hipHostMallocDefault));
-    HCHK(hipHostMalloc((void**)&c->hresult, 16 * sizeof(double), hipHostMallocDefault));
+    c->lane_rows = nmax;
+    alloc_lane(c, c->ln[0]);
+    c->nlanes = 1;
     HCHK(hipStreamSynchronize(s));
   });
   if (st != NS3D_OK) {
@@ -535,10 +697,15 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
 
 void ns3d_destroy(ns3d_ctx* c) {
   if (!c) return;
+  // rounds an engine queued before this context's last solve returned may
+  // still name its state buffers
+  if (c->engine) c->engine->fence();
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) (void)hipFree(p);
-  if (c->hstate) (void)hipHostFree(c->hstate);
-  if (c->hresult) (void)hipHostFree(c->hresult);
+  for (Lane& L : c->ln) {
+    if (L.hstate) (void)hipHostFree(L.hstate);
+    if (L.hresult) (void)hipHostFree(L.hresult);
+  }
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->tev)
@@ -649,16 +816,40 @@ ns3d_status ns3d_compute_velocity_field(ns3d_ctx* c, double alpha, int limit, do
     const Geo g = c->geo();
     hipStream_t s = c->stream;
     for (int q = 0; q < 3; ++q) info[q] = ns3d_solve_info{};
+    // the three systems are independent (each right-hand side reads the
+    // pressure and its own component only): with an engine and three lanes
+    // they are solved at once, else one after the other on lane 0
+    const bool together = c->engine && c->nlanes >= 3;
+    int sid[3];
+    double* xs[3];
+    const double* bs[3];
+    Lane* Ls[3];
+    int m = 0;
+    auto finish = [&](int q, Lane& L) {
+      const int64_t n = c->sys[q].n;
+      double* x = c->vel[NS3D_SET_VEL][q];
+      if (info[q].iterations == 0 || info[q].info != 0)
+        HCHK(hipMemcpyAsync(x, L.temp, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+      if (limit) HCHK(launch_clamp(x, L.temp, n, dvmax, s));
+    };
     for (int q = 0; q < 3; ++q) {
       SysMem& S = c->sys[q];
       if (q == 2 && S.n <= 0) break;
+      Lane& L = c->ln[together ? q : 0];
       double* x = c->vel[NS3D_SET_VEL][q];
-      HCHK(launch_rhs(q, g, S.view(), c->p, x, c->vb, ph, s));
-      HCHK(hipMemcpyAsync(c->temp, x, (size_t)S.n * 8, hipMemcpyDeviceToDevice, s));
-      bicgstab(c, q, x, c->vb, tol, maxit, &info[q]);
-      if (info[q].iterations == 0 || info[q].info != 0)
-        HCHK(hipMemcpyAsync(x, c->temp, (size_t)S.n * 8, hipMemcpyDeviceToDevice, s));
-      if (limit) HCHK(launch_clamp(x, c->temp, S.n, dvmax, s));
+      HCHK(launch_rhs(q, g, S.view(), c->p, x, L.vb, ph, s));
+      HCHK(hipMemcpyAsync(L.temp, x, (size_t)S.n * 8, hipMemcpyDeviceToDevice, s));
+      if (!together) {
+        bicgstab(c, q, x, L.vb, tol, maxit, &info[q]);
+        finish(q, L);
+        continue;
+      }
+      sid[m] = q; xs[m] = x; bs[m] = L.vb; Ls[m] = &L;
+      ++m;
+    }
+    if (together) {
+      bicgstab_many(c, m, sid, xs, bs, tol, maxit, info, Ls);
+      for (int q = 0; q < m; ++q) finish(q, c->ln[q]);
     }
     HCHK(hipStreamSynchronize(s));
   });
@@ -700,7 +891,7 @@ ns3d_status ns3d_compute_pressure_field(ns3d_ctx* c, double tol, int64_t maxit, 
   return guarded([&] {
     hipStream_t s = c->stream;
     const int64_t n = c->sys[3].n;
-    double* ptemp = c->temp;  // temp holds max(n) doubles
+    double* ptemp = c->ln[0].temp;  // temp holds max(n) doubles
     HCHK(hipMemcpyAsync(ptemp, c->p, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
     bicgstab(c, 3, ptemp, c->b, tol, maxit, info);
     int64_t work = info->work;
@@ -732,12 +923,12 @@ ns3d_status ns3d_correct_pressure(ns3d_ctx* c, double beta, double out[3]) {
   return guarded([&] {
     const int64_t n = c->sys[3].n;
     need(n > 0, "ns3d_correct_pressure: no pressure nodes");
-    HCHK(launch_correct_pressure(c->p, c->pc, n, beta, c->partials, c->result, c->stream));
-    HCHK(hipMemcpyAsync(c->hresult, c->result, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HCHK(launch_correct_pressure(c->p, c->pc, n, beta, c->ln[0].partials, c->ln[0].result, c->stream));
+    HCHK(hipMemcpyAsync(c->ln[0].hresult, c->ln[0].result, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HCHK(hipStreamSynchronize(c->stream));
-    out[0] = beta * c->hresult[0];
-    out[1] = beta * c->hresult[1];
-    out[2] = beta * std::sqrt(c->hresult[2]);
+    out[0] = beta * c->ln[0].hresult[0];
+    out[1] = beta * c->ln[0].hresult[1];
+    out[2] = beta * std::sqrt(c->ln[0].hresult[2]);
   });
 }
 
@@ -747,13 +938,13 @@ ns3d_status ns3d_compute_div(ns3d_ctx* c, int vel_set, double out[4]) {
     const Vel3 v = vel3(c, vel_set);
     if (c->sys[3].n == 0) { out[0] = out[1] = out[2] = out[3] = 0; return; }
     HCHK(launch_compute_div(c->geo(), c->sys[3].view(), v.u, v.v, v.w, c->prm.dx, c->prm.dy, c->prm.dz,
-                            c->partials, c->result, c->stream));
-    HCHK(hipMemcpyAsync(c->hresult, c->result, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                            c->ln[0].partials, c->ln[0].result, c->stream));
+    HCHK(hipMemcpyAsync(c->ln[0].hresult, c->ln[0].result, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HCHK(hipStreamSynchronize(c->stream));
-    out[0] = c->hresult[0];
-    out[1] = std::sqrt(c->hresult[1]);
-    out[2] = c->hresult[2] / (double)c->sys[3].n;
-    out[3] = c->hresult[3];
+    out[0] = c->ln[0].hresult[0];
+    out[1] = std::sqrt(c->ln[0].hresult[1]);
+    out[2] = c->ln[0].hresult[2] / (double)c->sys[3].n;
+    out[3] = c->ln[0].hresult[3];
   });
 }
 
@@ -761,10 +952,10 @@ ns3d_status ns3d_compute_permeability(ns3d_ctx* c, double out[5]) {
   if (!c || !c->have_params || !out) return fail_invalid("ns3d_compute_permeability: bad arguments");
   return guarded([&] {
     const int border = c->prm.border;
-    HCHK(launch_permeability(c->geo(), c->vel[NS3D_SET_VEL][0], c->p, border, c->partials, c->result, c->stream));
-    HCHK(hipMemcpyAsync(c->hresult, c->result, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HCHK(launch_permeability(c->geo(), c->vel[NS3D_SET_VEL][0], c->p, border, c->ln[0].partials, c->ln[0].result, c->stream));
+    HCHK(hipMemcpyAsync(c->ln[0].hresult, c->ln[0].result, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HCHK(hipStreamSynchronize(c->stream));
-    const double* r = c->hresult;
+    const double* r = c->ln[0].hresult;
     double vm = r[0];
     double p_in, p_out;
     if (border <= 0) {
@@ -825,16 +1016,9 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
     double* dx = c->alloc<double>(n);
     HCHK(hipMemcpyAsync(db, b, (size_t)n * 8, hipMemcpyHostToDevice, s));
     HCHK(hipMemcpyAsync(dx, x, (size_t)n * 8, hipMemcpyHostToDevice, s));
-    BWork& w = c->bw;
-    w.r = c->alloc<double>(n); w.r0 = c->alloc<double>(n);
-    w.p[0] = c->alloc<double>(n); w.p[1] = c->alloc<double>(n);
-    w.v[0] = c->alloc<double>(n); w.v[1] = c->alloc<double>(n);
-    w.t = c->alloc<double>(n);
-    c->partials = c->alloc<double>((int64_t)kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks));
-    w.st = c->alloc<BState>(1);
-    HCHK(hipMemsetAsync(w.st, 0, sizeof(BState), s));
-    HCHK(hipHostMalloc((void**)&c->hstate, sizeof(BState), hipHostMallocDefault));
-    HCHK(hipHostMalloc((void**)&c->hresult, 16 * sizeof(double), hipHostMallocDefault));
+    c->lane_rows = n;
+    alloc_lane(c, c->ln[0]);
+    c->nlanes = 1;
     bicgstab(c, 3, dx, db, tol, maxit, info);
     HCHK(hipMemcpyAsync(x, dx, (size_t)n * 8, hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
@@ -875,22 +1059,8 @@ ns3d_status ns3d_set_precision(ns3d_ctx* c, int mode, double inner_tol) {
   if (mode == NS3D_PRECISION_MIXED && !(inner_tol > 0 && inner_tol < 1))
     return fail_invalid("ns3d_set_precision: inner tolerance must be in (0, 1)");
   return guarded([&] {
-    if (mode == NS3D_PRECISION_MIXED && !c->d32) {
-      int64_t nmax = 1;
-      for (int q = 0; q < 4; ++q) nmax = std::max<int64_t>(nmax, c->sys[q].n);
-      SWork<float>& w = c->bw32;
-      w.r = c->alloc<float>(nmax);
-      w.r0 = c->alloc<float>(nmax);
-      w.p[0] = c->alloc<float>(nmax);
-      w.p[1] = c->alloc<float>(nmax);
-      w.v[0] = c->alloc<float>(nmax);
-      w.v[1] = c->alloc<float>(nmax);
-      w.t = c->alloc<float>(nmax);
-      w.st = c->bw.st;
-      c->d32 = c->alloc<float>(nmax);
-      c->r32 = c->alloc<float>(nmax);
-      c->rres = c->alloc<double>(nmax);
-    }
+    if (mode == NS3D_PRECISION_MIXED)
+      for (int q = 0; q < c->nlanes; ++q) alloc_lane32(c, c->ln[q]);
     c->precision = mode;
     if (mode == NS3D_PRECISION_MIXED) c->inner_tol = inner_tol;
     for (int q = 0; q < 4; ++q) refresh_f32(c, q);
@@ -922,6 +1092,16 @@ void ns3d_engine_destroy(ns3d_engine* e) { delete e; }
 ns3d_status ns3d_attach_engine(ns3d_ctx* c, ns3d_engine* e) {
   if (!c) return fail_invalid("ns3d_attach_engine: null context");
   if (e && e->device != c->device) return fail_invalid("ns3d_attach_engine: engine and context on different devices");
+  if (c->engine && c->engine != e) c->engine->fence();
+  if (e && c->nlanes < 3 && c->sys[0].n > 0) {
+    // lanes for the momentum solves to run at once
+    const ns3d_status st = guarded([&] {
+      for (int q = c->nlanes; q < 3; ++q) alloc_lane(c, c->ln[q]);
+      c->nlanes = 3;
+      HCHK(hipStreamSynchronize(c->stream));
+    });
+    if (st != NS3D_OK) return st;
+  }
   c->engine = e;
   return NS3D_OK;
 }
@@ -932,8 +1112,9 @@ ns3d_status ns3d_engine_stats(ns3d_engine* e, double out[32]) {
   return NS3D_OK;
 }
 
-ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, double* bytes) {
-  if (nx <= 0 || ny <= 0 || nz <= 0 || nfluid < 0 || !bytes) return fail_invalid("ns3d_footprint: bad arguments");
+ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, int lanes, double* bytes) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || nfluid < 0 || !bytes || (lanes != 1 && lanes != 3))
+    return fail_invalid("ns3d_footprint: bad arguments");
   const double N = (double)nx * ny * nz, plane = (double)ny * nz, nf = (double)nfluid;
   // rows: p, v, w <= fluid cells; u <= fluid cells + one outlet plane
   const double nu = nf + plane, nmax = nu, rows_all = 3 * nf + nu;
@@ -942,10 +1123,11 @@ ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, do
   b += rows_all * (4 + 24 + 56 + 8 + 8) + 3 * nmax * 8;  // cell, nbr, coef, invdiag, pk; rowsum (u, v, w)
   b += 4 * kMaxTilesPerSolve * sizeof(Tile);           // tile tables
   b += 3 * 8 * (3 * nf + plane) + 24 * nf;             // velocity sets, p, pc, b
-  b += (2 + 7) * 8 * nmax;                             // momentum rhs, saved field, BiCGSTAB vectors
-  b += 8.0 * kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks) + 4096;  // partials, state, descriptors
-  if (mixed) b += rows_all * (28 + 4) + nmax * (7 * 4 + 2 * 4 + 8);  // FP32 matrices, inner vectors, residual
-  b += 64.0 * 1024 * 48;                               // allocation rounding (48 allocations at 64 KiB)
+  // per lane: momentum rhs, saved field, BiCGSTAB vectors, partials, state
+  const double lane = (2 + 7) * 8 * nmax + 8.0 * kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks) + 1024;
+  b += lanes * lane + 4096;
+  if (mixed) b += rows_all * (28 + 4) + lanes * nmax * (7 * 4 + 2 * 4 + 8);  // FP32 matrices; inner vectors, residual
+  b += 64.0 * 1024 * (48 + 26 * (lanes - 1));          // allocation rounding (64 KiB per allocation)
   *bytes = b;
   return NS3D_OK;
 }

@@ -12,6 +12,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "kernels.h"
@@ -33,7 +34,13 @@ struct EngineRequest {
   hipEvent_t ready = nullptr;
   BState result{};
   bool done = false;
-  long long it_c = 0;  // engine-private: executed iterations at the last poll
+  // engine-private: admission serial; executed iterations expected when the
+  // next round is queued; iteration base of the last queued round and
+  // whether that round restarted this solve's group first
+  long long serial = 0;
+  long long it_next = 0;
+  long long it_queued = -1;
+  bool restart_queued = false;
 };
 
 }  // namespace ns3d
@@ -44,7 +51,14 @@ struct ns3d_engine {
   // Blocks until the solve has stopped (result holds the final state);
   // throws std::runtime_error if the engine failed.
   void solve(ns3d::EngineRequest& req);
+  // The same for n requests at once (admitted together, so they run in the
+  // same rounds); returns when all have stopped.
+  void solve_many(ns3d::EngineRequest* const* reqs, int n);
   void stats(double* out, int count);
+  // Returns once every round queued before the call has been processed, so
+  // no queued kernel refers to a solve that has already returned.  A context
+  // calls it before freeing memory its solves used.
+  void fence();
 
   int device = 0;
   int chunk = 8;
@@ -58,17 +72,20 @@ struct ns3d_engine {
     ns3d::Tile* d_itiles = nullptr;
     // host images of the tables, rewritten only between rounds (after the
     // stream has drained), so no copy in flight ever reads a changed buffer
-    std::vector<char> h_desc, h_idesc;
-    std::vector<ns3d::Tile> h_tiles, h_itiles;
+    // (two of each: a round is queued while the previous one is in flight)
+    std::vector<char> h_desc[2], h_idesc[2];
+    std::vector<ns3d::Tile> h_tiles[2], h_itiles[2];
     long long* d_it0 = nullptr;  // per active solve: executed iterations when the chunk is queued
-    std::vector<long long> h_it0;
+    std::vector<long long> h_it0[2];
     int ntiles = 0;
     bool dirty = true;
   };
   void run();
-  void admit(std::vector<ns3d::EngineRequest*>& reqs);
-  void rebuild(int g);
-  void upload_it0(int g);
+  void admit(std::vector<ns3d::EngineRequest*>& reqs, int buf);
+  void rebuild(int g, int buf);
+  void upload_it0(int g, int buf, bool restarted);
+  void queue_round(int buf, bool restart[2]);
+  bool process_round(int buf, bool restart[2]);
   template <typename T>
   ns3d::Batch<T> batch(int g, int q) const;
   void fail(const std::string& msg);
@@ -82,12 +99,21 @@ struct ns3d_engine {
   bool stopping = false;
   std::string fatal;
   Group grp[2];
-  ns3d::BState** d_stptr = nullptr;   // state pointers of all active solves
-  std::vector<ns3d::BState*> h_stptr;
-  ns3d::BState* d_states = nullptr;   // gathered copies
-  ns3d::BState* h_states = nullptr;
-  std::vector<ns3d::EngineRequest*> order;  // solve order of d_stptr
-  hipEvent_t tev[12] = {};
+  // per round buffer: state pointers of the round's solves, gathered states,
+  // the solves in gather order, the round's completion and sampled events
+  ns3d::BState** d_stptr[2] = {};
+  std::vector<ns3d::BState*> h_stptr[2];
+  ns3d::BState* d_states[2] = {};
+  ns3d::BState* h_states[2] = {};
+  // (serials of the round's solves: a solve retired by the previous round's
+  // results may still appear in this one, and its request is gone by then)
+  std::vector<long long> order[2];
+  std::unordered_map<long long, ns3d::EngineRequest*> live;
+  long long next_serial = 1;
+  std::vector<double> round_bytes[2];  // algorithmic bytes of A, B, C per group
+  hipEvent_t round_done[2] = {};
+  hipEvent_t tev[2][12] = {};
+  long long rounds_queued = 0, rounds_done = 0;
   std::vector<void*> allocs;
   // statistics
   double n_chunks = 0, n_launch = 0, n_done = 0, max_active = 0;

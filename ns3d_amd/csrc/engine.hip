@@ -75,15 +75,20 @@ ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) 
     g.d_tiles = (Tile*)dalloc(sizeof(Tile) * kEngMaxTiles);
     g.d_itiles = (Tile*)dalloc(sizeof(Tile) * kEngMaxTiles);
     g.d_it0 = (long long*)dalloc(sizeof(long long) * kEngMaxSolves);
-    g.h_it0.resize(kEngMaxSolves);
-    g.h_desc.resize(dsz * kEngMaxSolves);
-    g.h_idesc.resize(dsz * kEngMaxSolves);
+    for (int b = 0; b < 2; ++b) {
+      g.h_it0[b].resize(kEngMaxSolves);
+      g.h_desc[b].resize(dsz * kEngMaxSolves);
+      g.h_idesc[b].resize(dsz * kEngMaxSolves);
+    }
   }
-  d_stptr = (BState**)dalloc(sizeof(BState*) * 2 * kEngMaxSolves);
-  d_states = (BState*)dalloc(sizeof(BState) * 2 * kEngMaxSolves);
-  h_stptr.resize(2 * kEngMaxSolves);
-  ECHK(hipHostMalloc((void**)&h_states, sizeof(BState) * 2 * kEngMaxSolves, hipHostMallocDefault));
-  for (auto& e : tev) ECHK(hipEventCreate(&e));
+  for (int b = 0; b < 2; ++b) {
+    d_stptr[b] = (BState**)dalloc(sizeof(BState*) * 2 * kEngMaxSolves);
+    d_states[b] = (BState*)dalloc(sizeof(BState) * 2 * kEngMaxSolves);
+    h_stptr[b].resize(2 * kEngMaxSolves);
+    ECHK(hipHostMalloc((void**)&h_states[b], sizeof(BState) * 2 * kEngMaxSolves, hipHostMallocDefault));
+    ECHK(hipEventCreateWithFlags(&round_done[b], hipEventDisableTiming));
+    for (auto& e : tev[b]) ECHK(hipEventCreate(&e));
+  }
   thr = std::thread([this] { run(); });
 }
 
@@ -95,22 +100,43 @@ ns3d_engine::~ns3d_engine() {
   cv_work.notify_all();
   if (thr.joinable()) thr.join();
   if (stream) (void)hipStreamSynchronize(stream);
-  if (h_states) (void)hipHostFree(h_states);
+  for (int b = 0; b < 2; ++b) {
+    if (h_states[b]) (void)hipHostFree(h_states[b]);
+    if (round_done[b]) (void)hipEventDestroy(round_done[b]);
+    for (auto& e : tev[b])
+      if (e) (void)hipEventDestroy(e);
+  }
   for (void* p : allocs) (void)hipFree(p);
-  for (auto& e : tev)
-    if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
 void ns3d_engine::solve(EngineRequest& req) {
+  EngineRequest* r = &req;
+  solve_many(&r, 1);
+}
+
+void ns3d_engine::solve_many(EngineRequest* const* reqs, int n) {
   std::unique_lock<std::mutex> lk(mu);
   if (!fatal.empty()) throw std::runtime_error("solve engine failed: " + fatal);
   if (stopping) throw std::runtime_error("solve engine is shutting down");
-  req.done = false;
-  pending.push_back(&req);
+  for (int i = 0; i < n; ++i) {
+    reqs[i]->done = false;
+    pending.push_back(reqs[i]);
+  }
   cv_work.notify_one();
-  cv_done.wait(lk, [&] { return req.done || !fatal.empty(); });
-  if (!req.done) throw std::runtime_error("solve engine failed: " + fatal);
+  auto all_done = [&] {
+    for (int i = 0; i < n; ++i)
+      if (!reqs[i]->done) return false;
+    return true;
+  };
+  cv_done.wait(lk, [&] { return all_done() || !fatal.empty(); });
+  if (!all_done()) throw std::runtime_error("solve engine failed: " + fatal);
+}
+
+void ns3d_engine::fence() {
+  std::unique_lock<std::mutex> lk(mu);
+  const long long target = rounds_queued;
+  cv_done.wait(lk, [&] { return rounds_done >= target || !fatal.empty(); });
 }
 
 void ns3d_engine::stats(double* out, int count) {
@@ -145,7 +171,10 @@ Batch<T> ns3d_engine::batch(int g, int q) const {
   return B;
 }
 
-void ns3d_engine::admit(std::vector<EngineRequest*>& reqs) {
+// Host images of the tables (index buf) are rewritten only for a round queued
+// after the one that last used them has been processed, so no copy still in
+// flight reads a buffer that changed.
+void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
   for (EngineRequest* r : reqs)
     if (r->ready) ECHK(hipStreamWaitEvent(stream, r->ready, 0));
   for (int g = 0; g < 2; ++g) {
@@ -154,7 +183,7 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs) {
     int ns = 0;
     auto init = [&](auto tag) {
       using T = decltype(tag);
-      SolveDesc<T>* hd = (SolveDesc<T>*)G.h_idesc.data();
+      SolveDesc<T>* hd = (SolveDesc<T>*)G.h_idesc[buf].data();
       for (EngineRequest* r : reqs) {
         if (r->prec != g) continue;
         SolveDesc<T> d = desc_of<T>(r);
@@ -162,9 +191,10 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs) {
         hd[ns++] = d;
       }
       if (ns == 0) return;
-      G.h_itiles = tiles;
+      G.h_itiles[buf] = tiles;
       ECHK(hipMemcpyAsync(G.d_idesc, hd, ns * sizeof(SolveDesc<T>), hipMemcpyHostToDevice, stream));
-      ECHK(hipMemcpyAsync(G.d_itiles, G.h_itiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+      ECHK(hipMemcpyAsync(G.d_itiles, G.h_itiles[buf].data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                          stream));
       Batch<T> B{};
       B.d = (const SolveDesc<T>*)G.d_idesc;
       B.tiles = G.d_itiles;
@@ -176,28 +206,40 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs) {
     else init(float{});
     for (EngineRequest* r : reqs)
       if (r->prec == g) {
-        r->it_c = 0;
+        r->serial = next_serial++;
+        r->it_next = 0;
+        r->it_queued = -1;
+        r->restart_queued = false;
+        live[r->serial] = r;
         G.active.push_back(r);
       }
     if (ns) G.dirty = true;
   }
 }
 
-// Executed iterations of the group's active solves, in table order, for the
-// row kernels of the next chunk (launch q of the chunk is iteration it0 + q).
-void ns3d_engine::upload_it0(int g) {
+// Iteration bases of the group's active solves, in table order, for the
+// round being queued (launch q of the chunk is iteration it0 + q); each
+// solve is then expected to be a chunk further on.
+void ns3d_engine::upload_it0(int g, int buf, bool restarted) {
   Group& G = grp[g];
   if (G.active.empty()) return;
-  for (size_t i = 0; i < G.active.size(); ++i) G.h_it0[i] = G.active[i]->it_c;
-  ECHK(hipMemcpyAsync(G.d_it0, G.h_it0.data(), G.active.size() * sizeof(long long), hipMemcpyHostToDevice, stream));
+  for (size_t i = 0; i < G.active.size(); ++i) {
+    EngineRequest* r = G.active[i];
+    G.h_it0[buf][i] = r->it_next;
+    r->it_queued = r->it_next;
+    r->restart_queued = restarted;
+    r->it_next += chunk;
+  }
+  ECHK(hipMemcpyAsync(G.d_it0, G.h_it0[buf].data(), G.active.size() * sizeof(long long), hipMemcpyHostToDevice,
+                      stream));
 }
 
-void ns3d_engine::rebuild(int g) {
+void ns3d_engine::rebuild(int g, int buf) {
   Group& G = grp[g];
   std::vector<Tile> tiles;
   auto build = [&](auto tag) {
     using T = decltype(tag);
-    SolveDesc<T>* hd = (SolveDesc<T>*)G.h_desc.data();
+    SolveDesc<T>* hd = (SolveDesc<T>*)G.h_desc[buf].data();
     for (size_t i = 0; i < G.active.size(); ++i) {
       SolveDesc<T> d = desc_of<T>(G.active[i]);
       make_tiles((int)i, d.S.n, tiles, d.t0, d.t1);
@@ -205,25 +247,142 @@ void ns3d_engine::rebuild(int g) {
     }
     G.ntiles = (int)tiles.size();
     if (G.active.empty()) return;
-    G.h_tiles = tiles;
+    G.h_tiles[buf] = tiles;
     ECHK(hipMemcpyAsync(G.d_desc, hd, G.active.size() * sizeof(SolveDesc<T>), hipMemcpyHostToDevice, stream));
-    ECHK(hipMemcpyAsync(G.d_tiles, G.h_tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+    ECHK(hipMemcpyAsync(G.d_tiles, G.h_tiles[buf].data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                        stream));
   };
   if (g == 0) build(double{});
   else build(float{});
   G.dirty = false;
 }
 
+// Tables, restarts, one chunk, and the gather of the states of every solve
+// in the round, into buffer buf.
+void ns3d_engine::queue_round(int buf, bool restart[2]) {
+  for (int g = 0; g < 2; ++g) {
+    if (grp[g].dirty) rebuild(g, buf);
+    // the restart kernel acts only on solves whose status asks for it
+    const bool rs = restart[g] && !grp[g].active.empty();
+    if (rs) {
+      if (g == 0) ECHK(launch_bicg_restart<double>(batch<double>(0, 0), stream));
+      else ECHK(launch_bicg_restart<float>(batch<float>(1, 0), stream));
+    }
+    restart[g] = false;
+    upload_it0(g, buf, rs);
+  }
+  std::vector<long long>& ord = order[buf];
+  ord.clear();
+  int nact = 0;
+  round_bytes[buf].assign(6, 0.0);
+  for (int g = 0; g < 2; ++g)
+    for (EngineRequest* r : grp[g].active) {
+      ord.push_back(r->serial);
+      h_stptr[buf][nact++] = r->prec == 0 ? r->d64.w.st : r->d32.w.st;
+      for (int k = 0; k < 3; ++k) round_bytes[buf][3 * r->prec + k] += kernel_bytes_eng(r->n, r->nnz, k, r->prec ? 4 : 8);
+    }
+  max_active = std::max(max_active, (double)nact);
+  ECHK(hipMemcpyAsync(d_stptr[buf], h_stptr[buf].data(), nact * sizeof(BState*), hipMemcpyHostToDevice, stream));
+  // one chunk of iterations; the middle one carries dispatch events
+  const int qt = chunk / 2;
+  hipEvent_t* ev = tev[buf];
+  const bool g0 = !grp[0].active.empty(), g1 = !grp[1].active.empty();
+  for (int q = 0; q < chunk; ++q) {
+    const bool e = (q == qt);
+    if (g0) ECHK(launch_bicg_A<double>(batch<double>(0, q), stream, e ? ev : nullptr));
+    if (g1) ECHK(launch_bicg_A<float>(batch<float>(1, q), stream, e ? ev + 6 : nullptr));
+    if (g0) ECHK(launch_bicg_B<double>(batch<double>(0, q), stream, e ? ev + 2 : nullptr));
+    if (g1) ECHK(launch_bicg_B<float>(batch<float>(1, q), stream, e ? ev + 8 : nullptr));
+    if (g0) ECHK(launch_bicg_C<double>(batch<double>(0, q), stream, e ? ev + 4 : nullptr));
+    if (g1) ECHK(launch_bicg_C<float>(batch<float>(1, q), stream, e ? ev + 10 : nullptr));
+  }
+  n_launch += (double)chunk * 3 * ((g0 ? 1 : 0) + (g1 ? 1 : 0));
+  n_chunks += 1;
+  const int words = nact * (int)(sizeof(BState) / 8);
+  hipLaunchKernelGGL(gather_states_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, d_stptr[buf],
+                     d_states[buf], nact);
+  ECHK(hipGetLastError());
+  ECHK(hipMemcpyAsync(h_states[buf], d_states[buf], nact * sizeof(BState), hipMemcpyDeviceToHost, stream));
+  ECHK(hipEventRecord(round_done[buf], stream));
+  std::lock_guard<std::mutex> lk(mu);
+  rounds_queued += 1;
+}
+
+// After round buf has completed: sampled kernel times, then every solve of the
+// round that is still live.  The round queued since (with base it_queued)
+// advances a running solve by a chunk if that base equals its count, and
+// does nothing to it otherwise (its kernels exit at their iteration check);
+// a solve that asked for a restart is restarted by that round only if the
+// round restarts its group, else it is flagged for the next one.  Stopped
+// solves are retired and their callers woken.  Returns whether any was.
+bool ns3d_engine::process_round(int buf, bool restart[2]) {
+  ECHK(hipEventSynchronize(round_done[buf]));
+  for (int g = 0; g < 2; ++g) {
+    if (round_bytes[buf][3 * g] == 0) continue;
+    for (int k = 0; k < 3; ++k) {
+      float e = 0.f;
+      if (hipEventElapsedTime(&e, tev[buf][6 * g + 2 * k], tev[buf][6 * g + 2 * k + 1]) != hipSuccess) continue;
+      ms[g][k] += e;
+      launches[g][k] += 1;
+      bytes[g][k] += round_bytes[buf][3 * g + k];
+    }
+  }
+  bool any_done = false;
+  std::lock_guard<std::mutex> lk(mu);
+  const std::vector<long long>& ord = order[buf];
+  for (size_t i = 0; i < ord.size(); ++i) {
+    auto lv = live.find(ord[i]);
+    if (lv == live.end()) continue;  // retired by the previous round's results
+    EngineRequest* r = lv->second;
+    const BState& st = h_states[buf][i];
+    const bool next_runs = r->it_queued == st.it_c;
+    if (st.status == BS_RUNNING) {
+      r->it_next = next_runs ? st.it_c + chunk : st.it_c;
+      continue;
+    }
+    if (st.status == BS_RESTART) {
+      if (next_runs && r->restart_queued) {
+        r->it_next = st.it_c + chunk;
+      } else {
+        restart[r->prec] = true;
+        r->it_next = st.it_c;
+      }
+      continue;
+    }
+    live.erase(lv);
+    r->result = st;
+    r->done = true;
+    rows_x_iters += (double)r->n * (double)st.it_c;
+    n_done += 1;
+    any_done = true;
+    auto& act = grp[r->prec].active;
+    auto it = std::find(act.begin(), act.end(), r);
+    if (it != act.end()) {
+      act.erase(it);
+      grp[r->prec].dirty = true;
+    }
+  }
+  rounds_done += 1;
+  return any_done;
+}
+
+// Rounds are pipelined: round k+1 is queued before the states of round k are
+// processed, so the device keeps working while the host retires, admits and
+// rebuilds.  Round k+1 was queued before round k's stops were known: its
+// kernels of those solves exit at their status check, and their callers were
+// already woken -- fence() is what a context calls before freeing memory.
 void ns3d_engine::run() {
   try {
     ECHK(hipSetDevice(device));
     bool restart[2] = {false, false};
+    int qbuf = 0;       // buffer of the next round to queue
+    int inflight = -1;  // buffer of the queued round not yet processed
     for (;;) {
       std::vector<EngineRequest*> admitted;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_work.wait(lk, [&] { return stopping || !pending.empty() || any_active(); });
-        if (stopping && pending.empty() && !any_active()) break;
+        if (inflight < 0) cv_work.wait(lk, [&] { return stopping || !pending.empty() || any_active(); });
+        if (stopping && pending.empty() && !any_active() && inflight < 0) break;
         int tiles[2] = {grp[0].ntiles, grp[1].ntiles};
         int solves[2] = {(int)grp[0].active.size(), (int)grp[1].active.size()};
         while (!pending.empty()) {
@@ -238,84 +397,17 @@ void ns3d_engine::run() {
           pending.pop_front();
         }
       }
-      if (!admitted.empty()) admit(admitted);
-      // tables, iteration bases (each pinned staging buffer is written once
-      // per round, after the previous round's sync), restarts
-      for (int g = 0; g < 2; ++g) {
-        if (grp[g].dirty) rebuild(g);
-        upload_it0(g);
-        if (restart[g] && !grp[g].active.empty()) {
-          if (g == 0) ECHK(launch_bicg_restart<double>(batch<double>(0, 0), stream));
-          else ECHK(launch_bicg_restart<float>(batch<float>(1, 0), stream));
-        }
-        restart[g] = false;
+      bool queued = false;
+      if (!admitted.empty()) admit(admitted, qbuf);
+      if (any_active()) {
+        queue_round(qbuf, restart);
+        queued = true;
       }
-      // state pointers in solve order (group 0 then group 1)
-      order.clear();
-      for (int g = 0; g < 2; ++g)
-        for (EngineRequest* r : grp[g].active) order.push_back(r);
-      const int nact = (int)order.size();
-      max_active = std::max(max_active, (double)nact);
-      for (int i = 0; i < nact; ++i) h_stptr[i] = order[i]->prec == 0 ? order[i]->d64.w.st : order[i]->d32.w.st;
-      ECHK(hipMemcpyAsync(d_stptr, h_stptr.data(), nact * sizeof(BState*), hipMemcpyHostToDevice, stream));
-      // one chunk of iterations; the middle one carries dispatch events
-      const int qt = chunk / 2;
-      for (int q = 0; q < chunk; ++q) {
-        const bool ev = (q == qt);
-        if (!grp[0].active.empty()) ECHK(launch_bicg_A<double>(batch<double>(0, q), stream, ev ? tev : nullptr));
-        if (!grp[1].active.empty()) ECHK(launch_bicg_A<float>(batch<float>(1, q), stream, ev ? tev + 6 : nullptr));
-        if (!grp[0].active.empty()) ECHK(launch_bicg_B<double>(batch<double>(0, q), stream, ev ? tev + 2 : nullptr));
-        if (!grp[1].active.empty()) ECHK(launch_bicg_B<float>(batch<float>(1, q), stream, ev ? tev + 8 : nullptr));
-        if (!grp[0].active.empty()) ECHK(launch_bicg_C<double>(batch<double>(0, q), stream, ev ? tev + 4 : nullptr));
-        if (!grp[1].active.empty()) ECHK(launch_bicg_C<float>(batch<float>(1, q), stream, ev ? tev + 10 : nullptr));
-      }
-      n_launch += (double)chunk * 3 * ((grp[0].active.empty() ? 0 : 1) + (grp[1].active.empty() ? 0 : 1));
-      n_chunks += 1;
-      const int words = nact * (int)(sizeof(BState) / 8);
-      hipLaunchKernelGGL(gather_states_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, d_stptr, d_states,
-                         nact);
-      ECHK(hipGetLastError());
-      ECHK(hipMemcpyAsync(h_states, d_states, nact * sizeof(BState), hipMemcpyDeviceToHost, stream));
-      ECHK(hipStreamSynchronize(stream));
-      // sampled kernel times; bytes counted for every active solve (a solve
-      // that stopped inside the chunk makes the sample slightly optimistic)
-      for (int g = 0; g < 2; ++g) {
-        if (grp[g].active.empty()) continue;
-        double b[3] = {0, 0, 0};
-        for (EngineRequest* r : grp[g].active)
-          for (int k = 0; k < 3; ++k) b[k] += kernel_bytes_eng(r->n, r->nnz, k, g == 0 ? 8 : 4);
-        for (int k = 0; k < 3; ++k) {
-          float e = 0.f;
-          ECHK(hipEventElapsedTime(&e, tev[6 * g + 2 * k], tev[6 * g + 2 * k + 1]));
-          ms[g][k] += e;
-          launches[g][k] += 1;
-          bytes[g][k] += b[k];
-        }
-      }
-      // retire
-      bool any_done = false;
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        for (int i = 0; i < nact; ++i) {
-          EngineRequest* r = order[i];
-          const BState& st = h_states[i];
-          r->it_c = st.it_c;
-          if (st.status == BS_RUNNING) continue;
-          if (st.status == BS_RESTART) {
-            restart[r->prec] = true;
-            continue;
-          }
-          r->result = st;
-          r->done = true;
-          rows_x_iters += (double)r->n * (double)st.it_c;
-          n_done += 1;
-          any_done = true;
-          auto& act = grp[r->prec].active;
-          act.erase(std::find(act.begin(), act.end(), r));
-          grp[r->prec].dirty = true;
-        }
-      }
-      if (any_done) cv_done.notify_all();
+      bool woke = false;
+      if (inflight >= 0) woke = process_round(inflight, restart);
+      if (woke || inflight >= 0) cv_done.notify_all();
+      inflight = queued ? qbuf : -1;
+      if (queued) qbuf ^= 1;
     }
   } catch (const std::exception& e) {
     fail(e.what());

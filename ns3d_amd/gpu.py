@@ -80,7 +80,7 @@ def lib():
         L.ns3d_engine_destroy.argtypes = [vp]
         L.ns3d_attach_engine.argtypes = [vp, vp]
         L.ns3d_engine_stats.argtypes = [vp, P(dbl)]
-        L.ns3d_footprint.argtypes = [i32, i32, i32, i64, i32, P(dbl)]
+        L.ns3d_footprint.argtypes = [i32, i32, i32, i64, i32, i32, P(dbl)]
         L.ns3d_ctx_bytes.argtypes = [vp, P(dbl)]
         L.ns3d_prepare_image.argtypes = [P(C.c_uint8), i32, i32, i32, i32, i32, P(C.c_uint8), P(i32)]
         L.ns3d_create.argtypes = [P(C.c_uint8), i32, i32, i32, i32, P(vp)]
@@ -347,10 +347,12 @@ class Engine:
             pass
 
 
-def footprint(shape, nfluid, mixed=False):
-    """Projected device bytes of a context (upper bound)."""
+def footprint(shape, nfluid, mixed=False, engine=False):
+    """Projected device bytes of a context (upper bound); engine: attached to
+    a solve engine (two more solver workspaces)."""
     out = C.c_double()
-    _check(lib().ns3d_footprint(*[int(x) for x in shape], int(nfluid), int(bool(mixed)), C.byref(out)))
+    _check(lib().ns3d_footprint(*[int(x) for x in shape], int(nfluid), int(bool(mixed)), 3 if engine else 1,
+                                C.byref(out)))
     return out.value
 
 

@@ -60,8 +60,9 @@ void diagnostics(NavierStokesBase& b, Diag& d) {
   b.info_file << "Permeability K (m²): " << d.K << std::endl;
   b.say("Maximum value of the velocity field divergence : %g\n", d.maxdiv);
   b.say("Relative Mass Residual, Lref*(max DIV)/<u_out>, ||Div u|| %g, %g, %g\n", b.RMR, d.reldiv, d.normdiv);
-  b.note("  outer it: K %.10e RMR %.3e its u/v/w/p %lld/%lld/%lld/%lld\n", d.K, b.RMR, (long long)b.usolver.it,
-         (long long)b.vsolver.it, (long long)b.wsolver.it, (long long)b.psolver.it);
+  b.note("  outer it: K %.10e RMR %.3e its u/v/w/p %lld/%lld/%lld/%lld alpha %g beta %g\n", d.K, b.RMR,
+         (long long)b.usolver.it, (long long)b.vsolver.it, (long long)b.wsolver.it, (long long)b.psolver.it, b.alpha,
+         b.beta);
 }
 
 void track_old_rmr(NavierStokesBase& b, int it) {

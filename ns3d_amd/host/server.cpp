@@ -9,10 +9,12 @@
 // running requests hand their BiCGSTAB solves to the same engine, solves of
 // requests that overlap in time run in the same launches, and a request that
 // arrives while others are on the device joins them at the next chunk
-// boundary (continuous batching).  A mixed-precision request that does not
-// converge is re-run in plain FP64 when fallback is on.
+// boundary (continuous batching).  A mixed-precision request whose run fails
+// is run again in plain FP64 (fallback >= 1); with fallback 2 one that ends
+// unconverged at its outer-iteration cap is also continued in FP64.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -99,10 +101,24 @@ void ns3d_server::run_job(Job& j) {
   p.device = device;
   p.outdir = j.outdir.empty() ? nullptr : j.outdir.c_str();
   p.name = j.name.empty() ? nullptr : j.name.c_str();
-  const bool may_fall_back = p.precision != NS3D_PRECISION_FP64 && fallback;
+  const bool mixed = p.precision != NS3D_PRECISION_FP64;
+  const bool continue_unconverged = mixed && fallback >= NS3D_FALLBACK_UNCONVERGED;
   std::vector<double> fields[4];
-  attempt(p, j.res, nullptr, may_fall_back ? fields : nullptr);
-  if (may_fall_back && !j.res.converged) {
+  std::string failure;
+  try {
+    attempt(p, j.res, nullptr, continue_unconverged ? fields : nullptr);
+  } catch (const std::string& m) {
+    if (!mixed || fallback < NS3D_FALLBACK_FAILED) throw;
+    failure = m.empty() ? "mixed-precision run failed" : m;
+  }
+  const bool broken = !failure.empty() || !std::isfinite(j.res.K) || !std::isfinite(j.res.RMR);
+  if (mixed && fallback >= NS3D_FALLBACK_FAILED && broken) {
+    // the mixed run failed or produced non-finite values: run it again in
+    // FP64 from the start
+    p.precision = NS3D_PRECISION_FP64;
+    attempt(p, j.res, nullptr, nullptr);
+    j.res.fallback = 1;
+  } else if (continue_unconverged && !j.res.converged) {
     // continue in FP64 from the mixed run's fields, with a quarter of the
     // outer-iteration cap: where FP64 can converge it does so quickly from
     // there, and a request that stalls in any precision is not paid twice
@@ -173,7 +189,7 @@ int ns3d_server_create(int device, double budget_bytes, int max_slots, int chunk
   auto s = std::make_unique<ns3d_server>();
   s->device = device < 0 ? 0 : device;
   s->budget = budget_bytes;
-  s->fallback = fallback ? 1 : 0;
+  s->fallback = fallback < 0 ? 0 : (fallback > NS3D_FALLBACK_UNCONVERGED ? NS3D_FALLBACK_UNCONVERGED : fallback);
   s->t0 = std::chrono::steady_clock::now();
   if (ns3d_engine_create(s->device, chunk, &s->eng) != NS3D_OK) {
     g_srv_err = ns3d_last_error();
@@ -209,7 +225,7 @@ int ns3d_server_submit(ns3d_server* s, const ns3d_run_params* p, const uint8_t* 
   if (p->direction == 3) { d[0] = nz; d[2] = nx; }
   d[0] += 2 * p->border;
   nfluid += (int64_t)2 * p->border * d[1] * d[2];
-  if (ns3d_footprint(d[0], d[1], d[2], nfluid, p->precision != 0, &j->footprint) != NS3D_OK) {
+  if (ns3d_footprint(d[0], d[1], d[2], nfluid, p->precision != 0, 3, &j->footprint) != NS3D_OK) {
     g_srv_err = ns3d_last_error();
     return 2;
   }

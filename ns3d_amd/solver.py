@@ -222,19 +222,24 @@ def _summary(res):
     return summary
 
 
+FALLBACK_NONE, FALLBACK_FAILED, FALLBACK_UNCONVERGED = 0, 1, 2
+
+
 class Server:
     """Permeability requests served as they arrive (native server of
     libns3d_host.so): worker slots admit requests by projected device
-    footprint within budget_bytes, all running requests share one solve
-    engine, and a mixed-precision request that does not converge is re-run in
-    FP64 when fallback is on."""
+    footprint within budget_bytes, and all running requests share one solve
+    engine.  fallback: FALLBACK_FAILED (or True) runs a mixed-precision
+    request again in FP64 when its run fails or gives non-finite values;
+    FALLBACK_UNCONVERGED also continues one that ends unconverged at its
+    outer-iteration cap in FP64 from its fields."""
 
     STAT_NAMES = ["submitted", "queued", "running", "done", "failed", "fallbacks", "bytes_in_use",
                   "peak_bytes_in_use", "peak_running", "budget", "seconds"]
 
-    def __init__(self, device=0, budget_bytes=200e9, max_slots=16, chunk=8, fallback=True):
+    def __init__(self, device=0, budget_bytes=200e9, max_slots=16, chunk=8, fallback=FALLBACK_FAILED):
         h = C.c_void_p()
-        if lib().ns3d_server_create(device, float(budget_bytes), int(max_slots), int(chunk), int(bool(fallback)),
+        if lib().ns3d_server_create(device, float(budget_bytes), int(max_slots), int(chunk), int(fallback),
                                     C.byref(h)) != 0:
             raise _gpu.NS3DError(lib().ns3d_server_last_error().decode())
         self.h = h

@@ -85,20 +85,30 @@ def test_admission_respects_budget():
 
 
 def test_footprint_bounds_context_allocation():
-    for shape, seed, mixed in (((20, 18, 16), 1, False), ((24, 22, 20), 2, True), ((16, 30, 12), 3, True)):
-        img = generators.rsa_spheres(shape, 3.0, 0.3, seed)[0]
-        g = G.NavierStokesGPU(img)
-        if mixed:
-            g.set_precision(True, 1e-4)
-        used = g.device_bytes()
-        bound = G.footprint(img.shape, int((img == 0).sum()), mixed)
-        assert used <= bound <= 1.5 * used + 8e6, (shape, used, bound)
+    eng = G.Engine(0, 8)
+    try:
+        for shape, seed, mixed, attached in (((20, 18, 16), 1, False, False), ((24, 22, 20), 2, True, False),
+                                             ((16, 30, 12), 3, True, True), ((22, 20, 18), 4, False, True)):
+            img = generators.rsa_spheres(shape, 3.0, 0.3, seed)[0]
+            g = G.NavierStokesGPU(img)
+            if attached:
+                eng.attach(g.h)
+            if mixed:
+                g.set_precision(True, 1e-4)
+            used = g.device_bytes()
+            bound = G.footprint(img.shape, int((img == 0).sum()), mixed, engine=attached)
+            if attached:
+                eng.detach(g.h)
+            g.close()
+            assert used <= bound <= 1.5 * used + 8e6, (shape, mixed, attached, used, bound)
+    finally:
+        eng.close()
 
 
 def test_mixed_request_falls_back_to_fp64():
     img = generators.rsa_spheres((18, 16, 14), 3.0, 0.3, 9)[0]
     p = dict(PRM, method="SIMPLEC", gradp=1.0 / 18, precision=1, maxitSIMPLE=3)
-    srv = solver.Server(device=0, budget_bytes=16e9, max_slots=2, chunk=4, fallback=True)
+    srv = solver.Server(device=0, budget_bytes=16e9, max_slots=2, chunk=4, fallback=solver.FALLBACK_UNCONVERGED)
     try:
         r = srv.wait(srv.submit(img, **p))
         st = srv.stats()
@@ -108,6 +118,16 @@ def test_mixed_request_falls_back_to_fp64():
     # outer-iteration cap (at least 10): 3 + 10 outer iterations in all
     assert r["fallback"] == 1 and st["fallbacks"] == 1
     assert r["iterations"] == 3 + 10
+    # the default mode only replaces failed runs: an unconverged mixed
+    # result stands
+    srv = solver.Server(device=0, budget_bytes=16e9, max_slots=2, chunk=4)
+    try:
+        r = srv.wait(srv.submit(img, **p))
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert r["fallback"] == 0 and st["fallbacks"] == 0
+    assert r["iterations"] == 3 and r["converged"] == 0
 
 
 def test_warm_start_from_converged_fields():
