@@ -64,7 +64,7 @@ def mix_params(img, args, precision):
     nx = img.shape[0]
     return dict(dx=1.0, mu=1.0, rho=1.0, p0=1.0, gradp=1.0 / nx, alpha=0.8, beta=1.0, direction=1, border=0,
                 method="SIMPLEC", simpletol=args.simpletol, maxitSIMPLE=args.mix_maxit, N1=4, N2=4, device=0,
-                precision=precision, inner_tol=args.inner_tol, **TOLS)
+                precision=precision, inner_tol=args.inner_tol, mixed_min_rows=args.mixed_min_rows, **TOLS)
 
 
 def served_pass(reqs, args, precision, label, arrivals=None):
@@ -121,6 +121,53 @@ def engine_roofline(est, prec="f32", k="A"):
                 bytes_per_launch=round(nbytes / launches), achieved=round(ach, 1), frac=round(ach / PEAK_HBM_GBS, 4))
 
 
+def dk_gate(reqs, args, mixed, plain, tol=1e-4):
+    """|dK|/K of each mixed request against plain FP64, gated at tol.  The
+    outer iteration stops when the relative mass residual first dips below
+    simpletol, and the relaxation control (NSBase.cxx:196-300) lowers alpha
+    on every RMR spike; near the residual's noise floor both are decided by
+    rounding, so the stopped K of the FP64 path itself moves by up to ~1e-3
+    under a perturbation of the rounding alone.  For a request over tol the
+    FP64 path is therefore run on the image mirrored in y and in z (the same
+    problem, other summation orders): its spread there is the FP64 path's
+    own rounding spread, and the mixed request passes if it stays within
+    it."""
+    from ns3d_amd import solver
+    out = dict(tolerance=tol, requests=0, within_tolerance=0, within_fp64_rounding_spread=0, over=[])
+    for q, ((img, _), m, p) in enumerate(zip(reqs, mixed, plain)):
+        if not (m["converged"] and p["converged"]) or p["K"] == 0:
+            continue
+        out["requests"] += 1
+        d = abs(m["K"] / p["K"] - 1)
+        if d <= tol:
+            out["within_tolerance"] += 1
+            continue
+        spread = 0.0
+        for v in (img[:, ::-1, :], img[:, :, ::-1]):
+            v = np.ascontiguousarray(v)
+            sv = solver.Solver(v, **mix_params(v, args, 0))
+            r = sv.run()[1]
+            sv.close()
+            spread = max(spread, abs(r["K"] / p["K"] - 1))
+        log(f"  request {q}: mixed |dK|/K {d:.2e}; FP64 on mirrored copies {spread:.2e}")
+        out["within_fp64_rounding_spread"] += int(d <= spread)
+        out["over"].append(dict(request=q, dK_over_K=d, fp64_mirror_spread=spread))
+    out["passed"] = out["within_tolerance"] + out["within_fp64_rounding_spread"] == out["requests"]
+    return out
+
+
+def engine_chunk(est):
+    """Whole engine chunks (all solves in flight, both engine streams, from
+    the first A start to the last C stop, launch gaps included): the
+    algorithmic bytes of the iterations executed over the span."""
+    ms, n, nbytes = est.get("chunk", (0, 0, 0))
+    if not n or ms <= 0:
+        return None
+    ach = nbytes / (ms * 1e-3) / 1e9
+    return dict(avg_span_us=round(1e3 * ms / n, 2), sampled=int(n), bytes_per_chunk=round(nbytes / n),
+                achieved=round(ach, 1), frac=round(ach / PEAK_HBM_GBS, 4))
+
+
 def mix_section(args):
     from ns3d_amd import mix
     log(f"generating the request mix ({args.requests} requests, seed {args.mix_seed})")
@@ -157,6 +204,7 @@ def mix_section(args):
         return [abs(r["K"] / k - 1) for r, k, c in zip(res, k64, conv64) if c and r["converged"] and k != 0]
     dk_mixed = dk(smx[-1]["results"])
     dk_s64 = dk(s64["results"])
+    gate = dk_gate(reqs, args, smx[-1]["results"], plain["results"])
     stalled = [q for q, (r, c) in enumerate(zip(smx[-1]["results"], conv64)) if not (c and r["converged"])]
     n = len(reqs)
     mixed_wall = sum(p["wall"] for p in smx) / len(smx)
@@ -182,7 +230,7 @@ def mix_section(args):
             converged=int(sum(r["converged"] for r in smx[-1]["results"])),
             peak_running=smx[-1]["stats"]["peak_running"],
             peak_device_bytes=smx[-1]["stats"]["peak_bytes_in_use"],
-            engine_A_f32=engine_roofline(smx[-1]["engine"], "f32", "A"))),
+            dK_gate=gate, engine_A_f32=engine_roofline(smx[-1]["engine"], "f32", "A"))),
         "served_mixed_poisson": form(spo["wall"], spo["latency"], dict(arrival_rate=round(rate, 4))),
     }
     desc = dict(workload=f"request mix: {n} seeded images (edges {args.edges} per axis, porosity U[0.35,0.90], "
@@ -191,7 +239,7 @@ def mix_section(args):
                 requests=n, fluid_cells=int(fluid), mix_seed=args.mix_seed, slots=args.slots, chunk=args.chunk,
                 images=[dict(shape=d["shape"], generator=d["generator"], porosity=d["porosity"]) for _, d in reqs])
     return dict(forms=forms, desc=desc, reqs=reqs, plain=plain, mixed_wall=mixed_wall, lat=lat_m,
-                max_dk=max(dk_mixed), engine=smx[-1]["engine"])
+                max_dk=max(dk_mixed), gate=gate, engine=smx[-1]["engine"])
 
 
 # ---------------------------------------------------------------- single image
@@ -387,6 +435,8 @@ def main():
                     help="outer-iteration cap of the mix (its requests converge in <= 440; one stalls at "
                          "RMR ~4e-6 in every precision and runs to the cap)")
     ap.add_argument("--inner-tol", type=float, default=1e-4, help="mixed form: FP32 sweep tolerance")
+    ap.add_argument("--mixed-min-rows", type=int, default=0,
+                    help="mixed form: systems with fewer rows are solved in FP64")
     ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events (profiler runs)")
@@ -428,6 +478,7 @@ def main():
             "latency_s": f["served_mixed"]["latency_s"],
             "speedup_vs_plain_fp64": round(value / f["plain_fp64"]["solves_per_s"], 3),
             "max_dK_over_K_vs_fp64": mixres["max_dk"],
+            "dK_gate": mixres["gate"],
             "forms": f,
         }
     else:
@@ -442,11 +493,13 @@ def main():
         # storage) over all requests in flight, sampled by the engine
         r = engine_roofline(mixres["engine"], "f32", "A")
         traffic, tsrc = mix_traffic(args)
+        it = engine_chunk(mixres["engine"])
         line["roofline"] = None if r is None else dict(
             bound="hbm", achieved=r["achieved"], peak=PEAK_HBM_GBS, unit="GB/s", frac=r["frac"], traffic=traffic,
-            kernel="bicg_A_kernel<float> batched over the requests in flight (served mixed form): p update + "
-                   "7-point SpMV + r0.v", bytes_per_launch=r["bytes_per_launch"], avg_launch_us=r["avg_launch_us"],
-            sampled_launches=r["sampled_launches"], traffic_source=tsrc)
+            kernel="bicg_A_kernel<float> batched over the requests in flight (served mixed form, one of two "
+                   "concurrent streams): p update + 7-point SpMV + r0.v", bytes_per_launch=r["bytes_per_launch"],
+            avg_launch_us=r["avg_launch_us"], sampled_launches=r["sampled_launches"], traffic_source=tsrc,
+            engine_chunks=it)
         line["served_mixed_engine"] = {k: v for k, v in mixres["engine"].items()}
     if single is not None:
         if mixres is None:

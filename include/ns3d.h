@@ -211,6 +211,13 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
  * the reference. */
 ns3d_status ns3d_set_precision(ns3d_ctx* ctx, int mode, double inner_tol);
 
+/* Mixed form only: systems with fewer than `rows` rows are solved with the
+ * plain FP64 BiCGSTAB.  FP32 sweeps save HBM bandwidth, which small systems
+ * hardly spend (their iterations are bound by launch latency), while every
+ * defect-correction step costs a synchronisation.  0 (default): every system
+ * in the mixed form.  Not part of the reference. */
+ns3d_status ns3d_set_mixed_min_rows(ns3d_ctx* ctx, int64_t rows);
+
 /* Device time of the fused BiCGSTAB kernels (enable with ns3d_timing(ctx,
  * 1)): one iteration per queued chunk of iterations is sampled, its three
  * kernels launched with start/stop events taken from their own dispatches
@@ -245,7 +252,11 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
 /* out[32]: chunks, launches, solves done, max active solves, sum of rows x
  * iterations; then per precision q (0 FP64, 1 FP32) and kernel k (A, B, C),
  * out[5 + 9q + 3k] = sampled milliseconds, sampled launches, algorithmic
- * bytes of those launches. */
+ * bytes of the solves that iterated in them (one iteration per chunk is
+ * sampled, each launch timed by its own dispatch); out[23..25] = whole
+ * chunks (the engine runs its solves on two streams; from the first A start
+ * to the last C stop over all of them): milliseconds, count, algorithmic
+ * bytes of the iterations they executed. */
 ns3d_status ns3d_engine_stats(ns3d_engine* eng, double out[32]);
 
 /* Upper bound of the device bytes a context of an nx x ny x nz image with

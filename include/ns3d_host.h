@@ -36,7 +36,8 @@ typedef struct {
   const char* outdir;           /* directory for the output files */
   const char* name;             /* image file name (output names derive from it) */
   int32_t precision;            /* NS3D_PRECISION_FP64 (default) or _MIXED (ns3d.h) */
-  int32_t reserved;
+  int32_t mixed_min_rows;       /* mixed form: systems with fewer rows are solved in
+                                   FP64 (ns3d_set_mixed_min_rows; 0: none) */
   double inner_tol;             /* mixed form: relative tolerance of the FP32 sweeps */
 } ns3d_run_params;
 

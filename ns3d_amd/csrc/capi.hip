@@ -105,6 +105,7 @@ struct ns3d_ctx {
   int64_t lane_rows = 0;
   int precision = NS3D_PRECISION_FP64;
   double inner_tol = 1e-4;
+  int64_t mixed_min_rows = 0;  // mixed form: smaller systems are solved in FP64
   ns3d_params prm{};
   bool have_params = false;
   std::vector<void*> allocs;
@@ -485,8 +486,10 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
 
 bool mixed_ready(const ns3d_ctx* c, const int* sid, int n) {
   if (c->precision != NS3D_PRECISION_MIXED) return false;
-  for (int i = 0; i < n; ++i)
-    if (c->sys[sid[i]].n > 0 && !c->sys[sid[i]].coef32) return false;
+  for (int i = 0; i < n; ++i) {
+    const SysMem& S = c->sys[sid[i]];
+    if (S.n > 0 && (!S.coef32 || S.n < c->mixed_min_rows)) return false;
+  }
   return true;
 }
 
@@ -1066,6 +1069,12 @@ ns3d_status ns3d_set_precision(ns3d_ctx* c, int mode, double inner_tol) {
     for (int q = 0; q < 4; ++q) refresh_f32(c, q);
     HCHK(hipStreamSynchronize(c->stream));
   });
+}
+
+ns3d_status ns3d_set_mixed_min_rows(ns3d_ctx* c, int64_t rows) {
+  if (!c || rows < 0) return fail_invalid("ns3d_set_mixed_min_rows: bad arguments");
+  c->mixed_min_rows = rows;
+  return NS3D_OK;
 }
 
 ns3d_status ns3d_matrix_nnz(ns3d_ctx* c, int64_t out[4]) {

@@ -190,6 +190,35 @@ __device__ __forceinline__ void block_sums_at(double (&v)[NV], P base, int64_t q
 // value (slot q at partials[q * qstride + b], qstride = nb by default).  The order of the additions is
 // fixed and the same in every block, so every block computes the same
 // bits; this replaces a separate finalize launch between two row kernels.
+// In two steps: partials_load (each thread's share, so the loads can be in
+// flight with others) and partials_reduce (the block reduction).
+template <int NV, typename P>
+__device__ __forceinline__ void partials_load(P partials, int nb, int64_t qstride, double (&a)[NV]) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q) a[q] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) a[q] += partials[q * qstride + b];
+  }
+}
+template <int NV>
+__device__ __forceinline__ void partials_reduce(double (&a)[NV], double (&out)[NV]) {
+  __shared__ double sh[NV][kWaves];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) a[q] = wave_reduce<RSUM>(a[q]);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sh[q][threadIdx.x >> 6] = a[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double r = sh[q][0];
+#pragma unroll
+    for (int wv = 1; wv < kWaves; ++wv) r += sh[q][wv];
+    out[q] = r;
+  }
+}
 template <int NV, typename P>
 __device__ __forceinline__ void sum_partials_all(P partials, int nb, double (&out)[NV], int64_t qstride = -1) {
   if (qstride < 0) qstride = nb;

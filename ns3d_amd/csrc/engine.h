@@ -38,7 +38,9 @@ struct EngineRequest {
   // next round is queued; iteration base of the last queued round and
   // whether that round restarted this solve's group first
   long long serial = 0;
+  int group = 0;  // 2 * prec + lane
   long long it_next = 0;
+  long long it_done = 0;  // executed iterations as last read back
   long long it_queued = -1;
   bool restart_queued = false;
 };
@@ -67,38 +69,52 @@ struct ns3d_engine {
   struct Group {
     std::vector<ns3d::EngineRequest*> active;
     void* d_desc = nullptr;
-    ns3d::Tile* d_tiles = nullptr;
+    void* d_tiles = nullptr;   // TileDesc<T> entries
     void* d_idesc = nullptr;
-    ns3d::Tile* d_itiles = nullptr;
-    // host images of the tables, rewritten only between rounds (after the
-    // stream has drained), so no copy in flight ever reads a changed buffer
-    // (two of each: a round is queued while the previous one is in flight)
+    void* d_itiles = nullptr;
+    // host images of the tables, rewritten only for a round queued after
+    // the one that last used them has completed, so no copy in flight ever
+    // reads a changed buffer (two of each: a round is queued while the
+    // previous one is in flight).  The tile table carries each solve's
+    // iteration base and is uploaded every round.
     std::vector<char> h_desc[2], h_idesc[2];
-    std::vector<ns3d::Tile> h_tiles[2], h_itiles[2];
-    long long* d_it0 = nullptr;  // per active solve: executed iterations when the chunk is queued
-    std::vector<long long> h_it0[2];
+    std::vector<char> h_tiles[2], h_itiles[2];
+    std::vector<ns3d::Tile> tiles;  // current tiles (rebuilt when the active set changes)
+    std::vector<char> descs;        // current descriptors, one per active solve
     int ntiles = 0;
     bool dirty = true;
   };
+  // Solves are split into kGroups groups: group g holds precision g / 2
+  // (0 FP64, 1 FP32) on lane g % 2.  Each lane has its own stream, so the
+  // launches of the two lanes of a precision run concurrently and one's
+  // ramp-up and drain overlap the other's work.
+  static constexpr int kLanes = 2;
+  static constexpr int kGroups = 2 * kLanes;
   void run();
   void admit(std::vector<ns3d::EngineRequest*>& reqs, int buf);
   void rebuild(int g, int buf);
-  void upload_it0(int g, int buf, bool restarted);
-  void queue_round(int buf, bool restart[2]);
-  bool process_round(int buf, bool restart[2]);
+  void upload_tiles(int g, int buf, bool restarted);
+  void queue_round(int buf, bool restart[kGroups]);
+  bool process_round(int buf, bool restart[kGroups]);
   template <typename T>
   ns3d::Batch<T> batch(int g, int q) const;
   void fail(const std::string& msg);
-  bool any_active() const { return !grp[0].active.empty() || !grp[1].active.empty(); }
+  bool any_active() const {
+    for (const Group& G : grp)
+      if (!G.active.empty()) return true;
+    return false;
+  }
+  int64_t group_rows(int g) const;
 
-  hipStream_t stream = nullptr;
+  hipStream_t lane_stream[kLanes] = {};
+  hipStream_t stream = nullptr;  // lane 0's: gathers and completion events
   std::thread thr;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<ns3d::EngineRequest*> pending;
   bool stopping = false;
   std::string fatal;
-  Group grp[2];
+  Group grp[kGroups];
   // per round buffer: state pointers of the round's solves, gathered states,
   // the solves in gather order, the round's completion and sampled events
   ns3d::BState** d_stptr[2] = {};
@@ -107,16 +123,31 @@ struct ns3d_engine {
   ns3d::BState* h_states[2] = {};
   // (serials of the round's solves: a solve retired by the previous round's
   // results may still appear in this one, and its request is gone by then)
-  std::vector<long long> order[2];
+  struct RoundEntry {
+    long long serial;
+    long long it0;  // the solve's iteration base in the round
+    int group;
+    int64_t n, nnz;
+  };
+  std::vector<RoundEntry> order[2];
   std::unordered_map<long long, ns3d::EngineRequest*> live;
   long long next_serial = 1;
-  std::vector<double> round_bytes[2];  // algorithmic bytes of A, B, C per group
+  bool round_on[2][kGroups] = {};      // groups with launches in the round
   hipEvent_t round_done[2] = {};
-  hipEvent_t tev[2][12] = {};
+  hipEvent_t lane_join[2] = {};        // lane 1's chunk done (lane 0 waits before the gather)
+  // dispatch events per group: [A, B, C of the sampled iteration][start,
+  // stop], then the chunk's first A and last C [start, stop]
+  static constexpr int kEv = 10;
+  hipEvent_t tev[2][kGroups * kEv] = {};
   long long rounds_queued = 0, rounds_done = 0;
   std::vector<void*> allocs;
   // statistics
   double n_chunks = 0, n_launch = 0, n_done = 0, max_active = 0;
+  // per precision and kernel: sampled launch milliseconds, launches, and
+  // the algorithmic bytes of the solves that iterated in them; whole chunks
+  // (all groups, from the first A start to the last C stop): milliseconds,
+  // count, algorithmic bytes of the iterations executed in them
   double ms[2][3] = {}, launches[2][3] = {}, bytes[2][3] = {};
+  double ch_ms = 0, ch_n = 0, ch_bytes = 0;
   double rows_x_iters = 0;
 };

@@ -1,20 +1,22 @@
 // engine.hip -- the batched solve engine (see engine.h).
 //
 // The dispatcher loop, per round:
-//   1. admit the pending requests that fit (tiles and solve slots): wait for
-//      each request's ready event, launch init over a table holding only the
-//      new solves, and add them to the active set of their precision;
-//   2. rebuild the descriptor and tile tables of a precision group whose
-//      membership changed (a few KB copied from pinned memory);
-//   3. queue `chunk` iterations, each A, B, C for the FP64 group and the FP32
-//      group (one launch per kernel per group, over all active solves);
-//   4. gather every active solve's BState into one array (one kernel, one
-//      copy), synchronise, retire the solves that stopped and wake their
-//      callers, and queue restart kernels for those that asked for one.
-// Kernels of solves that stopped exit at once, so a chunk never computes
-// past a solve's end.  The host images of the tables are only rewritten
-// between rounds, after the stream has drained, so no copy in flight reads a
-// buffer that changed.
+//   1. admit the pending requests that fit (tiles and solve slots) to the
+//      lane of their precision with fewer rows: wait for each request's
+//      ready event, launch init over a table holding only the new solves,
+//      and add them to the active set of their group;
+//   2. rebuild the descriptor table and tiles of a group whose membership
+//      changed; upload every active group's tile table (it carries the
+//      iteration bases);
+//   3. queue `chunk` iterations, each A, B, C for every active group (one
+//      launch per kernel per group, over all its solves, on its lane's
+//      stream);
+//   4. join the lanes, gather every active solve's BState into one array
+//      (one kernel, one copy) and record the round's completion.
+// Rounds are pipelined: round k+1 is queued before round k's states are
+// read back and processed (retire stopped solves and wake their callers,
+// flag restarts).  Kernels of solves that stopped exit at once, so a chunk
+// never computes past a solve's end.
 #include "engine.h"
 
 #include <hip/hip_ext.h>
@@ -61,7 +63,8 @@ SolveDesc<float>& desc_of<float>(EngineRequest* r) { return r->d32; }
 
 ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) {
   ECHK(hipSetDevice(device));
-  ECHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  for (auto& st : lane_stream) ECHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  stream = lane_stream[0];
   auto dalloc = [&](size_t bytes) {
     void* p = nullptr;
     ECHK(hipMalloc(&p, bytes));
@@ -69,24 +72,24 @@ ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) 
     return p;
   };
   const size_t dsz = std::max(sizeof(SolveDesc<double>), sizeof(SolveDesc<float>));
+  const size_t tsz = std::max(sizeof(TileDesc<double>), sizeof(TileDesc<float>));
   for (Group& g : grp) {
     g.d_desc = dalloc(dsz * kEngMaxSolves);
     g.d_idesc = dalloc(dsz * kEngMaxSolves);
-    g.d_tiles = (Tile*)dalloc(sizeof(Tile) * kEngMaxTiles);
-    g.d_itiles = (Tile*)dalloc(sizeof(Tile) * kEngMaxTiles);
-    g.d_it0 = (long long*)dalloc(sizeof(long long) * kEngMaxSolves);
+    g.d_tiles = dalloc(tsz * kEngMaxTiles);
+    g.d_itiles = dalloc(tsz * kEngMaxTiles);
     for (int b = 0; b < 2; ++b) {
-      g.h_it0[b].resize(kEngMaxSolves);
       g.h_desc[b].resize(dsz * kEngMaxSolves);
       g.h_idesc[b].resize(dsz * kEngMaxSolves);
     }
   }
   for (int b = 0; b < 2; ++b) {
-    d_stptr[b] = (BState**)dalloc(sizeof(BState*) * 2 * kEngMaxSolves);
-    d_states[b] = (BState*)dalloc(sizeof(BState) * 2 * kEngMaxSolves);
-    h_stptr[b].resize(2 * kEngMaxSolves);
-    ECHK(hipHostMalloc((void**)&h_states[b], sizeof(BState) * 2 * kEngMaxSolves, hipHostMallocDefault));
+    d_stptr[b] = (BState**)dalloc(sizeof(BState*) * kGroups * kEngMaxSolves);
+    d_states[b] = (BState*)dalloc(sizeof(BState) * kGroups * kEngMaxSolves);
+    h_stptr[b].resize(kGroups * kEngMaxSolves);
+    ECHK(hipHostMalloc((void**)&h_states[b], sizeof(BState) * kGroups * kEngMaxSolves, hipHostMallocDefault));
     ECHK(hipEventCreateWithFlags(&round_done[b], hipEventDisableTiming));
+    ECHK(hipEventCreateWithFlags(&lane_join[b], hipEventDisableTiming));
     for (auto& e : tev[b]) ECHK(hipEventCreate(&e));
   }
   thr = std::thread([this] { run(); });
@@ -99,15 +102,18 @@ ns3d_engine::~ns3d_engine() {
   }
   cv_work.notify_all();
   if (thr.joinable()) thr.join();
-  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& st : lane_stream)
+    if (st) (void)hipStreamSynchronize(st);
   for (int b = 0; b < 2; ++b) {
     if (h_states[b]) (void)hipHostFree(h_states[b]);
     if (round_done[b]) (void)hipEventDestroy(round_done[b]);
+    if (lane_join[b]) (void)hipEventDestroy(lane_join[b]);
     for (auto& e : tev[b])
       if (e) (void)hipEventDestroy(e);
   }
   for (void* p : allocs) (void)hipFree(p);
-  if (stream) (void)hipStreamDestroy(stream);
+  for (auto& st : lane_stream)
+    if (st) (void)hipStreamDestroy(st);
 }
 
 void ns3d_engine::solve(EngineRequest& req) {
@@ -149,6 +155,9 @@ void ns3d_engine::stats(double* out, int count) {
       v[6 + 9 * p + 3 * k] = launches[p][k];
       v[7 + 9 * p + 3 * k] = bytes[p][k];
     }
+  v[23] = ch_ms;
+  v[24] = ch_n;
+  v[25] = ch_bytes;
   for (int q = 0; q < count && q < 32; ++q) out[q] = v[q];
 }
 
@@ -158,13 +167,18 @@ void ns3d_engine::fail(const std::string& msg) {
   cv_done.notify_all();
 }
 
+int64_t ns3d_engine::group_rows(int g) const {
+  int64_t rows = 0;
+  for (const EngineRequest* r : grp[g].active) rows += r->n;
+  return rows;
+}
+
 template <typename T>
 Batch<T> ns3d_engine::batch(int g, int q) const {
   const Group& G = grp[g];
   Batch<T> B{};
   B.d = (const SolveDesc<T>*)G.d_desc;
-  B.tiles = G.d_tiles;
-  B.it0 = G.d_it0;
+  B.tiles = (const TileDesc<T>*)G.d_tiles;
   B.nsolves = (int)G.active.size();
   B.ntiles = G.ntiles;
   B.q = q;
@@ -174,40 +188,65 @@ Batch<T> ns3d_engine::batch(int g, int q) const {
 // Host images of the tables (index buf) are rewritten only for a round queued
 // after the one that last used them has been processed, so no copy still in
 // flight reads a buffer that changed.
+template <typename T>
+static void tile_table(std::vector<char>& img, const std::vector<Tile>& tiles, const SolveDesc<T>* desc,
+                       const std::vector<EngineRequest*>* reqs) {
+  img.resize(tiles.size() * sizeof(TileDesc<T>));
+  TileDesc<T>* e = (TileDesc<T>*)img.data();
+  for (size_t i = 0; i < tiles.size(); ++i) {
+    const int s = tiles[i].solve;
+    e[i].d = desc[s];
+    e[i].begin = tiles[i].begin;
+    e[i].end = tiles[i].end;
+    e[i].it0 = reqs ? (*reqs)[s]->it_queued : 0;
+    e[i].solve = s;
+    e[i].pad = 0;
+  }
+}
+
 void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
-  for (EngineRequest* r : reqs)
-    if (r->ready) ECHK(hipStreamWaitEvent(stream, r->ready, 0));
-  for (int g = 0; g < 2; ++g) {
+  // lane: the one of the request's precision with fewer rows so far
+  int64_t rows[kGroups];
+  for (int g = 0; g < kGroups; ++g) rows[g] = group_rows(g);
+  for (EngineRequest* r : reqs) {
+    const int g0 = 2 * r->prec;
+    r->group = rows[g0 + 1] < rows[g0] ? g0 + 1 : g0;
+    rows[r->group] += r->n;
+  }
+  for (int g = 0; g < kGroups; ++g) {
     Group& G = grp[g];
+    hipStream_t gs = lane_stream[g % kLanes];
     std::vector<Tile> tiles;
     int ns = 0;
+    for (EngineRequest* r : reqs)
+      if (r->group == g && r->ready) ECHK(hipStreamWaitEvent(gs, r->ready, 0));
     auto init = [&](auto tag) {
       using T = decltype(tag);
       SolveDesc<T>* hd = (SolveDesc<T>*)G.h_idesc[buf].data();
       for (EngineRequest* r : reqs) {
-        if (r->prec != g) continue;
+        if (r->group != g) continue;
         SolveDesc<T> d = desc_of<T>(r);
         make_tiles(ns, d.S.n, tiles, d.t0, d.t1);
         hd[ns++] = d;
       }
       if (ns == 0) return;
-      G.h_itiles[buf] = tiles;
-      ECHK(hipMemcpyAsync(G.d_idesc, hd, ns * sizeof(SolveDesc<T>), hipMemcpyHostToDevice, stream));
-      ECHK(hipMemcpyAsync(G.d_itiles, G.h_itiles[buf].data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
-                          stream));
+      tile_table(G.h_itiles[buf], tiles, hd, nullptr);
+      ECHK(hipMemcpyAsync(G.d_idesc, hd, ns * sizeof(SolveDesc<T>), hipMemcpyHostToDevice, gs));
+      ECHK(hipMemcpyAsync(G.d_itiles, G.h_itiles[buf].data(), G.h_itiles[buf].size(), hipMemcpyHostToDevice, gs));
       Batch<T> B{};
       B.d = (const SolveDesc<T>*)G.d_idesc;
-      B.tiles = G.d_itiles;
+      B.tiles = (const TileDesc<T>*)G.d_itiles;
       B.nsolves = ns;
       B.ntiles = (int)tiles.size();
-      ECHK(launch_bicg_init<T>(B, stream));
+      ECHK(launch_bicg_init<T>(B, gs));
     };
-    if (g == 0) init(double{});
+    if (g / kLanes == 0) init(double{});
     else init(float{});
     for (EngineRequest* r : reqs)
-      if (r->prec == g) {
+      if (r->group == g) {
         r->serial = next_serial++;
         r->it_next = 0;
+        r->it_done = 0;
         r->it_queued = -1;
         r->restart_queued = false;
         live[r->serial] = r;
@@ -217,87 +256,113 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
   }
 }
 
-// Iteration bases of the group's active solves, in table order, for the
-// round being queued (launch q of the chunk is iteration it0 + q); each
-// solve is then expected to be a chunk further on.
-void ns3d_engine::upload_it0(int g, int buf, bool restarted) {
+// The tile table of the round being queued, with each active solve's
+// iteration base (launch q of the chunk is iteration it0 + q); each solve is
+// then expected to be a chunk further on.
+void ns3d_engine::upload_tiles(int g, int buf, bool restarted) {
   Group& G = grp[g];
   if (G.active.empty()) return;
-  for (size_t i = 0; i < G.active.size(); ++i) {
-    EngineRequest* r = G.active[i];
-    G.h_it0[buf][i] = r->it_next;
+  for (EngineRequest* r : G.active) {
     r->it_queued = r->it_next;
     r->restart_queued = restarted;
     r->it_next += chunk;
   }
-  ECHK(hipMemcpyAsync(G.d_it0, G.h_it0[buf].data(), G.active.size() * sizeof(long long), hipMemcpyHostToDevice,
-                      stream));
+  auto up = [&](auto tag) {
+    using T = decltype(tag);
+    tile_table(G.h_tiles[buf], G.tiles, (const SolveDesc<T>*)G.descs.data(), &G.active);
+  };
+  if (g / kLanes == 0) up(double{});
+  else up(float{});
+  ECHK(hipMemcpyAsync(G.d_tiles, G.h_tiles[buf].data(), G.h_tiles[buf].size(), hipMemcpyHostToDevice,
+                      lane_stream[g % kLanes]));
 }
 
+// Tiles and descriptors of the group's active solves (after it changed).
 void ns3d_engine::rebuild(int g, int buf) {
   Group& G = grp[g];
-  std::vector<Tile> tiles;
+  G.tiles.clear();
   auto build = [&](auto tag) {
     using T = decltype(tag);
-    SolveDesc<T>* hd = (SolveDesc<T>*)G.h_desc[buf].data();
+    G.descs.resize(std::max<size_t>(G.active.size(), 1) * sizeof(SolveDesc<T>));
+    SolveDesc<T>* cur = (SolveDesc<T>*)G.descs.data();
     for (size_t i = 0; i < G.active.size(); ++i) {
       SolveDesc<T> d = desc_of<T>(G.active[i]);
-      make_tiles((int)i, d.S.n, tiles, d.t0, d.t1);
-      hd[i] = d;
+      make_tiles((int)i, d.S.n, G.tiles, d.t0, d.t1);
+      cur[i] = d;
     }
-    G.ntiles = (int)tiles.size();
+    G.ntiles = (int)G.tiles.size();
     if (G.active.empty()) return;
-    G.h_tiles[buf] = tiles;
-    ECHK(hipMemcpyAsync(G.d_desc, hd, G.active.size() * sizeof(SolveDesc<T>), hipMemcpyHostToDevice, stream));
-    ECHK(hipMemcpyAsync(G.d_tiles, G.h_tiles[buf].data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice,
-                        stream));
+    SolveDesc<T>* hd = (SolveDesc<T>*)G.h_desc[buf].data();
+    std::copy(cur, cur + G.active.size(), hd);
+    ECHK(hipMemcpyAsync(G.d_desc, hd, G.active.size() * sizeof(SolveDesc<T>), hipMemcpyHostToDevice,
+                        lane_stream[g % kLanes]));
   };
-  if (g == 0) build(double{});
+  if (g / kLanes == 0) build(double{});
   else build(float{});
   G.dirty = false;
 }
 
-// Tables, restarts, one chunk, and the gather of the states of every solve
-// in the round, into buffer buf.
-void ns3d_engine::queue_round(int buf, bool restart[2]) {
-  for (int g = 0; g < 2; ++g) {
+// Tables, restarts, one chunk on each lane, the join, and the gather of the
+// states of every solve in the round, into buffer buf.
+void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
+  // the other lanes start this round after the previous round's gather (on
+  // lane 0), so that gather never reads a state this round is writing
+  for (int l = 1; l < kLanes; ++l) ECHK(hipStreamWaitEvent(lane_stream[l], round_done[buf ^ 1], 0));
+  for (int g = 0; g < kGroups; ++g) {
     if (grp[g].dirty) rebuild(g, buf);
     // the restart kernel acts only on solves whose status asks for it
     const bool rs = restart[g] && !grp[g].active.empty();
+    upload_tiles(g, buf, rs);
     if (rs) {
-      if (g == 0) ECHK(launch_bicg_restart<double>(batch<double>(0, 0), stream));
-      else ECHK(launch_bicg_restart<float>(batch<float>(1, 0), stream));
+      hipStream_t gs = lane_stream[g % kLanes];
+      if (g / kLanes == 0) ECHK(launch_bicg_restart<double>(batch<double>(g, 0), gs));
+      else ECHK(launch_bicg_restart<float>(batch<float>(g, 0), gs));
     }
     restart[g] = false;
-    upload_it0(g, buf, rs);
   }
-  std::vector<long long>& ord = order[buf];
+  std::vector<RoundEntry>& ord = order[buf];
   ord.clear();
   int nact = 0;
-  round_bytes[buf].assign(6, 0.0);
-  for (int g = 0; g < 2; ++g)
+  for (int g = 0; g < kGroups; ++g)
     for (EngineRequest* r : grp[g].active) {
-      ord.push_back(r->serial);
+      ord.push_back(RoundEntry{r->serial, r->it_queued, g, r->n, r->nnz});
       h_stptr[buf][nact++] = r->prec == 0 ? r->d64.w.st : r->d32.w.st;
-      for (int k = 0; k < 3; ++k) round_bytes[buf][3 * r->prec + k] += kernel_bytes_eng(r->n, r->nnz, k, r->prec ? 4 : 8);
     }
   max_active = std::max(max_active, (double)nact);
   ECHK(hipMemcpyAsync(d_stptr[buf], h_stptr[buf].data(), nact * sizeof(BState*), hipMemcpyHostToDevice, stream));
-  // one chunk of iterations; the middle one carries dispatch events
+  // one chunk of iterations; the middle one's launches carry dispatch
+  // events, and (chunks of 3 or more) the chunk's first A and last C
   const int qt = chunk / 2;
-  hipEvent_t* ev = tev[buf];
-  const bool g0 = !grp[0].active.empty(), g1 = !grp[1].active.empty();
-  for (int q = 0; q < chunk; ++q) {
-    const bool e = (q == qt);
-    if (g0) ECHK(launch_bicg_A<double>(batch<double>(0, q), stream, e ? ev : nullptr));
-    if (g1) ECHK(launch_bicg_A<float>(batch<float>(1, q), stream, e ? ev + 6 : nullptr));
-    if (g0) ECHK(launch_bicg_B<double>(batch<double>(0, q), stream, e ? ev + 2 : nullptr));
-    if (g1) ECHK(launch_bicg_B<float>(batch<float>(1, q), stream, e ? ev + 8 : nullptr));
-    if (g0) ECHK(launch_bicg_C<double>(batch<double>(0, q), stream, e ? ev + 4 : nullptr));
-    if (g1) ECHK(launch_bicg_C<float>(batch<float>(1, q), stream, e ? ev + 10 : nullptr));
-  }
-  n_launch += (double)chunk * 3 * ((g0 ? 1 : 0) + (g1 ? 1 : 0));
+  const bool span = chunk >= 3;
+  bool* on = round_on[buf];
+  int ngroups = 0;
+  for (int g = 0; g < kGroups; ++g) ngroups += (on[g] = !grp[g].active.empty());
+  for (int q = 0; q < chunk; ++q)
+    for (int k = 0; k < 3; ++k)
+      for (int g = 0; g < kGroups; ++g) {
+        if (!on[g]) continue;
+        hipStream_t gs = lane_stream[g % kLanes];
+        hipEvent_t* ev = nullptr;
+        if (q == qt) ev = tev[buf] + kEv * g + 2 * k;
+        else if (span && q == 0 && k == 0) ev = tev[buf] + kEv * g + 6;
+        else if (span && q == chunk - 1 && k == 2) ev = tev[buf] + kEv * g + 8;
+        if (g / kLanes == 0) {
+          const Batch<double> B = batch<double>(g, q);
+          ECHK(k == 0 ? launch_bicg_A<double>(B, gs, ev)
+                      : k == 1 ? launch_bicg_B<double>(B, gs, ev) : launch_bicg_C<double>(B, gs, ev));
+        } else {
+          const Batch<float> B = batch<float>(g, q);
+          ECHK(k == 0 ? launch_bicg_A<float>(B, gs, ev)
+                      : k == 1 ? launch_bicg_B<float>(B, gs, ev) : launch_bicg_C<float>(B, gs, ev));
+        }
+      }
+  n_launch += (double)chunk * 3 * ngroups;
   n_chunks += 1;
+  // join: lane 0 waits for lane 1's chunk before the gather
+  for (int l = 1; l < kLanes; ++l) {
+    ECHK(hipEventRecord(lane_join[buf], lane_stream[l]));
+    ECHK(hipStreamWaitEvent(stream, lane_join[buf], 0));
+  }
   const int words = nact * (int)(sizeof(BState) / 8);
   hipLaunchKernelGGL(gather_states_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, d_stptr[buf],
                      d_states[buf], nact);
@@ -315,23 +380,68 @@ void ns3d_engine::queue_round(int buf, bool restart[2]) {
 // a solve that asked for a restart is restarted by that round only if the
 // round restarts its group, else it is flagged for the next one.  Stopped
 // solves are retired and their callers woken.  Returns whether any was.
-bool ns3d_engine::process_round(int buf, bool restart[2]) {
+bool ns3d_engine::process_round(int buf, bool restart[kGroups]) {
   ECHK(hipEventSynchronize(round_done[buf]));
-  for (int g = 0; g < 2; ++g) {
-    if (round_bytes[buf][3 * g] == 0) continue;
-    for (int k = 0; k < 3; ++k) {
-      float e = 0.f;
-      if (hipEventElapsedTime(&e, tev[buf][6 * g + 2 * k], tev[buf][6 * g + 2 * k + 1]) != hipSuccess) continue;
-      ms[g][k] += e;
-      launches[g][k] += 1;
-      bytes[g][k] += round_bytes[buf][3 * g + k];
-    }
-  }
   bool any_done = false;
   std::lock_guard<std::mutex> lk(mu);
-  const std::vector<long long>& ord = order[buf];
+  const std::vector<RoundEntry>& ord = order[buf];
+  // algorithmic bytes of the work the round did: a solve iterated from its
+  // count before the round (the last one read back) or its base, whichever
+  // is later, up to its count after it; the sampled launches count the
+  // solves whose iteration it0 + qt was among those
+  const int qt = chunk / 2;
+  double sb[kGroups][3] = {};
+  double cb = 0;
   for (size_t i = 0; i < ord.size(); ++i) {
-    auto lv = live.find(ord[i]);
+    const RoundEntry& o = ord[i];
+    auto lv = live.find(o.serial);
+    if (lv == live.end()) continue;  // retired by the previous round's results
+    EngineRequest* r = lv->second;
+    const long long lo = std::max(r->it_done, o.it0);
+    const long long hi = std::min<long long>(h_states[buf][i].it_c, o.it0 + chunk);
+    r->it_done = h_states[buf][i].it_c;
+    if (hi <= lo) continue;
+    const int vb = o.group / kLanes ? 4 : 8;
+    double per = 0;
+    for (int k = 0; k < 3; ++k) {
+      const double kb = kernel_bytes_eng(o.n, o.nnz, k, vb);
+      per += kb;
+      if (lo <= o.it0 + qt && o.it0 + qt < hi) sb[o.group][k] += kb;
+    }
+    cb += per * (double)(hi - lo);
+  }
+  const bool* on = round_on[buf];
+  hipEvent_t ref = nullptr;
+  double lo_t = 0, hi_t = 0;
+  bool span_ok = chunk >= 3;
+  for (int g = 0; g < kGroups; ++g) {
+    if (!on[g]) continue;
+    hipEvent_t* e = tev[buf] + kEv * g;
+    for (int k = 0; k < 3; ++k) {
+      float d = 0.f;
+      if (sb[g][k] <= 0 || hipEventElapsedTime(&d, e[2 * k], e[2 * k + 1]) != hipSuccess) continue;
+      ms[g / kLanes][k] += d;
+      launches[g / kLanes][k] += 1;
+      bytes[g / kLanes][k] += sb[g][k];
+    }
+    if (!span_ok) continue;
+    float t0 = 0.f, t1 = 0.f;
+    if (!ref) {
+      ref = e[6];
+      if (hipEventElapsedTime(&t1, ref, e[9]) != hipSuccess) span_ok = false;
+    } else if (hipEventElapsedTime(&t0, ref, e[6]) != hipSuccess || hipEventElapsedTime(&t1, ref, e[9]) != hipSuccess) {
+      span_ok = false;
+    }
+    lo_t = std::min(lo_t, (double)t0);
+    hi_t = std::max(hi_t, (double)t1);
+  }
+  if (span_ok && ref && hi_t > lo_t && cb > 0) {
+    ch_ms += hi_t - lo_t;
+    ch_n += 1;
+    ch_bytes += cb;
+  }
+  for (size_t i = 0; i < ord.size(); ++i) {
+    auto lv = live.find(ord[i].serial);
     if (lv == live.end()) continue;  // retired by the previous round's results
     EngineRequest* r = lv->second;
     const BState& st = h_states[buf][i];
@@ -344,7 +454,7 @@ bool ns3d_engine::process_round(int buf, bool restart[2]) {
       if (next_runs && r->restart_queued) {
         r->it_next = st.it_c + chunk;
       } else {
-        restart[r->prec] = true;
+        restart[r->group] = true;
         r->it_next = st.it_c;
       }
       continue;
@@ -355,11 +465,11 @@ bool ns3d_engine::process_round(int buf, bool restart[2]) {
     rows_x_iters += (double)r->n * (double)st.it_c;
     n_done += 1;
     any_done = true;
-    auto& act = grp[r->prec].active;
+    auto& act = grp[r->group].active;
     auto it = std::find(act.begin(), act.end(), r);
     if (it != act.end()) {
       act.erase(it);
-      grp[r->prec].dirty = true;
+      grp[r->group].dirty = true;
     }
   }
   rounds_done += 1;
@@ -374,7 +484,7 @@ bool ns3d_engine::process_round(int buf, bool restart[2]) {
 void ns3d_engine::run() {
   try {
     ECHK(hipSetDevice(device));
-    bool restart[2] = {false, false};
+    bool restart[kGroups] = {};
     int qbuf = 0;       // buffer of the next round to queue
     int inflight = -1;  // buffer of the queued round not yet processed
     for (;;) {
@@ -383,8 +493,12 @@ void ns3d_engine::run() {
         std::unique_lock<std::mutex> lk(mu);
         if (inflight < 0) cv_work.wait(lk, [&] { return stopping || !pending.empty() || any_active(); });
         if (stopping && pending.empty() && !any_active() && inflight < 0) break;
-        int tiles[2] = {grp[0].ntiles, grp[1].ntiles};
-        int solves[2] = {(int)grp[0].active.size(), (int)grp[1].active.size()};
+        // room per precision (both lanes together; each lane gets at most that)
+        int tiles[2] = {0, 0}, solves[2] = {0, 0};
+        for (int g = 0; g < kGroups; ++g) {
+          tiles[g / kLanes] += grp[g].ntiles;
+          solves[g / kLanes] += (int)grp[g].active.size();
+        }
         while (!pending.empty()) {
           EngineRequest* r = pending.front();
           const int64_t n = r->prec == 0 ? r->d64.S.n : r->d32.S.n;

@@ -126,10 +126,22 @@ struct Tile {
   int pad;
   long long begin, end;
 };
+// One entry of a batched launch's tile table: a tile with its solve's
+// descriptor and iteration base (the solve's executed iterations when the
+// chunk was queued) copied in, so a block reaches all its operands with one
+// table load and no dependent one.
+template <typename T>
+struct TileDesc {
+  SolveDesc<T> d;
+  long long begin, end;
+  long long it0;
+  int solve;
+  int pad;
+};
 // The operands of one launch over a set of solves of storage type T.
-// Either a table (d, tiles, it0: the executed iterations of each solve when
-// the chunk was queued) or one solve passed by value (single = 1: `one`,
-// tiles of `per` rows, one_it0).  q is the launch's index in its chunk, so a
+// Either tables (d: one descriptor per solve, for the per-solve kernels;
+// tiles) or one solve passed by value (single = 1: `one`, tiles of `per`
+// rows, one_it0).  q is the launch's index in its chunk, so a
 // row kernel knows its iteration (it0 + q) -- and with it the p/v buffer and
 // the scalar slot -- without a dependent load; it exits if the solve has
 // stopped or its own count disagrees.
@@ -137,8 +149,7 @@ constexpr int kPartialSlots = 8;
 template <typename T>
 struct Batch {
   const SolveDesc<T>* d;
-  const Tile* tiles;
-  const long long* it0;
+  const TileDesc<T>* tiles;
   int nsolves, ntiles;
   int q;
   int single;
@@ -152,7 +163,10 @@ Batch<T> single_batch(const SolveDesc<T>& d, long long it0, int q);
 // Tiles of one solve of n rows: between 1 and kMaxTilesPerSolve tiles of
 // a multiple of kBlock rows, at least kTileRows rows each where n allows.
 constexpr int kMaxTilesPerSolve = 1024;
-constexpr int kTileRows = 1536;
+#ifndef NS3D_TILE_ROWS
+#define NS3D_TILE_ROWS 1536
+#endif
+constexpr int kTileRows = NS3D_TILE_ROWS;
 int tiles_for(int64_t n);
 // Appends the tiles of solve `solve` (n rows) to out; returns its [t0, t1).
 void make_tiles(int solve, int64_t n, std::vector<Tile>& out, int& t0, int& t1);

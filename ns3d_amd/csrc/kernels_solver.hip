@@ -249,9 +249,12 @@ __device__ __forceinline__ void tile_of(const Batch<T>& B, int t, Tile& tl, GDes
     tl.end = b + B.per < n ? b + B.per : n;
     it = B.one_it0 + B.q;
   } else {
-    tl = B.tiles[t];
-    D = load_desc(B.d + tl.solve);
-    it = (B.it0 ? B.it0[tl.solve] : 0) + B.q;
+    const TileDesc<T>* e = B.tiles + t;
+    D = to_gdesc(e->d);
+    tl.solve = e->solve;
+    tl.begin = e->begin;
+    tl.end = e->end;
+    it = e->it0 + B.q;
   }
 }
 template <typename T>
@@ -399,6 +402,12 @@ bicg_A_kernel(Batch<T> B) {
   long long it;
   tile_of(B, t, tl, D, it);
   NS3D_GLOBAL BState* st = D.w.st;
+  // the previous kernel's partials are loaded together with the status;
+  // launches queued after the solve stopped exit here, before any row load
+  const int status = st->status;
+  const long long done = st->it_c;
+  double pa[2];
+  partials_load<2>(D.slot(kSlotC), D.t1 - D.t0, D.pstride, pa);
   const bool lead = leader_of(t, D.t0);
   const int sl = (int)(it & 1);
   const int par = sl;
@@ -408,21 +417,20 @@ bicg_A_kernel(Batch<T> B) {
   const NS3D_GLOBAL T* __restrict__ vold = par ? D.w.v[1] : D.w.v[0];
   NS3D_GLOBAL T* __restrict__ pnew = par ? D.w.p[0] : D.w.p[1];
   NS3D_GLOBAL T* __restrict__ vnew = par ? D.w.v[0] : D.w.v[1];
-  // launches queued after the solve stopped exit here, before any row load
-  if (st->status != BS_RUNNING || st->it_c != it) return;
+  if (status != BS_RUNNING || done != it) return;
   double acc = 0.0;
   bool stop = false;
   by_layout(D.S.layout, [&](auto LC) {
     constexpr int L = decltype(LC)::value;
     const int64_t n = D.S.n;
     int64_t row = tl.begin + threadIdx.x;
-    // first row, scalars and partials in flight together
+    // first row in flight during the reduction
     AIn<T> cur;
     if (row < tl.end) a_fetch<T, L>(D.S, D.w, pold, vold, (int)row, n, cur);
     const long long fresh_at = st->fresh_at;
     const double rho_prev = st->rho[sl ^ 1], alpha_prev = st->alpha[sl ^ 1], w_prev = st->w[sl ^ 1];
     double red[2];
-    sum_partials_all<2>(D.slot(kSlotC), D.t1 - D.t0, red, D.pstride);
+    partials_reduce<2>(pa, red);
     double rho;
     if (fresh_at == it) {
       rho = st->rho_fresh;
@@ -479,11 +487,15 @@ bicg_B_kernel(Batch<T> B) {
   long long it;
   tile_of(B, t, tl, D, it);
   NS3D_GLOBAL BState* st = D.w.st;
+  const int status = st->status;
+  const long long done = st->it_a;
+  double pa[1];
+  partials_load<1>(D.slot(kSlotA), D.t1 - D.t0, D.pstride, pa);
   const bool lead = leader_of(t, D.t0);
   const int sl = (int)(it & 1);
   const int par = sl ^ 1;
   const NS3D_GLOBAL T* __restrict__ vn = par ? D.w.v[1] : D.w.v[0];
-  if (st->status != BS_RUNNING || st->it_a != it) return;
+  if (status != BS_RUNNING || done != it) return;
   double a0 = 0.0, a1 = 0.0;
   by_layout(D.S.layout, [&](auto LC) {
     constexpr int L = decltype(LC)::value;
@@ -493,7 +505,7 @@ bicg_B_kernel(Batch<T> B) {
     if (row < tl.end) b_fetch<T, L>(D.S, D.w, vn, (int)row, n, cur);
     const double rho = st->rho[sl];
     double red[1];
-    sum_partials_all<1>(D.slot(kSlotA), D.t1 - D.t0, red, D.pstride);
+    partials_reduce<1>(pa, red);
     const double alpha = rho / red[0];
     if (lead) {
       st->r0v = red[0];
@@ -529,6 +541,10 @@ bicg_C_kernel(Batch<T> B) {
   long long it;
   tile_of(B, t, tl, D, it);
   NS3D_GLOBAL BState* st = D.w.st;
+  const int status = st->status;
+  const long long done = st->it_a;
+  double pa[2];
+  partials_load<2>(D.slot(kSlotB), D.t1 - D.t0, D.pstride, pa);
   const bool lead = leader_of(t, D.t0);
   const int sl = (int)(it & 1);
   const int par = sl ^ 1;
@@ -536,13 +552,13 @@ bicg_C_kernel(Batch<T> B) {
   const NS3D_GLOBAL T* __restrict__ invdiag = D.S.invdiag;
   const NS3D_GLOBAL T* __restrict__ pn = par ? D.w.p[1] : D.w.p[0];
   const NS3D_GLOBAL T* __restrict__ vn = par ? D.w.v[1] : D.w.v[0];
-  if (st->status != BS_RUNNING || st->it_a != it) return;
+  if (status != BS_RUNNING || done != it) return;
   int64_t row = tl.begin + threadIdx.x;
   CIn<T> cur;
   if (row < tl.end) c_fetch<T>(invdiag, x, D.w, pn, vn, row, cur);
   const double alpha = st->alpha[sl];
   double red[2];
-  sum_partials_all<2>(D.slot(kSlotB), D.t1 - D.t0, red, D.pstride);
+  partials_reduce<2>(pa, red);
   const double om = (red[1] > 0.0) ? red[0] / red[1] : 0.0;
   if (lead) {
     st->ts = red[0];

@@ -32,7 +32,8 @@ EXPORTS = [
     "ns3d_compute_velocity_field", "ns3d_compute_hat", "ns3d_assemble_pressure",
     "ns3d_compute_pressure_correction", "ns3d_compute_pressure_field", "ns3d_correct_velocity",
     "ns3d_correct_pressure", "ns3d_compute_div", "ns3d_compute_permeability", "ns3d_get_matrix",
-    "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_set_precision", "ns3d_sync",
+    "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_set_precision",
+    "ns3d_set_mixed_min_rows", "ns3d_sync",
     "ns3d_engine_create", "ns3d_engine_destroy", "ns3d_attach_engine", "ns3d_engine_stats",
     "ns3d_footprint", "ns3d_ctx_bytes",
 ]
@@ -110,6 +111,7 @@ def lib():
         L.ns3d_timing_read.argtypes = [vp, P(dbl)]
         L.ns3d_matrix_nnz.argtypes = [vp, P(i64)]
         L.ns3d_set_precision.argtypes = [vp, i32, dbl]
+        L.ns3d_set_mixed_min_rows.argtypes = [vp, i64]
         L.ns3d_sync.argtypes = [vp]
         _lib = L
     return _lib
@@ -293,8 +295,10 @@ class NavierStokesGPU:
         _check(lib().ns3d_ctx_bytes(self.h, C.byref(out)))
         return out.value
 
-    def set_precision(self, mixed: bool, inner_tol: float = 1e-4):
+    def set_precision(self, mixed: bool, inner_tol: float = 1e-4, min_rows: int = 0):
+        """min_rows: in the mixed form, systems with fewer rows are solved in FP64."""
         _check(lib().ns3d_set_precision(self.h, int(bool(mixed)), inner_tol))
+        _check(lib().ns3d_set_mixed_min_rows(self.h, int(min_rows)))
 
     def nnz(self):
         out = np.zeros(4, np.int64)
@@ -333,6 +337,7 @@ class Engine:
             for k, name in enumerate("ABC"):
                 o = 5 + 9 * q + 3 * k
                 d[f"{prec}_{name}"] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
+        d["chunk"] = (float(out[23]), float(out[24]), float(out[25]))
         return d
 
     def close(self):

@@ -41,7 +41,10 @@ int ns3d_solver_create(const ns3d_run_params* p, const uint8_t* img, int nx, int
         p->vsolvertol, p->psolvertol, p->pcsolvertol, p->maxitsolver, p->alpha, p->beta, p->p0, p->gradp, p->mu,
         p->rho, p->porevalue, p->direction, p->deltaRMR, p->save_step, p->border, p->device, p->verbose,
         p->write_files != 0, p->outdir ? p->outdir : "");
-    if (p->precision != 0) check(ns3d_set_precision(s->base->ctx, p->precision, p->inner_tol), "ns3d_set_precision");
+    if (p->precision != 0) {
+      check(ns3d_set_precision(s->base->ctx, p->precision, p->inner_tol), "ns3d_set_precision");
+      check(ns3d_set_mixed_min_rows(s->base->ctx, p->mixed_min_rows), "ns3d_set_mixed_min_rows");
+    }
     *out = s.release();
     return 0;
   } catch (const std::exception& e) {

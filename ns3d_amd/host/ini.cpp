@@ -85,7 +85,7 @@ extern "C" void ns3d_default_run_params(ns3d_run_params* p) {
   p->outdir = nullptr;
   p->name = nullptr;
   p->precision = 0;
-  p->reserved = 0;
+  p->mixed_min_rows = 0;
   p->inner_tol = 1e-4;
 }
 

@@ -12,7 +12,9 @@ import numpy as np
 from . import gpu as _gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libns3d_host.so")
+# NS3D_HOST_LIBRARY may name another build of the library (one next to a
+# libns3d_gpu.so built with other tuning macros, which it then loads).
+LIB_PATH = os.environ.get("NS3D_HOST_LIBRARY") or os.path.join(HERE, "libns3d_host.so")
 METHODS = {"SIMPLEC": 0, "SIMPLER": 1, "SIMPLERR": 2, "IDEAL": 3}
 
 EXPORTS = ["ns3d_default_run_params", "ns3d_solver_create", "ns3d_solver_info", "ns3d_solver_run",
@@ -30,7 +32,7 @@ class RunParams(C.Structure):
                 ("save_step", C.c_int32), ("porevalue", C.c_int32), ("direction", C.c_int32),
                 ("border", C.c_int32), ("method", C.c_int32), ("verbose", C.c_int32),
                 ("write_files", C.c_int32), ("device", C.c_int32), ("outdir", C.c_char_p),
-                ("name", C.c_char_p), ("precision", C.c_int32), ("reserved", C.c_int32),
+                ("name", C.c_char_p), ("precision", C.c_int32), ("mixed_min_rows", C.c_int32),
                 ("inner_tol", C.c_double)]
 
 
