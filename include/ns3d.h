@@ -256,7 +256,9 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
  * sampled, each launch timed by its own dispatch); out[23..25] = whole
  * chunks (the engine runs its solves on two streams; from the first A start
  * to the last C stop over all of them): milliseconds, count, algorithmic
- * bytes of the iterations they executed. */
+ * bytes of the iterations they executed; out[26..28] = the dispatcher
+ * thread's milliseconds admitting and queueing rounds, processing their
+ * states, and waiting for the round in flight. */
 ns3d_status ns3d_engine_stats(ns3d_engine* eng, double out[32]);
 
 /* Upper bound of the device bytes a context of an nx x ny x nz image with

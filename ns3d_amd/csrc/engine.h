@@ -38,7 +38,7 @@ struct EngineRequest {
   // next round is queued; iteration base of the last queued round and
   // whether that round restarted this solve's group first
   long long serial = 0;
-  int group = 0;  // 2 * prec + lane
+  int group = 0;  // kLanes * prec + lane
   long long it_next = 0;
   long long it_done = 0;  // executed iterations as last read back
   long long it_queued = -1;
@@ -84,11 +84,14 @@ struct ns3d_engine {
     int ntiles = 0;
     bool dirty = true;
   };
-  // Solves are split into kGroups groups: group g holds precision g / 2
-  // (0 FP64, 1 FP32) on lane g % 2.  Each lane has its own stream, so the
+  // Solves are split into kGroups groups: group g holds precision g / kLanes
+  // (0 FP64, 1 FP32) on lane g % kLanes.  Each lane has its own stream, so the
   // launches of the two lanes of a precision run concurrently and one's
   // ramp-up and drain overlap the other's work.
-  static constexpr int kLanes = 2;
+#ifndef NS3D_ENGINE_LANES
+#define NS3D_ENGINE_LANES 2
+#endif
+  static constexpr int kLanes = NS3D_ENGINE_LANES;
   static constexpr int kGroups = 2 * kLanes;
   void run();
   void admit(std::vector<ns3d::EngineRequest*>& reqs, int buf);
@@ -134,7 +137,7 @@ struct ns3d_engine {
   long long next_serial = 1;
   bool round_on[2][kGroups] = {};      // groups with launches in the round
   hipEvent_t round_done[2] = {};
-  hipEvent_t lane_join[2] = {};        // lane 1's chunk done (lane 0 waits before the gather)
+  hipEvent_t lane_join[2][kLanes] = {};  // lane l's chunk done (lane 0 waits before the gather)
   // dispatch events per group: [A, B, C of the sampled iteration][start,
   // stop], then the chunk's first A and last C [start, stop]
   static constexpr int kEv = 10;
@@ -149,5 +152,8 @@ struct ns3d_engine {
   // count, algorithmic bytes of the iterations executed in them
   double ms[2][3] = {}, launches[2][3] = {}, bytes[2][3] = {};
   double ch_ms = 0, ch_n = 0, ch_bytes = 0;
+  // dispatcher thread: milliseconds admitting and queueing, waiting for the
+  // round in flight, processing its states
+  double host_queue_ms = 0, host_process_ms = 0, host_wait_ms = 0;
   double rows_x_iters = 0;
 };

@@ -22,6 +22,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <stdexcept>
 
@@ -89,7 +90,7 @@ ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) 
     h_stptr[b].resize(kGroups * kEngMaxSolves);
     ECHK(hipHostMalloc((void**)&h_states[b], sizeof(BState) * kGroups * kEngMaxSolves, hipHostMallocDefault));
     ECHK(hipEventCreateWithFlags(&round_done[b], hipEventDisableTiming));
-    ECHK(hipEventCreateWithFlags(&lane_join[b], hipEventDisableTiming));
+    for (auto& e : lane_join[b]) ECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : tev[b]) ECHK(hipEventCreate(&e));
   }
   thr = std::thread([this] { run(); });
@@ -107,7 +108,8 @@ ns3d_engine::~ns3d_engine() {
   for (int b = 0; b < 2; ++b) {
     if (h_states[b]) (void)hipHostFree(h_states[b]);
     if (round_done[b]) (void)hipEventDestroy(round_done[b]);
-    if (lane_join[b]) (void)hipEventDestroy(lane_join[b]);
+    for (auto& e : lane_join[b])
+      if (e) (void)hipEventDestroy(e);
     for (auto& e : tev[b])
       if (e) (void)hipEventDestroy(e);
   }
@@ -158,6 +160,9 @@ void ns3d_engine::stats(double* out, int count) {
   v[23] = ch_ms;
   v[24] = ch_n;
   v[25] = ch_bytes;
+  v[26] = host_queue_ms;
+  v[27] = host_process_ms;
+  v[28] = host_wait_ms;
   for (int q = 0; q < count && q < 32; ++q) out[q] = v[q];
 }
 
@@ -209,8 +214,10 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
   int64_t rows[kGroups];
   for (int g = 0; g < kGroups; ++g) rows[g] = group_rows(g);
   for (EngineRequest* r : reqs) {
-    const int g0 = 2 * r->prec;
-    r->group = rows[g0 + 1] < rows[g0] ? g0 + 1 : g0;
+    const int g0 = kLanes * r->prec;
+    r->group = g0;
+    for (int l = 1; l < kLanes; ++l)
+      if (rows[g0 + l] < rows[r->group]) r->group = g0 + l;
     rows[r->group] += r->n;
   }
   for (int g = 0; g < kGroups; ++g) {
@@ -360,8 +367,8 @@ void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
   n_chunks += 1;
   // join: lane 0 waits for lane 1's chunk before the gather
   for (int l = 1; l < kLanes; ++l) {
-    ECHK(hipEventRecord(lane_join[buf], lane_stream[l]));
-    ECHK(hipStreamWaitEvent(stream, lane_join[buf], 0));
+    ECHK(hipEventRecord(lane_join[buf][l], lane_stream[l]));
+    ECHK(hipStreamWaitEvent(stream, lane_join[buf][l], 0));
   }
   const int words = nact * (int)(sizeof(BState) / 8);
   hipLaunchKernelGGL(gather_states_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, d_stptr[buf],
@@ -512,13 +519,23 @@ void ns3d_engine::run() {
         }
       }
       bool queued = false;
+      const auto h0 = std::chrono::steady_clock::now();
       if (!admitted.empty()) admit(admitted, qbuf);
       if (any_active()) {
         queue_round(qbuf, restart);
         queued = true;
       }
+      const auto h1 = std::chrono::steady_clock::now();
       bool woke = false;
-      if (inflight >= 0) woke = process_round(inflight, restart);
+      if (inflight >= 0) {
+        ECHK(hipEventSynchronize(round_done[inflight]));
+        const auto h2 = std::chrono::steady_clock::now();
+        woke = process_round(inflight, restart);
+        const auto h3 = std::chrono::steady_clock::now();
+        host_wait_ms += std::chrono::duration<double, std::milli>(h2 - h1).count();
+        host_process_ms += std::chrono::duration<double, std::milli>(h3 - h2).count();
+      }
+      host_queue_ms += std::chrono::duration<double, std::milli>(h1 - h0).count();
       if (woke || inflight >= 0) cv_done.notify_all();
       inflight = queued ? qbuf : -1;
       if (queued) qbuf ^= 1;

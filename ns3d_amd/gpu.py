@@ -338,6 +338,7 @@ class Engine:
                 o = 5 + 9 * q + 3 * k
                 d[f"{prec}_{name}"] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
         d["chunk"] = (float(out[23]), float(out[24]), float(out[25]))
+        d["host_ms"] = dict(queue=float(out[26]), process=float(out[27]), wait=float(out[28]))
         return d
 
     def close(self):
