@@ -249,6 +249,12 @@ void ns3d_engine_destroy(ns3d_engine* eng);
  * workspaces, so that the three momentum solves of
  * ns3d_compute_velocity_field go to the engine together. */
 ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
+/* on (default): each chunk of iterations is submitted as one HIP graph per
+ * engine stream (reused while the grids stay the same; the kernels read the
+ * table sizes from device memory), except every 8th round, which is launched
+ * kernel by kernel with dispatch events for ns3d_engine_stats; off: every
+ * round kernel by kernel.  Results are the same either way. */
+ns3d_status ns3d_engine_set_graphs(ns3d_engine* eng, int on);
 /* out[32]: chunks, launches, solves done, max active solves, sum of rows x
  * iterations; then per precision q (0 FP64, 1 FP32) and kernel k (A, B, C),
  * out[5 + 9q + 3k] = sampled milliseconds, sampled launches, algorithmic
@@ -258,7 +264,9 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
  * to the last C stop over all of them): milliseconds, count, algorithmic
  * bytes of the iterations they executed; out[26..28] = the dispatcher
  * thread's milliseconds admitting and queueing rounds, processing their
- * states, and waiting for the round in flight. */
+ * states, and waiting for the round in flight; out[29..30] = over the
+ * sampled iterations, the tiles of solves that iterated and all tiles
+ * launched. */
 ns3d_status ns3d_engine_stats(ns3d_engine* eng, double out[32]);
 
 /* Upper bound of the device bytes a context of an nx x ny x nz image with

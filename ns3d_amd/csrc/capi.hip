@@ -1115,6 +1115,12 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* c, ns3d_engine* e) {
   return NS3D_OK;
 }
 
+ns3d_status ns3d_engine_set_graphs(ns3d_engine* e, int on) {
+  if (!e) return fail_invalid("ns3d_engine_set_graphs: null engine");
+  e->set_graphs(on != 0);
+  return NS3D_OK;
+}
+
 ns3d_status ns3d_engine_stats(ns3d_engine* e, double out[32]) {
   if (!e || !out) return fail_invalid("ns3d_engine_stats: bad arguments");
   e->stats(out, 32);

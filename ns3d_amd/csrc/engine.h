@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <mutex>
@@ -39,6 +40,7 @@ struct EngineRequest {
   // whether that round restarted this solve's group first
   long long serial = 0;
   int group = 0;  // kLanes * prec + lane
+  int tiles = 0;  // tiles in the current table
   long long it_next = 0;
   long long it_done = 0;  // executed iterations as last read back
   long long it_queued = -1;
@@ -57,6 +59,11 @@ struct ns3d_engine {
   // same rounds); returns when all have stopped.
   void solve_many(ns3d::EngineRequest* const* reqs, int n);
   void stats(double* out, int count);
+  // Chunks are submitted as one graph per lane (graph kernels read each
+  // table's size from the device), except every kSampleEvery-th round, whose
+  // launches go one by one with dispatch events for the statistics; off:
+  // every round is launched one by one.
+  void set_graphs(bool on);
   // Returns once every round queued before the call has been processed, so
   // no queued kernel refers to a solve that has already returned.  A context
   // calls it before freeing memory its solves used.
@@ -69,7 +76,8 @@ struct ns3d_engine {
   struct Group {
     std::vector<ns3d::EngineRequest*> active;
     void* d_desc = nullptr;
-    void* d_tiles = nullptr;   // TileDesc<T> entries
+    void* d_tiles = nullptr;   // TileDesc<T> entries, after a kTableHeader-byte header
+    int* d_nt = nullptr;       // the header: the table's tile count (read by graph launches)
     void* d_idesc = nullptr;
     void* d_itiles = nullptr;
     // host images of the tables, rewritten only for a round queued after
@@ -97,6 +105,10 @@ struct ns3d_engine {
   void admit(std::vector<ns3d::EngineRequest*>& reqs, int buf);
   void rebuild(int g, int buf);
   void upload_tiles(int g, int buf, bool restarted);
+  // The instantiated graph of one chunk on lane `lane` for the given grids
+  // (per precision; 0: no launches), built on first use.
+  hipGraphExec_t chunk_graph(int lane, const int grid[2]);
+  void launch_direct(int buf, const bool* on);
   void queue_round(int buf, bool restart[kGroups]);
   bool process_round(int buf, bool restart[kGroups]);
   template <typename T>
@@ -109,6 +121,11 @@ struct ns3d_engine {
   }
   int64_t group_rows(int g) const;
 
+  static constexpr int kSampleEvery = 8;
+  std::atomic<bool> graphs{true};
+  long long rounds_started = 0;
+  bool round_sampled[2] = {};
+  std::unordered_map<uint64_t, hipGraphExec_t> gcache[kLanes];
   hipStream_t lane_stream[kLanes] = {};
   hipStream_t stream = nullptr;  // lane 0's: gathers and completion events
   std::thread thr;
@@ -131,6 +148,7 @@ struct ns3d_engine {
     long long it0;  // the solve's iteration base in the round
     int group;
     int64_t n, nnz;
+    int tiles;
   };
   std::vector<RoundEntry> order[2];
   std::unordered_map<long long, ns3d::EngineRequest*> live;
@@ -155,5 +173,8 @@ struct ns3d_engine {
   // dispatcher thread: milliseconds admitting and queueing, waiting for the
   // round in flight, processing its states
   double host_queue_ms = 0, host_process_ms = 0, host_wait_ms = 0;
+  // sampled rounds: tiles of the solves that iterated at the sampled
+  // iteration, and all tiles launched in it
+  double tiles_live = 0, tiles_all = 0;
   double rows_x_iters = 0;
 };

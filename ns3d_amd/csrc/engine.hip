@@ -32,6 +32,19 @@ namespace {
 
 constexpr int kEngMaxSolves = 4096;
 constexpr int kEngMaxTiles = 1 << 17;
+constexpr int kTableHeader = 256;  // bytes before a tile table's entries (its tile count)
+
+// Grid of a graph launch over ntiles tiles: rounded up to one of four steps
+// per octave, so graphs are reused while the tile count drifts.
+int graph_grid(int ntiles) {
+  int g = 64;
+  while (g < ntiles) g *= 2;
+  if (g > 64) {
+    const int step = g / 8;  // four steps in (g/2, g]
+    g = (ntiles + step - 1) / step * step;
+  }
+  return std::min(tile_grid(g), tile_grid(kEngMaxTiles));
+}
 
 #define ECHK(expr)                                                                     \
   do {                                                                                 \
@@ -77,8 +90,10 @@ ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) 
   for (Group& g : grp) {
     g.d_desc = dalloc(dsz * kEngMaxSolves);
     g.d_idesc = dalloc(dsz * kEngMaxSolves);
-    g.d_tiles = dalloc(tsz * kEngMaxTiles);
-    g.d_itiles = dalloc(tsz * kEngMaxTiles);
+    char* tb = (char*)dalloc(kTableHeader + tsz * kEngMaxTiles);
+    g.d_nt = (int*)tb;
+    g.d_tiles = tb + kTableHeader;
+    g.d_itiles = (char*)dalloc(kTableHeader + tsz * kEngMaxTiles) + kTableHeader;
     for (int b = 0; b < 2; ++b) {
       g.h_desc[b].resize(dsz * kEngMaxSolves);
       g.h_idesc[b].resize(dsz * kEngMaxSolves);
@@ -113,10 +128,14 @@ ns3d_engine::~ns3d_engine() {
     for (auto& e : tev[b])
       if (e) (void)hipEventDestroy(e);
   }
+  for (auto& c : gcache)
+    for (auto& kv : c) (void)hipGraphExecDestroy(kv.second);
   for (void* p : allocs) (void)hipFree(p);
   for (auto& st : lane_stream)
     if (st) (void)hipStreamDestroy(st);
 }
+
+void ns3d_engine::set_graphs(bool on) { graphs = on; }
 
 void ns3d_engine::solve(EngineRequest& req) {
   EngineRequest* r = &req;
@@ -163,6 +182,8 @@ void ns3d_engine::stats(double* out, int count) {
   v[26] = host_queue_ms;
   v[27] = host_process_ms;
   v[28] = host_wait_ms;
+  v[29] = tiles_live;
+  v[30] = tiles_all;
   for (int q = 0; q < count && q < 32; ++q) out[q] = v[q];
 }
 
@@ -196,8 +217,9 @@ Batch<T> ns3d_engine::batch(int g, int q) const {
 template <typename T>
 static void tile_table(std::vector<char>& img, const std::vector<Tile>& tiles, const SolveDesc<T>* desc,
                        const std::vector<EngineRequest*>* reqs) {
-  img.resize(tiles.size() * sizeof(TileDesc<T>));
-  TileDesc<T>* e = (TileDesc<T>*)img.data();
+  img.assign(kTableHeader + tiles.size() * sizeof(TileDesc<T>), 0);
+  *(int*)img.data() = (int)tiles.size();
+  TileDesc<T>* e = (TileDesc<T>*)(img.data() + kTableHeader);
   for (size_t i = 0; i < tiles.size(); ++i) {
     const int s = tiles[i].solve;
     e[i].d = desc[s];
@@ -239,7 +261,8 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
       if (ns == 0) return;
       tile_table(G.h_itiles[buf], tiles, hd, nullptr);
       ECHK(hipMemcpyAsync(G.d_idesc, hd, ns * sizeof(SolveDesc<T>), hipMemcpyHostToDevice, gs));
-      ECHK(hipMemcpyAsync(G.d_itiles, G.h_itiles[buf].data(), G.h_itiles[buf].size(), hipMemcpyHostToDevice, gs));
+      ECHK(hipMemcpyAsync((char*)G.d_itiles - kTableHeader, G.h_itiles[buf].data(), G.h_itiles[buf].size(),
+                          hipMemcpyHostToDevice, gs));
       Batch<T> B{};
       B.d = (const SolveDesc<T>*)G.d_idesc;
       B.tiles = (const TileDesc<T>*)G.d_itiles;
@@ -280,7 +303,7 @@ void ns3d_engine::upload_tiles(int g, int buf, bool restarted) {
   };
   if (g / kLanes == 0) up(double{});
   else up(float{});
-  ECHK(hipMemcpyAsync(G.d_tiles, G.h_tiles[buf].data(), G.h_tiles[buf].size(), hipMemcpyHostToDevice,
+  ECHK(hipMemcpyAsync(G.d_nt, G.h_tiles[buf].data(), G.h_tiles[buf].size(), hipMemcpyHostToDevice,
                       lane_stream[g % kLanes]));
 }
 
@@ -295,6 +318,7 @@ void ns3d_engine::rebuild(int g, int buf) {
     for (size_t i = 0; i < G.active.size(); ++i) {
       SolveDesc<T> d = desc_of<T>(G.active[i]);
       make_tiles((int)i, d.S.n, G.tiles, d.t0, d.t1);
+      G.active[i]->tiles = d.t1 - d.t0;
       cur[i] = d;
     }
     G.ntiles = (int)G.tiles.size();
@@ -332,18 +356,49 @@ void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
   int nact = 0;
   for (int g = 0; g < kGroups; ++g)
     for (EngineRequest* r : grp[g].active) {
-      ord.push_back(RoundEntry{r->serial, r->it_queued, g, r->n, r->nnz});
+      ord.push_back(RoundEntry{r->serial, r->it_queued, g, r->n, r->nnz, r->tiles});
       h_stptr[buf][nact++] = r->prec == 0 ? r->d64.w.st : r->d32.w.st;
     }
   max_active = std::max(max_active, (double)nact);
   ECHK(hipMemcpyAsync(d_stptr[buf], h_stptr[buf].data(), nact * sizeof(BState*), hipMemcpyHostToDevice, stream));
-  // one chunk of iterations; the middle one's launches carry dispatch
-  // events, and (chunks of 3 or more) the chunk's first A and last C
-  const int qt = chunk / 2;
-  const bool span = chunk >= 3;
   bool* on = round_on[buf];
   int ngroups = 0;
   for (int g = 0; g < kGroups; ++g) ngroups += (on[g] = !grp[g].active.empty());
+  const bool sampled = !graphs || rounds_started % kSampleEvery == 0;
+  round_sampled[buf] = sampled;
+  ++rounds_started;
+  if (sampled) {
+    launch_direct(buf, on);
+  } else {
+    for (int l = 0; l < kLanes; ++l) {
+      int grid[2] = {0, 0};
+      for (int p = 0; p < 2; ++p)
+        if (on[p * kLanes + l]) grid[p] = graph_grid(grp[p * kLanes + l].ntiles);
+      if (grid[0] || grid[1]) ECHK(hipGraphLaunch(chunk_graph(l, grid), lane_stream[l]));
+    }
+  }
+  n_launch += (double)chunk * 3 * ngroups;
+  n_chunks += 1;
+  // join: lane 0 waits for lane 1's chunk before the gather
+  for (int l = 1; l < kLanes; ++l) {
+    ECHK(hipEventRecord(lane_join[buf][l], lane_stream[l]));
+    ECHK(hipStreamWaitEvent(stream, lane_join[buf][l], 0));
+  }
+  const int words = nact * (int)(sizeof(BState) / 8);
+  hipLaunchKernelGGL(gather_states_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, d_stptr[buf],
+                     d_states[buf], nact);
+  ECHK(hipGetLastError());
+  ECHK(hipMemcpyAsync(h_states[buf], d_states[buf], nact * sizeof(BState), hipMemcpyDeviceToHost, stream));
+  ECHK(hipEventRecord(round_done[buf], stream));
+  std::lock_guard<std::mutex> lk(mu);
+  rounds_queued += 1;
+}
+
+// One chunk launched kernel by kernel: the middle iteration's launches carry
+// dispatch events, and (chunks of 3 or more) the chunk's first A and last C.
+void ns3d_engine::launch_direct(int buf, const bool* on) {
+  const int qt = chunk / 2;
+  const bool span = chunk >= 3;
   for (int q = 0; q < chunk; ++q)
     for (int k = 0; k < 3; ++k)
       for (int g = 0; g < kGroups; ++g) {
@@ -363,21 +418,48 @@ void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
                       : k == 1 ? launch_bicg_B<float>(B, gs, ev) : launch_bicg_C<float>(B, gs, ev));
         }
       }
-  n_launch += (double)chunk * 3 * ngroups;
-  n_chunks += 1;
-  // join: lane 0 waits for lane 1's chunk before the gather
-  for (int l = 1; l < kLanes; ++l) {
-    ECHK(hipEventRecord(lane_join[buf][l], lane_stream[l]));
-    ECHK(hipStreamWaitEvent(stream, lane_join[buf][l], 0));
+}
+
+// A chain of chunk x (A, B, C) kernel nodes per active precision of the
+// lane, in the order of the direct launches; the kernels take their tables
+// from fixed device buffers and the tile counts from the table headers, so
+// the graph serves every round with the same grids.
+hipGraphExec_t ns3d_engine::chunk_graph(int lane, const int grid[2]) {
+  const uint64_t key = ((uint64_t)chunk << 48) | ((uint64_t)grid[0] << 24) | (uint64_t)grid[1];
+  auto& cache = gcache[lane];
+  auto f = cache.find(key);
+  if (f != cache.end()) return f->second;
+  if (cache.size() >= 64) {
+    ECHK(hipStreamSynchronize(lane_stream[lane]));
+    for (auto& kv : cache) ECHK(hipGraphExecDestroy(kv.second));
+    cache.clear();
   }
-  const int words = nact * (int)(sizeof(BState) / 8);
-  hipLaunchKernelGGL(gather_states_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, d_stptr[buf],
-                     d_states[buf], nact);
-  ECHK(hipGetLastError());
-  ECHK(hipMemcpyAsync(h_states[buf], d_states[buf], nact * sizeof(BState), hipMemcpyDeviceToHost, stream));
-  ECHK(hipEventRecord(round_done[buf], stream));
-  std::lock_guard<std::mutex> lk(mu);
-  rounds_queued += 1;
+  hipGraph_t gr = nullptr;
+  ECHK(hipGraphCreate(&gr, 0));
+  hipGraphNode_t prev = nullptr;
+  for (int q = 0; q < chunk; ++q)
+    for (int k = 0; k < 3; ++k)
+      for (int p = 0; p < 2; ++p) {
+        if (!grid[p]) continue;
+        const int g = p * kLanes + lane;
+        hipGraphNode_t nd = nullptr;
+        if (p == 0) {
+          Batch<double> B = batch<double>(g, q);
+          B.ntiles_dev = grp[g].d_nt;
+          ECHK(graph_add_bicg<double>(gr, k, B, grid[p], prev, &nd));
+        } else {
+          Batch<float> B = batch<float>(g, q);
+          B.ntiles_dev = grp[g].d_nt;
+          ECHK(graph_add_bicg<float>(gr, k, B, grid[p], prev, &nd));
+        }
+        prev = nd;
+      }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  ECHK(e);
+  cache[key] = ex;
+  return ex;
 }
 
 // After round buf has completed: sampled kernel times, then every solve of the
@@ -410,18 +492,22 @@ bool ns3d_engine::process_round(int buf, bool restart[kGroups]) {
     if (hi <= lo) continue;
     const int vb = o.group / kLanes ? 4 : 8;
     double per = 0;
+    const bool at_qt = lo <= o.it0 + qt && o.it0 + qt < hi;
     for (int k = 0; k < 3; ++k) {
       const double kb = kernel_bytes_eng(o.n, o.nnz, k, vb);
       per += kb;
-      if (lo <= o.it0 + qt && o.it0 + qt < hi) sb[o.group][k] += kb;
+      if (at_qt) sb[o.group][k] += kb;
     }
+    if (at_qt && round_sampled[buf]) tiles_live += o.tiles;
     cb += per * (double)(hi - lo);
   }
   const bool* on = round_on[buf];
+  if (round_sampled[buf])
+    for (const RoundEntry& o : ord) tiles_all += o.tiles;
   hipEvent_t ref = nullptr;
   double lo_t = 0, hi_t = 0;
   bool span_ok = chunk >= 3;
-  for (int g = 0; g < kGroups; ++g) {
+  for (int g = 0; g < kGroups && round_sampled[buf]; ++g) {
     if (!on[g]) continue;
     hipEvent_t* e = tev[buf] + kEv * g;
     for (int k = 0; k < 3; ++k) {
@@ -442,7 +528,7 @@ bool ns3d_engine::process_round(int buf, bool restart[kGroups]) {
     lo_t = std::min(lo_t, (double)t0);
     hi_t = std::max(hi_t, (double)t1);
   }
-  if (span_ok && ref && hi_t > lo_t && cb > 0) {
+  if (round_sampled[buf] && span_ok && ref && hi_t > lo_t && cb > 0) {
     ch_ms += hi_t - lo_t;
     ch_n += 1;
     ch_bytes += cb;

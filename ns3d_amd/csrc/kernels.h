@@ -150,6 +150,10 @@ template <typename T>
 struct Batch {
   const SolveDesc<T>* d;
   const TileDesc<T>* tiles;
+  // when set, the row kernels read the tile count from here (a table whose
+  // size changes between launches of a reused graph) and the grid may have
+  // more blocks than tiles
+  const int* ntiles_dev;
   int nsolves, ntiles;
   int q;
   int single;
@@ -164,7 +168,7 @@ Batch<T> single_batch(const SolveDesc<T>& d, long long it0, int q);
 // a multiple of kBlock rows, at least kTileRows rows each where n allows.
 constexpr int kMaxTilesPerSolve = 1024;
 #ifndef NS3D_TILE_ROWS
-#define NS3D_TILE_ROWS 1536
+#define NS3D_TILE_ROWS 1024
 #endif
 constexpr int kTileRows = NS3D_TILE_ROWS;
 int tiles_for(int64_t n);
@@ -188,6 +192,11 @@ template <typename T>
 hipError_t launch_bicg_B(Batch<T> B, hipStream_t s, hipEvent_t* ev = nullptr);
 template <typename T>
 hipError_t launch_bicg_C(Batch<T> B, hipStream_t s, hipEvent_t* ev = nullptr);
+// Adds the row kernel k (0 A, 1 B, 2 C) over B with `grid` blocks to graph
+// g, after `dep` (may be null).
+template <typename T>
+hipError_t graph_add_bicg(hipGraph_t g, int k, const Batch<T>& B, int grid, hipGraphNode_t dep,
+                          hipGraphNode_t* out);
 hipError_t launch_invdiag(SysView S, hipStream_t s);
 // Packs nbr[6][n] into pk[n]; *bad (zeroed by the caller) becomes nonzero if
 // a distance does not fit or a neighbour is not on its expected side.

@@ -169,6 +169,18 @@ __device__ __forceinline__ int block_tile(int ntiles) {
   const int t = logical_block();
   return t < ntiles ? t : -1;
 }
+// The same with the tile count read from device memory when the batch says
+// so: each XCD takes one contiguous eighth of the tiles, whatever the grid.
+template <typename T>
+__device__ __forceinline__ int block_tile(const Batch<T>& B) {
+  if (!B.ntiles_dev) return block_tile(B.ntiles);
+  const int nt = *B.ntiles_dev;
+  const int per = (nt + kXcd - 1) / kXcd;
+  const int i = (int)blockIdx.x / kXcd;
+  if (i >= per) return -1;
+  const int t = ((int)blockIdx.x % kXcd) * per + i;
+  return t < nt ? t : -1;
+}
 
 // Pointers read from a table are generic pointers to the compiler, which
 // then emits flat loads and stores (slower, and more registers).  The
@@ -395,7 +407,7 @@ __device__ __forceinline__ bool leader_of(int t, int t0) { return t == t0 && thr
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 bicg_A_kernel(Batch<T> B) {
-  const int t = block_tile(B.ntiles);
+  const int t = block_tile(B);
   if (t < 0) return;
   Tile tl;
   GDesc<T> D;
@@ -480,7 +492,7 @@ bicg_A_kernel(Batch<T> B) {
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 bicg_B_kernel(Batch<T> B) {
-  const int t = block_tile(B.ntiles);
+  const int t = block_tile(B);
   if (t < 0) return;
   Tile tl;
   GDesc<T> D;
@@ -534,7 +546,7 @@ bicg_B_kernel(Batch<T> B) {
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 bicg_C_kernel(Batch<T> B) {
-  const int t = block_tile(B.ntiles);
+  const int t = block_tile(B);
   if (t < 0) return;
   Tile tl;
   GDesc<T> D;
@@ -747,7 +759,22 @@ hipError_t launch_bicg_C(Batch<T> B, hipStream_t s, hipEvent_t* ev) {
   return launch_timed(bicg_C_kernel<T>, tile_grid(B.ntiles), s, ev, B);
 }
 
+template <typename T>
+hipError_t graph_add_bicg(hipGraph_t g, int k, const Batch<T>& B, int grid, hipGraphNode_t dep,
+                          hipGraphNode_t* out) {
+  hipKernelNodeParams prm{};
+  prm.func = k == 0 ? (void*)bicg_A_kernel<T> : k == 1 ? (void*)bicg_B_kernel<T> : (void*)bicg_C_kernel<T>;
+  prm.gridDim = dim3(grid);
+  prm.blockDim = dim3(kBlock);
+  prm.sharedMemBytes = 0;
+  void* args[1] = {(void*)&B};
+  prm.kernelParams = args;
+  prm.extra = nullptr;
+  return hipGraphAddKernelNode(out, g, dep ? &dep : nullptr, dep ? 1 : 0, &prm);
+}
+
 #define NS3D_INSTANTIATE(T)                                                         \
+  template hipError_t graph_add_bicg<T>(hipGraph_t, int, const Batch<T>&, int, hipGraphNode_t, hipGraphNode_t*); \
   template Batch<T> single_batch<T>(const SolveDesc<T>&, long long, int);          \
   template hipError_t launch_bicg_init<T>(Batch<T>, hipStream_t);                  \
   template hipError_t launch_bicg_restart<T>(Batch<T>, hipStream_t);               \

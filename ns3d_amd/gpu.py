@@ -35,6 +35,7 @@ EXPORTS = [
     "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_set_precision",
     "ns3d_set_mixed_min_rows", "ns3d_sync",
     "ns3d_engine_create", "ns3d_engine_destroy", "ns3d_attach_engine", "ns3d_engine_stats",
+    "ns3d_engine_set_graphs",
     "ns3d_footprint", "ns3d_ctx_bytes",
 ]
 
@@ -81,6 +82,7 @@ def lib():
         L.ns3d_engine_destroy.argtypes = [vp]
         L.ns3d_attach_engine.argtypes = [vp, vp]
         L.ns3d_engine_stats.argtypes = [vp, P(dbl)]
+        L.ns3d_engine_set_graphs.argtypes = [vp, i32]
         L.ns3d_footprint.argtypes = [i32, i32, i32, i64, i32, i32, P(dbl)]
         L.ns3d_ctx_bytes.argtypes = [vp, P(dbl)]
         L.ns3d_prepare_image.argtypes = [P(C.c_uint8), i32, i32, i32, i32, i32, P(C.c_uint8), P(i32)]
@@ -313,10 +315,11 @@ class Engine:
 
     STAT_NAMES = ["chunks", "launches", "solves", "max_active", "rows_x_iters"]
 
-    def __init__(self, device: int = 0, chunk: int = 8):
+    def __init__(self, device: int = 0, chunk: int = 8, graphs: bool = True):
         h = C.c_void_p()
         _check(lib().ns3d_engine_create(device, chunk, C.byref(h)))
         self.h = h
+        _check(lib().ns3d_engine_set_graphs(self.h, int(bool(graphs))))
 
     def attach(self, ctx):
         _check(lib().ns3d_attach_engine(ctx, self.h))
@@ -339,6 +342,7 @@ class Engine:
                 d[f"{prec}_{name}"] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
         d["chunk"] = (float(out[23]), float(out[24]), float(out[25]))
         d["host_ms"] = dict(queue=float(out[26]), process=float(out[27]), wait=float(out[28]))
+        d["sampled_tiles"] = dict(live=float(out[29]), launched=float(out[30]))
         return d
 
     def close(self):

@@ -302,16 +302,18 @@ class Server:
             pass
 
 
-def solve_batch(images, params, engine=None, chunk=8):
+def solve_batch(images, params, engine=None, chunk=8, graphs=True):
     """Solves several images concurrently: one thread per image runs its
-    driver, their BiCGSTAB solves batched on one engine.  params: one dict
-    per image (or one dict for all).  Returns [(rows, summary)] in order;
+    driver, their BiCGSTAB solves batched on one engine (created here unless
+    given; graphs: its chunks submitted as HIP graphs).  params: one dict per
+    image (or one dict for all).  Returns [(rows, summary)] in order;
     summaries carry the final fields."""
     import threading
     if isinstance(params, dict):
         params = [params] * len(images)
     own = engine is None
-    eng = _gpu.Engine(device=params[0].get("device", 0) if params else 0, chunk=chunk) if own else engine
+    eng = _gpu.Engine(device=params[0].get("device", 0) if params else 0, chunk=chunk, graphs=graphs) \
+        if own else engine
     solvers = [Solver(img, **p) for img, p in zip(images, params)]
     out = [None] * len(solvers)
     errs = []

@@ -52,6 +52,22 @@ def test_batch_results_equal_solo_bit_for_bit():
     assert st["solves"] > 0 and st["max_active"] >= 2, st
 
 
+def test_graph_rounds_equal_direct_rounds():
+    """Chunks submitted as HIP graphs -- each graph replayed round after
+    round, the tables rewritten and the host's own work in between -- give
+    the same bits as the same chunks launched kernel by kernel."""
+    imgs, params = mix()
+    direct = solver.solve_batch(imgs, params, chunk=4, graphs=False)
+    graph = solver.solve_batch(imgs, params, chunk=4, graphs=True)
+    for i, ((r0, s0), (r1, s1)) in enumerate(zip(direct, graph)):
+        assert s0["K"] == s1["K"], (i, s0["K"], s1["K"])
+        assert s0["solver_iterations"] == s1["solver_iterations"], i
+        assert np.array_equal(r0, r1), i
+        for f in "puvw":
+            assert np.array_equal(s0["fields"][f], s1["fields"][f]), (i, f)
+    assert direct[0][1]["engine"]["chunks"] > 0 and graph[0][1]["engine"]["chunks"] > 0
+
+
 def test_engine_lifecycle_and_detach():
     """A context detached from an engine solves on its own stream again; an
     engine with nothing attached shuts down cleanly; bad arguments fail."""
