@@ -73,6 +73,8 @@ def served_pass(reqs, args, precision, label, arrivals=None):
     from ns3d_amd import solver
     srv = solver.Server(device=0, budget_bytes=args.budget_gb * 1e9, max_slots=args.slots, chunk=args.chunk,
                         fallback=solver.FALLBACK_FAILED)
+    if getattr(args, "no_graphs", False):
+        srv.set_graphs(False)
     try:
         t0 = time.perf_counter()
         tickets = []
@@ -128,10 +130,11 @@ def dk_gate(reqs, args, mixed, plain, tol=1e-4):
     on every RMR spike; near the residual's noise floor both are decided by
     rounding, so the stopped K of the FP64 path itself moves by up to ~1e-3
     under a perturbation of the rounding alone.  For a request over tol the
-    FP64 path is therefore run on the image mirrored in y and in z (the same
-    problem, other summation orders): its spread there is the FP64 path's
-    own rounding spread, and the mixed request passes if it stays within
-    it."""
+    FP64 path is therefore run on the image mirrored in y, in z and in both
+    (the same problem, other summation orders): the largest of those moves is
+    the FP64 path's own rounding spread, and the mixed request passes if it
+    stays within twice that (the mixed result is one more draw of the same
+    kind of perturbation, compared with the largest of three)."""
     from ns3d_amd import solver
     out = dict(tolerance=tol, requests=0, within_tolerance=0, within_fp64_rounding_spread=0, over=[])
     for q, ((img, _), m, p) in enumerate(zip(reqs, mixed, plain)):
@@ -143,14 +146,14 @@ def dk_gate(reqs, args, mixed, plain, tol=1e-4):
             out["within_tolerance"] += 1
             continue
         spread = 0.0
-        for v in (img[:, ::-1, :], img[:, :, ::-1]):
+        for v in (img[:, ::-1, :], img[:, :, ::-1], img[:, ::-1, ::-1]):
             v = np.ascontiguousarray(v)
             sv = solver.Solver(v, **mix_params(v, args, 0))
             r = sv.run()[1]
             sv.close()
             spread = max(spread, abs(r["K"] / p["K"] - 1))
         log(f"  request {q}: mixed |dK|/K {d:.2e}; FP64 on mirrored copies {spread:.2e}")
-        out["within_fp64_rounding_spread"] += int(d <= spread)
+        out["within_fp64_rounding_spread"] += int(d <= 2 * spread)
         out["over"].append(dict(request=q, dK_over_K=d, fp64_mirror_spread=spread))
     out["passed"] = out["within_tolerance"] + out["within_fp64_rounding_spread"] == out["requests"]
     return out
@@ -175,9 +178,15 @@ def mix_section(args):
     fluid = sum(d["fluid"] for _, d in reqs)
     forms_run = set(args.mix_forms.split(","))
     if forms_run != {"plain", "served64", "served", "poisson"}:
-        # a partial run (profiling): only the served mixed pass(es)
+        # a partial run (profiling): only the served mixed pass(es), with the
+        # engine's own sampled kernel figures for comparison
         for q in range(args.steps):
-            served_pass(reqs, args, 1, f"served_mixed step {q}")
+            res = served_pass(reqs, args, 1, f"served_mixed step {q}")
+            for prec in ("f32", "f64"):
+                for k in "ABC":
+                    r = engine_roofline(res["engine"], prec, k)
+                    if r:
+                        log(f"  engine bicg_{k}_kernel<{'float' if prec == 'f32' else 'double'}>: {r}")
         return None
     for w in range(args.warmup):
         log(f"warmup pass {w} (served mixed)")
@@ -325,7 +334,8 @@ def mix_traffic(args):
     if not os.path.exists(path):
         return None, None
     d = json.load(open(path))
-    if (d.get("requests"), d.get("mix_seed"), d.get("edges")) != (args.requests, args.mix_seed, args.edges):
+    if (d.get("requests"), d.get("mix_seed"), d.get("edges"), d.get("mix_maxit")) != \
+            (args.requests, args.mix_seed, args.edges, args.mix_maxit):
         return None, None
     k = d["kernels"].get("bicg_A_kernel<float>")
     if not k:
@@ -440,6 +450,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events (profiler runs)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="served forms: launch the engine's chunks kernel by kernel, not as HIP graphs")
     ap.add_argument("--progress", action="store_true", help="one stderr line per outer iteration (single)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scales HBM byte counters under a profiler)")

@@ -18,7 +18,10 @@
 
 namespace ns3d {
 
-constexpr int kBlock = 256;   // threads per block for row kernels (4 wave64)
+#ifndef NS3D_BLOCK
+#define NS3D_BLOCK 256
+#endif
+constexpr int kBlock = NS3D_BLOCK;  // threads per block for row kernels (wave64s)
 constexpr int kWaves = kBlock / 64;
 constexpr int kXcd = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
 

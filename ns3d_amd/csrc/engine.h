@@ -100,6 +100,11 @@ struct ns3d_engine {
 #define NS3D_ENGINE_LANES 2
 #endif
   static constexpr int kLanes = NS3D_ENGINE_LANES;
+  // with three or more lanes: solves of fewer rows go to the last lane
+#ifndef NS3D_SMALL_ROWS
+#define NS3D_SMALL_ROWS 0
+#endif
+  static constexpr int64_t kSmallRows = NS3D_SMALL_ROWS;
   static constexpr int kGroups = 2 * kLanes;
   void run();
   void admit(std::vector<ns3d::EngineRequest*>& reqs, int buf);

@@ -237,9 +237,15 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
   for (int g = 0; g < kGroups; ++g) rows[g] = group_rows(g);
   for (EngineRequest* r : reqs) {
     const int g0 = kLanes * r->prec;
-    r->group = g0;
-    for (int l = 1; l < kLanes; ++l)
-      if (rows[g0 + l] < rows[r->group]) r->group = g0 + l;
+    if (kSmallRows > 0 && kLanes > 2 && r->n < kSmallRows) {
+      // small solves iterate on a lane of their own, at their own pace
+      r->group = g0 + kLanes - 1;
+    } else {
+      const int big = kSmallRows > 0 && kLanes > 2 ? kLanes - 1 : kLanes;
+      r->group = g0;
+      for (int l = 1; l < big; ++l)
+        if (rows[g0 + l] < rows[r->group]) r->group = g0 + l;
+    }
     rows[r->group] += r->n;
   }
   for (int g = 0; g < kGroups; ++g) {

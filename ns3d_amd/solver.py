@@ -290,6 +290,11 @@ class Server:
                                                  out.ctypes.data_as(C.POINTER(C.c_double))))
         return _gpu.Engine.unpack_stats(out)
 
+    def set_graphs(self, on: bool):
+        """Chunks of the server's engine as HIP graphs (default) or launched
+        kernel by kernel (see gpu.Engine)."""
+        _gpu._check(_gpu.lib().ns3d_engine_set_graphs(C.c_void_p(lib().ns3d_server_engine(self.h)), int(bool(on))))
+
     def close(self):
         if getattr(self, "h", None):
             lib().ns3d_server_destroy(self.h)
