@@ -73,8 +73,7 @@ def served_pass(reqs, args, precision, label, arrivals=None):
     from ns3d_amd import solver
     srv = solver.Server(device=0, budget_bytes=args.budget_gb * 1e9, max_slots=args.slots, chunk=args.chunk,
                         fallback=solver.FALLBACK_FAILED)
-    if getattr(args, "no_graphs", False):
-        srv.set_graphs(False)
+    srv.set_schedule(args.chunk, args.lanes, not getattr(args, "no_graphs", False))
     try:
         t0 = time.perf_counter()
         tickets = []
@@ -188,6 +187,20 @@ def mix_section(args):
                     if r:
                         log(f"  engine bicg_{k}_kernel<{'float' if prec == 'f32' else 'double'}>: {r}")
         return None
+    tuned = None
+    if args.autotune:
+        from ns3d_amd import autotune
+        log(f"autotuning the engine schedule on the mix with each request capped at {args.tune_maxit} outer "
+            f"iterations")
+        sample = []
+        for img, _ in reqs:
+            p = mix_params(img, args, 1)
+            p["maxitSIMPLE"] = args.tune_maxit
+            sample.append((img, p))
+        tuned = autotune.tune(sample, candidates=[(c, l, True) for c in (6, 8, 12) for l in (1, 2, 3)] + [(8, 2, False)],
+                              log=log, budget_gb=args.budget_gb, slots=args.slots)
+        args.chunk, args.lanes, args.no_graphs = tuned["chunk"], tuned["lanes"], not tuned["graphs"]
+        log(f"  chosen: chunk {args.chunk}, lanes {args.lanes}, graphs {int(not args.no_graphs)}")
     for w in range(args.warmup):
         log(f"warmup pass {w} (served mixed)")
         served_pass(reqs, args, 1, "warmup")
@@ -245,10 +258,12 @@ def mix_section(args):
     desc = dict(workload=f"request mix: {n} seeded images (edges {args.edges} per axis, porosity U[0.35,0.90], "
                          f"RSA / Boolean / fibres), SIMPLEC, flow x, simpletol {args.simpletol:g}, at most "
                          f"{args.mix_maxit} outer iterations",
-                requests=n, fluid_cells=int(fluid), mix_seed=args.mix_seed, slots=args.slots, chunk=args.chunk,
+                requests=n, fluid_cells=int(fluid), mix_seed=args.mix_seed, slots=args.slots,
+                schedule=dict(chunk=args.chunk, lanes=args.lanes, graphs=not args.no_graphs,
+                              autotuned=tuned is not None), chunk=args.chunk,
                 images=[dict(shape=d["shape"], generator=d["generator"], porosity=d["porosity"]) for _, d in reqs])
     return dict(forms=forms, desc=desc, reqs=reqs, plain=plain, mixed_wall=mixed_wall, lat=lat_m,
-                max_dk=max(dk_mixed), gate=gate, engine=smx[-1]["engine"])
+                max_dk=max(dk_mixed), gate=gate, engine=smx[-1]["engine"], autotune=tuned)
 
 
 # ---------------------------------------------------------------- single image
@@ -452,6 +467,10 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events (profiler runs)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="served forms: launch the engine's chunks kernel by kernel, not as HIP graphs")
+    ap.add_argument("--lanes", type=int, default=2, help="served forms: engine streams")
+    ap.add_argument("--tune-maxit", type=int, default=60, help="autotuner: outer-iteration cap of its trials")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
+                    help="pick chunk / lanes / graphs with ns3d_amd.autotune before the mix (full runs)")
     ap.add_argument("--progress", action="store_true", help="one stderr line per outer iteration (single)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (scales HBM byte counters under a profiler)")
@@ -491,6 +510,7 @@ def main():
             "speedup_vs_plain_fp64": round(value / f["plain_fp64"]["solves_per_s"], 3),
             "max_dK_over_K_vs_fp64": mixres["max_dk"],
             "dK_gate": mixres["gate"],
+            "autotune": mixres["autotune"],
             "forms": f,
         }
     else:

@@ -255,6 +255,14 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
  * kernel by kernel with dispatch events for ns3d_engine_stats; off: every
  * round kernel by kernel.  Results are the same either way. */
 ns3d_status ns3d_engine_set_graphs(ns3d_engine* eng, int on);
+/* The engine's schedule: chunk (iterations queued per round, 1..64), lanes
+ * (streams the solves are spread over, 1..4; default 2) and graphs (as
+ * ns3d_engine_set_graphs).  Only while no solve is pending or running
+ * (NS3D_ERR_INVALID otherwise).  No choice changes any result; an autotuner
+ * (ns3d_amd/autotune.py) picks them by timing a request sample.  get:
+ * out = {chunk, lanes, graphs}. */
+ns3d_status ns3d_engine_set_schedule(ns3d_engine* eng, int chunk, int lanes, int graphs);
+ns3d_status ns3d_engine_get_schedule(ns3d_engine* eng, int out[3]);
 /* out[32]: chunks, launches, solves done, max active solves, sum of rows x
  * iterations; then per precision q (0 FP64, 1 FP32) and kernel k (A, B, C),
  * out[5 + 9q + 3k] = sampled milliseconds, sampled launches, algorithmic

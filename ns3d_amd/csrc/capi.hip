@@ -1115,6 +1115,24 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* c, ns3d_engine* e) {
   return NS3D_OK;
 }
 
+ns3d_status ns3d_engine_set_schedule(ns3d_engine* e, int chunk, int lanes, int graphs) {
+  if (!e) return fail_invalid("ns3d_engine_set_schedule: null engine");
+  if (chunk < 1 || chunk > 64 || lanes < 1 || lanes > ns3d_engine::kMaxLanes)
+    return fail_invalid("ns3d_engine_set_schedule: chunk must be in [1, 64] and lanes in [1, 4]");
+  return guarded([&] {
+    if (!e->set_schedule(chunk, lanes, graphs != 0))
+      throw Invalid{"ns3d_engine_set_schedule: the engine has solves pending or running"};
+  });
+}
+
+ns3d_status ns3d_engine_get_schedule(ns3d_engine* e, int out[3]) {
+  if (!e || !out) return fail_invalid("ns3d_engine_get_schedule: bad arguments");
+  out[0] = e->chunk;
+  out[1] = e->nlanes;
+  out[2] = e->graphs ? 1 : 0;
+  return NS3D_OK;
+}
+
 ns3d_status ns3d_engine_set_graphs(ns3d_engine* e, int on) {
   if (!e) return fail_invalid("ns3d_engine_set_graphs: null engine");
   e->set_graphs(on != 0);

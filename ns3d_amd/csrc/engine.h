@@ -39,7 +39,7 @@ struct EngineRequest {
   // next round is queued; iteration base of the last queued round and
   // whether that round restarted this solve's group first
   long long serial = 0;
-  int group = 0;  // kLanes * prec + lane
+  int group = 0;  // kMaxLanes * prec + lane
   int tiles = 0;  // tiles in the current table
   long long it_next = 0;
   long long it_done = 0;  // executed iterations as last read back
@@ -64,13 +64,20 @@ struct ns3d_engine {
   // launches go one by one with dispatch events for the statistics; off:
   // every round is launched one by one.
   void set_graphs(bool on);
+  // Chunk length, lane count (1..kMaxLanes) and graph use; only while no
+  // solve is pending or running.  Returns false (nothing changed) otherwise.
+  // None of these changes any result.
+  bool set_schedule(int chunk, int lanes, bool graphs);
   // Returns once every round queued before the call has been processed, so
   // no queued kernel refers to a solve that has already returned.  A context
   // calls it before freeing memory its solves used.
   void fence();
 
+  static constexpr int kMaxLanes = 4;
   int device = 0;
   int chunk = 8;
+  int nlanes = 2;  // streams in use (1..kMaxLanes)
+  std::atomic<bool> graphs{true};
 
  private:
   struct Group {
@@ -92,20 +99,12 @@ struct ns3d_engine {
     int ntiles = 0;
     bool dirty = true;
   };
-  // Solves are split into kGroups groups: group g holds precision g / kLanes
-  // (0 FP64, 1 FP32) on lane g % kLanes.  Each lane has its own stream, so the
-  // launches of the two lanes of a precision run concurrently and one's
-  // ramp-up and drain overlap the other's work.
-#ifndef NS3D_ENGINE_LANES
-#define NS3D_ENGINE_LANES 2
-#endif
-  static constexpr int kLanes = NS3D_ENGINE_LANES;
-  // with three or more lanes: solves of fewer rows go to the last lane
-#ifndef NS3D_SMALL_ROWS
-#define NS3D_SMALL_ROWS 0
-#endif
-  static constexpr int64_t kSmallRows = NS3D_SMALL_ROWS;
-  static constexpr int kGroups = 2 * kLanes;
+  // Solves are split into groups: group g holds precision g / kMaxLanes
+  // (0 FP64, 1 FP32) on lane g % kMaxLanes, for the first `nlanes` lanes.
+  // Each lane has its own stream, so the launches of the lanes of a
+  // precision run concurrently and one's ramp-up and drain overlap the
+  // others' work.
+  static constexpr int kGroups = 2 * kMaxLanes;
   void run();
   void admit(std::vector<ns3d::EngineRequest*>& reqs, int buf);
   void rebuild(int g, int buf);
@@ -127,11 +126,10 @@ struct ns3d_engine {
   int64_t group_rows(int g) const;
 
   static constexpr int kSampleEvery = 8;
-  std::atomic<bool> graphs{true};
   long long rounds_started = 0;
   bool round_sampled[2] = {};
-  std::unordered_map<uint64_t, hipGraphExec_t> gcache[kLanes];
-  hipStream_t lane_stream[kLanes] = {};
+  std::unordered_map<uint64_t, hipGraphExec_t> gcache[kMaxLanes];
+  hipStream_t lane_stream[kMaxLanes] = {};
   hipStream_t stream = nullptr;  // lane 0's: gathers and completion events
   std::thread thr;
   std::mutex mu;
@@ -160,7 +158,7 @@ struct ns3d_engine {
   long long next_serial = 1;
   bool round_on[2][kGroups] = {};      // groups with launches in the round
   hipEvent_t round_done[2] = {};
-  hipEvent_t lane_join[2][kLanes] = {};  // lane l's chunk done (lane 0 waits before the gather)
+  hipEvent_t lane_join[2][kMaxLanes] = {};  // lane l's chunk done (lane 0 waits before the gather)
   // dispatch events per group: [A, B, C of the sampled iteration][start,
   // stop], then the chunk's first A and last C [start, stop]
   static constexpr int kEv = 10;

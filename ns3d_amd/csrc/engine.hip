@@ -77,7 +77,9 @@ SolveDesc<float>& desc_of<float>(EngineRequest* r) { return r->d32; }
 
 ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) {
   ECHK(hipSetDevice(device));
-  for (auto& st : lane_stream) ECHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  // streams of the lanes in use only (each takes one of the process's HIP
+  // hardware queues); more are created if the schedule asks for them
+  for (int l = 0; l < nlanes; ++l) ECHK(hipStreamCreateWithFlags(&lane_stream[l], hipStreamNonBlocking));
   stream = lane_stream[0];
   auto dalloc = [&](size_t bytes) {
     void* p = nullptr;
@@ -136,6 +138,26 @@ ns3d_engine::~ns3d_engine() {
 }
 
 void ns3d_engine::set_graphs(bool on) { graphs = on; }
+
+bool ns3d_engine::set_schedule(int ch, int lanes, bool use_graphs) {
+  if (ch < 1 || ch > 64 || lanes < 1 || lanes > kMaxLanes) throw std::runtime_error("set_schedule: bad arguments");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!pending.empty() || any_active() || rounds_done != rounds_queued) return false;
+  ECHK(hipSetDevice(device));
+  for (int l = nlanes; l < lanes; ++l)
+    if (!lane_stream[l]) ECHK(hipStreamCreateWithFlags(&lane_stream[l], hipStreamNonBlocking));
+  if (ch != chunk) {
+    for (int l = 0; l < kMaxLanes; ++l) {
+      if (lane_stream[l]) ECHK(hipStreamSynchronize(lane_stream[l]));
+      for (auto& kv : gcache[l]) ECHK(hipGraphExecDestroy(kv.second));
+      gcache[l].clear();
+    }
+  }
+  chunk = ch;
+  nlanes = lanes;
+  graphs = use_graphs;
+  return true;
+}
 
 void ns3d_engine::solve(EngineRequest& req) {
   EngineRequest* r = &req;
@@ -236,21 +258,15 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
   int64_t rows[kGroups];
   for (int g = 0; g < kGroups; ++g) rows[g] = group_rows(g);
   for (EngineRequest* r : reqs) {
-    const int g0 = kLanes * r->prec;
-    if (kSmallRows > 0 && kLanes > 2 && r->n < kSmallRows) {
-      // small solves iterate on a lane of their own, at their own pace
-      r->group = g0 + kLanes - 1;
-    } else {
-      const int big = kSmallRows > 0 && kLanes > 2 ? kLanes - 1 : kLanes;
-      r->group = g0;
-      for (int l = 1; l < big; ++l)
-        if (rows[g0 + l] < rows[r->group]) r->group = g0 + l;
-    }
+    const int g0 = kMaxLanes * r->prec;
+    r->group = g0;
+    for (int l = 1; l < nlanes; ++l)
+      if (rows[g0 + l] < rows[r->group]) r->group = g0 + l;
     rows[r->group] += r->n;
   }
   for (int g = 0; g < kGroups; ++g) {
     Group& G = grp[g];
-    hipStream_t gs = lane_stream[g % kLanes];
+    hipStream_t gs = lane_stream[g % kMaxLanes];
     std::vector<Tile> tiles;
     int ns = 0;
     for (EngineRequest* r : reqs)
@@ -276,7 +292,7 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
       B.ntiles = (int)tiles.size();
       ECHK(launch_bicg_init<T>(B, gs));
     };
-    if (g / kLanes == 0) init(double{});
+    if (g / kMaxLanes == 0) init(double{});
     else init(float{});
     for (EngineRequest* r : reqs)
       if (r->group == g) {
@@ -307,10 +323,10 @@ void ns3d_engine::upload_tiles(int g, int buf, bool restarted) {
     using T = decltype(tag);
     tile_table(G.h_tiles[buf], G.tiles, (const SolveDesc<T>*)G.descs.data(), &G.active);
   };
-  if (g / kLanes == 0) up(double{});
+  if (g / kMaxLanes == 0) up(double{});
   else up(float{});
   ECHK(hipMemcpyAsync(G.d_nt, G.h_tiles[buf].data(), G.h_tiles[buf].size(), hipMemcpyHostToDevice,
-                      lane_stream[g % kLanes]));
+                      lane_stream[g % kMaxLanes]));
 }
 
 // Tiles and descriptors of the group's active solves (after it changed).
@@ -332,9 +348,9 @@ void ns3d_engine::rebuild(int g, int buf) {
     SolveDesc<T>* hd = (SolveDesc<T>*)G.h_desc[buf].data();
     std::copy(cur, cur + G.active.size(), hd);
     ECHK(hipMemcpyAsync(G.d_desc, hd, G.active.size() * sizeof(SolveDesc<T>), hipMemcpyHostToDevice,
-                        lane_stream[g % kLanes]));
+                        lane_stream[g % kMaxLanes]));
   };
-  if (g / kLanes == 0) build(double{});
+  if (g / kMaxLanes == 0) build(double{});
   else build(float{});
   G.dirty = false;
 }
@@ -344,15 +360,15 @@ void ns3d_engine::rebuild(int g, int buf) {
 void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
   // the other lanes start this round after the previous round's gather (on
   // lane 0), so that gather never reads a state this round is writing
-  for (int l = 1; l < kLanes; ++l) ECHK(hipStreamWaitEvent(lane_stream[l], round_done[buf ^ 1], 0));
+  for (int l = 1; l < nlanes; ++l) ECHK(hipStreamWaitEvent(lane_stream[l], round_done[buf ^ 1], 0));
   for (int g = 0; g < kGroups; ++g) {
     if (grp[g].dirty) rebuild(g, buf);
     // the restart kernel acts only on solves whose status asks for it
     const bool rs = restart[g] && !grp[g].active.empty();
     upload_tiles(g, buf, rs);
     if (rs) {
-      hipStream_t gs = lane_stream[g % kLanes];
-      if (g / kLanes == 0) ECHK(launch_bicg_restart<double>(batch<double>(g, 0), gs));
+      hipStream_t gs = lane_stream[g % kMaxLanes];
+      if (g / kMaxLanes == 0) ECHK(launch_bicg_restart<double>(batch<double>(g, 0), gs));
       else ECHK(launch_bicg_restart<float>(batch<float>(g, 0), gs));
     }
     restart[g] = false;
@@ -376,17 +392,17 @@ void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
   if (sampled) {
     launch_direct(buf, on);
   } else {
-    for (int l = 0; l < kLanes; ++l) {
+    for (int l = 0; l < nlanes; ++l) {
       int grid[2] = {0, 0};
       for (int p = 0; p < 2; ++p)
-        if (on[p * kLanes + l]) grid[p] = graph_grid(grp[p * kLanes + l].ntiles);
+        if (on[p * kMaxLanes + l]) grid[p] = graph_grid(grp[p * kMaxLanes + l].ntiles);
       if (grid[0] || grid[1]) ECHK(hipGraphLaunch(chunk_graph(l, grid), lane_stream[l]));
     }
   }
   n_launch += (double)chunk * 3 * ngroups;
   n_chunks += 1;
   // join: lane 0 waits for lane 1's chunk before the gather
-  for (int l = 1; l < kLanes; ++l) {
+  for (int l = 1; l < nlanes; ++l) {
     ECHK(hipEventRecord(lane_join[buf][l], lane_stream[l]));
     ECHK(hipStreamWaitEvent(stream, lane_join[buf][l], 0));
   }
@@ -409,12 +425,12 @@ void ns3d_engine::launch_direct(int buf, const bool* on) {
     for (int k = 0; k < 3; ++k)
       for (int g = 0; g < kGroups; ++g) {
         if (!on[g]) continue;
-        hipStream_t gs = lane_stream[g % kLanes];
+        hipStream_t gs = lane_stream[g % kMaxLanes];
         hipEvent_t* ev = nullptr;
         if (q == qt) ev = tev[buf] + kEv * g + 2 * k;
         else if (span && q == 0 && k == 0) ev = tev[buf] + kEv * g + 6;
         else if (span && q == chunk - 1 && k == 2) ev = tev[buf] + kEv * g + 8;
-        if (g / kLanes == 0) {
+        if (g / kMaxLanes == 0) {
           const Batch<double> B = batch<double>(g, q);
           ECHK(k == 0 ? launch_bicg_A<double>(B, gs, ev)
                       : k == 1 ? launch_bicg_B<double>(B, gs, ev) : launch_bicg_C<double>(B, gs, ev));
@@ -447,7 +463,7 @@ hipGraphExec_t ns3d_engine::chunk_graph(int lane, const int grid[2]) {
     for (int k = 0; k < 3; ++k)
       for (int p = 0; p < 2; ++p) {
         if (!grid[p]) continue;
-        const int g = p * kLanes + lane;
+        const int g = p * kMaxLanes + lane;
         hipGraphNode_t nd = nullptr;
         if (p == 0) {
           Batch<double> B = batch<double>(g, q);
@@ -496,7 +512,7 @@ bool ns3d_engine::process_round(int buf, bool restart[kGroups]) {
     const long long hi = std::min<long long>(h_states[buf][i].it_c, o.it0 + chunk);
     r->it_done = h_states[buf][i].it_c;
     if (hi <= lo) continue;
-    const int vb = o.group / kLanes ? 4 : 8;
+    const int vb = o.group / kMaxLanes ? 4 : 8;
     double per = 0;
     const bool at_qt = lo <= o.it0 + qt && o.it0 + qt < hi;
     for (int k = 0; k < 3; ++k) {
@@ -519,9 +535,9 @@ bool ns3d_engine::process_round(int buf, bool restart[kGroups]) {
     for (int k = 0; k < 3; ++k) {
       float d = 0.f;
       if (sb[g][k] <= 0 || hipEventElapsedTime(&d, e[2 * k], e[2 * k + 1]) != hipSuccess) continue;
-      ms[g / kLanes][k] += d;
-      launches[g / kLanes][k] += 1;
-      bytes[g / kLanes][k] += sb[g][k];
+      ms[g / kMaxLanes][k] += d;
+      launches[g / kMaxLanes][k] += 1;
+      bytes[g / kMaxLanes][k] += sb[g][k];
     }
     if (!span_ok) continue;
     float t0 = 0.f, t1 = 0.f;
@@ -595,8 +611,8 @@ void ns3d_engine::run() {
         // room per precision (both lanes together; each lane gets at most that)
         int tiles[2] = {0, 0}, solves[2] = {0, 0};
         for (int g = 0; g < kGroups; ++g) {
-          tiles[g / kLanes] += grp[g].ntiles;
-          solves[g / kLanes] += (int)grp[g].active.size();
+          tiles[g / kMaxLanes] += grp[g].ntiles;
+          solves[g / kMaxLanes] += (int)grp[g].active.size();
         }
         while (!pending.empty()) {
           EngineRequest* r = pending.front();

@@ -35,7 +35,7 @@ EXPORTS = [
     "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_set_precision",
     "ns3d_set_mixed_min_rows", "ns3d_sync",
     "ns3d_engine_create", "ns3d_engine_destroy", "ns3d_attach_engine", "ns3d_engine_stats",
-    "ns3d_engine_set_graphs",
+    "ns3d_engine_set_graphs", "ns3d_engine_set_schedule", "ns3d_engine_get_schedule",
     "ns3d_footprint", "ns3d_ctx_bytes",
 ]
 
@@ -83,6 +83,8 @@ def lib():
         L.ns3d_attach_engine.argtypes = [vp, vp]
         L.ns3d_engine_stats.argtypes = [vp, P(dbl)]
         L.ns3d_engine_set_graphs.argtypes = [vp, i32]
+        L.ns3d_engine_set_schedule.argtypes = [vp, i32, i32, i32]
+        L.ns3d_engine_get_schedule.argtypes = [vp, P(i32)]
         L.ns3d_footprint.argtypes = [i32, i32, i32, i64, i32, i32, P(dbl)]
         L.ns3d_ctx_bytes.argtypes = [vp, P(dbl)]
         L.ns3d_prepare_image.argtypes = [P(C.c_uint8), i32, i32, i32, i32, i32, P(C.c_uint8), P(i32)]
@@ -315,11 +317,21 @@ class Engine:
 
     STAT_NAMES = ["chunks", "launches", "solves", "max_active", "rows_x_iters"]
 
-    def __init__(self, device: int = 0, chunk: int = 8, graphs: bool = True):
+    def __init__(self, device: int = 0, chunk: int = 8, graphs: bool = True, lanes: int = 2):
         h = C.c_void_p()
         _check(lib().ns3d_engine_create(device, chunk, C.byref(h)))
         self.h = h
-        _check(lib().ns3d_engine_set_graphs(self.h, int(bool(graphs))))
+        _check(lib().ns3d_engine_set_schedule(self.h, chunk, lanes, int(bool(graphs))))
+
+    @staticmethod
+    def set_schedule_of(h, chunk: int, lanes: int, graphs: bool):
+        _check(lib().ns3d_engine_set_schedule(h, int(chunk), int(lanes), int(bool(graphs))))
+
+    @staticmethod
+    def schedule_of(h) -> dict:
+        out = np.zeros(3, np.int32)
+        _check(lib().ns3d_engine_get_schedule(h, _p(out, C.c_int32)))
+        return dict(chunk=int(out[0]), lanes=int(out[1]), graphs=bool(out[2]))
 
     def attach(self, ctx):
         _check(lib().ns3d_attach_engine(ctx, self.h))

@@ -295,6 +295,13 @@ class Server:
         kernel by kernel (see gpu.Engine)."""
         _gpu._check(_gpu.lib().ns3d_engine_set_graphs(C.c_void_p(lib().ns3d_server_engine(self.h)), int(bool(on))))
 
+    def set_schedule(self, chunk: int, lanes: int, graphs: bool = True):
+        """The engine's schedule (only while nothing is queued or running)."""
+        _gpu.Engine.set_schedule_of(C.c_void_p(lib().ns3d_server_engine(self.h)), chunk, lanes, graphs)
+
+    def schedule(self) -> dict:
+        return _gpu.Engine.schedule_of(C.c_void_p(lib().ns3d_server_engine(self.h)))
+
     def close(self):
         if getattr(self, "h", None):
             lib().ns3d_server_destroy(self.h)

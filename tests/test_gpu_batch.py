@@ -68,6 +68,28 @@ def test_graph_rounds_equal_direct_rounds():
     assert direct[0][1]["engine"]["chunks"] > 0 and graph[0][1]["engine"]["chunks"] > 0
 
 
+def test_schedules_give_the_same_bits():
+    """The engine's schedule (chunk length, streams, graphs: what the
+    autotuner chooses) moves only time: one stream with short chunks
+    launched kernel by kernel and three streams with long graph chunks give
+    the same results bit for bit; an out-of-range schedule is refused."""
+    imgs, params = mix()
+    out = []
+    for chunk, lanes, graphs in ((3, 1, False), (12, 3, True)):
+        eng = G.Engine(device=0, chunk=chunk, graphs=graphs, lanes=lanes)
+        try:
+            assert G.Engine.schedule_of(eng.h) == dict(chunk=chunk, lanes=lanes, graphs=graphs)
+            out.append(solver.solve_batch(imgs, params, engine=eng))
+        finally:
+            eng.close()
+    for i, ((r0, s0), (r1, s1)) in enumerate(zip(*out)):
+        assert s0["K"] == s1["K"] and np.array_equal(r0, r1), i
+        for f in "puvw":
+            assert np.array_equal(s0["fields"][f], s1["fields"][f]), (i, f)
+    with pytest.raises(G.NS3DError):
+        G.Engine(device=0, chunk=4, lanes=5)
+
+
 def test_engine_lifecycle_and_detach():
     """A context detached from an engine solves on its own stream again; an
     engine with nothing attached shuts down cleanly; bad arguments fail."""
