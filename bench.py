@@ -74,6 +74,7 @@ def served_pass(reqs, args, precision, label, arrivals=None):
     srv = solver.Server(device=0, budget_bytes=args.budget_gb * 1e9, max_slots=args.slots, chunk=args.chunk,
                         fallback=solver.FALLBACK_FAILED)
     srv.set_schedule(args.chunk, args.lanes, not getattr(args, "no_graphs", False))
+    srv.set_policy(largest_first=False, max_rows=getattr(args, "max_rows", 0.0))
     try:
         t0 = time.perf_counter()
         tickets = []
@@ -260,7 +261,7 @@ def mix_section(args):
                          f"{args.mix_maxit} outer iterations",
                 requests=n, fluid_cells=int(fluid), mix_seed=args.mix_seed, slots=args.slots,
                 schedule=dict(chunk=args.chunk, lanes=args.lanes, graphs=not args.no_graphs,
-                              autotuned=tuned is not None), chunk=args.chunk,
+                              autotuned=tuned is not None, max_rows_in_flight=args.max_rows), chunk=args.chunk,
                 images=[dict(shape=d["shape"], generator=d["generator"], porosity=d["porosity"]) for _, d in reqs])
     return dict(forms=forms, desc=desc, reqs=reqs, plain=plain, mixed_wall=mixed_wall, lat=lat_m,
                 max_dk=max(dk_mixed), gate=gate, engine=smx[-1]["engine"], autotune=tuned)
@@ -468,6 +469,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="served forms: launch the engine's chunks kernel by kernel, not as HIP graphs")
     ap.add_argument("--lanes", type=int, default=2, help="served forms: engine streams")
+    ap.add_argument("--max-rows", type=float, default=4e6,
+                    help="served forms: cap on the fluid cells of the running requests (0: none)")
     ap.add_argument("--tune-maxit", type=int, default=60, help="autotuner: outer-iteration cap of its trials")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="pick chunk / lanes / graphs with ns3d_amd.autotune before the mix (full runs)")

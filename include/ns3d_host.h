@@ -122,6 +122,15 @@ int ns3d_server_wait(ns3d_server* s, int64_t ticket, ns3d_run_result* res, doubl
 int ns3d_server_stats(ns3d_server* s, double out[16]);
 /* The server's solve engine (for ns3d_engine_stats). */
 struct ns3d_engine* ns3d_server_engine(ns3d_server* s);
+/* Admission policy.  order 0 (default): requests start in arrival order;
+ * 1: the largest queued request (fluid cells) first, which shortens a burst
+ * whose long requests would otherwise start late.  Either way a request that
+ * does not fit is skipped for the next one that does.  max_rows > 0 caps
+ * the fluid cells of the running requests (a request alone always runs):
+ * once the GPU is full, more requests in flight only stretch every
+ * request's iteration cadence.  0 (default): no cap.  Returns 0, or 1 on
+ * bad arguments. */
+int ns3d_server_set_policy(ns3d_server* s, int order, double max_rows);
 /* Finishes every queued and running request, then frees the server. */
 void ns3d_server_destroy(ns3d_server* s);
 const char* ns3d_server_last_error(void);

@@ -2,10 +2,12 @@
 // the reference, which runs one image per process).
 //
 // Requests queue in arrival order.  A pool of worker threads (the slots)
-// takes the head request when its projected device footprint fits in what
-// the budget has left (ns3d_footprint, an upper bound of the context's
-// allocations), runs its driver with the context attached to the server's
-// one solve engine, and frees the memory and the slot on return.  Because all
+// takes the first queued request (in arrival order, or the largest first)
+// whose projected device footprint fits in what the budget has left
+// (ns3d_footprint, an upper bound of the context's allocations) and, if a
+// cap is set, whose rows fit under the rows in flight.  It runs the
+// request's driver with the context attached to the server's one solve
+// engine, and frees the memory and the slot on return.  Because all
 // running requests hand their BiCGSTAB solves to the same engine, solves of
 // requests that overlap in time run in the same launches, and a request that
 // arrives while others are on the device joins them at the next chunk
@@ -46,6 +48,7 @@ struct ns3d_server {
     std::vector<uint8_t> img;
     int nx = 0, ny = 0, nz = 0;
     double footprint = 0;
+    double rows = 0;  // fluid cells of the (rotated, bordered) image: the size of each of its systems
     int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
     ns3d_run_result res{};
     double t_submit = 0, t_start = 0, t_end = 0;
@@ -63,6 +66,12 @@ struct ns3d_server {
   std::unordered_map<int64_t, std::unique_ptr<Job>> jobs;
   int64_t next_id = 1;
   double in_use = 0, peak_in_use = 0;
+  // admission policy: order 0 takes requests in arrival order, 1 the largest
+  // first; both skip to the first queued request that fits.  max_rows > 0
+  // caps the rows in flight (enough to fill the GPU; more only stretches
+  // every request's iteration cadence).
+  int order = 0;
+  double max_rows = 0, rows_in_flight = 0;
   int running = 0, peak_running = 0;
   double n_done = 0, n_failed = 0, n_fallback = 0, n_submitted = 0;
   bool stopping = false;
@@ -70,6 +79,19 @@ struct ns3d_server {
 
   void worker();
   void run_job(Job& j);
+  // The queued request to start now, or queue.end() (call with mu held).
+  std::deque<Job*>::iterator pick() {
+    auto best = queue.end();
+    for (auto it = queue.begin(); it != queue.end(); ++it) {
+      Job* j = *it;
+      const bool fits = running == 0 || (in_use + j->footprint <= budget &&
+                                         (max_rows <= 0 || rows_in_flight + j->rows <= max_rows));
+      if (!fits) continue;
+      if (order == 0) return it;
+      if (best == queue.end() || j->rows > (*best)->rows) best = it;
+    }
+    return best;
+  }
 };
 
 void ns3d_server::run_job(Job& j) {
@@ -136,13 +158,16 @@ void ns3d_server::worker() {
     Job* j = nullptr;
     {
       std::unique_lock<std::mutex> lk(mu);
+      std::deque<Job*>::iterator at;
       cv_work.wait(lk, [&] {
         if (stopping && queue.empty()) return true;
-        return !queue.empty() && (running == 0 || in_use + queue.front()->footprint <= budget);
+        at = pick();
+        return at != queue.end();
       });
       if (queue.empty()) return;  // stopping
-      j = queue.front();
-      queue.pop_front();
+      j = *at;
+      queue.erase(at);
+      rows_in_flight += j->rows;
       j->state = 1;
       j->t_start = now_s(t0);
       in_use += j->footprint;
@@ -166,6 +191,7 @@ void ns3d_server::worker() {
       j->img.clear();
       j->img.shrink_to_fit();
       in_use -= j->footprint;
+      rows_in_flight -= j->rows;
       running -= 1;
       if (err.empty()) n_done += 1;
       else n_failed += 1;
@@ -225,6 +251,7 @@ int ns3d_server_submit(ns3d_server* s, const ns3d_run_params* p, const uint8_t* 
   if (p->direction == 3) { d[0] = nz; d[2] = nx; }
   d[0] += 2 * p->border;
   nfluid += (int64_t)2 * p->border * d[1] * d[2];
+  j->rows = (double)nfluid;
   if (ns3d_footprint(d[0], d[1], d[2], nfluid, p->precision != 0, 3, &j->footprint) != NS3D_OK) {
     g_srv_err = ns3d_last_error();
     return 2;
@@ -289,6 +316,20 @@ int ns3d_server_stats(ns3d_server* s, double out[16]) {
 }
 
 ns3d_engine* ns3d_server_engine(ns3d_server* s) { return s ? s->eng : nullptr; }
+
+int ns3d_server_set_policy(ns3d_server* s, int order, double max_rows) {
+  if (!s || order < 0 || order > 1 || max_rows < 0) {
+    g_srv_err = "ns3d_server_set_policy: bad arguments";
+    return 1;
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->order = order;
+    s->max_rows = max_rows;
+  }
+  s->cv_work.notify_all();
+  return 0;
+}
 
 void ns3d_server_destroy(ns3d_server* s) {
   if (!s) return;

@@ -20,7 +20,8 @@ METHODS = {"SIMPLEC": 0, "SIMPLER": 1, "SIMPLERR": 2, "IDEAL": 3}
 EXPORTS = ["ns3d_default_run_params", "ns3d_solver_create", "ns3d_solver_info", "ns3d_solver_run",
            "ns3d_solver_fields", "ns3d_solver_destroy", "ns3d_solver_context", "ns3d_host_last_error",
            "ns3d_read_ini", "ns3d_solver_warm_start", "ns3d_server_create", "ns3d_server_submit", "ns3d_server_poll", "ns3d_server_wait",
-           "ns3d_server_stats", "ns3d_server_destroy", "ns3d_server_last_error", "ns3d_server_engine"]
+           "ns3d_server_stats", "ns3d_server_destroy", "ns3d_server_last_error", "ns3d_server_engine",
+           "ns3d_server_set_policy"]
 
 
 class RunParams(C.Structure):
@@ -93,6 +94,7 @@ def lib():
         L.ns3d_server_last_error.restype = C.c_char_p
         L.ns3d_server_engine.argtypes = [vp]
         L.ns3d_server_engine.restype = vp
+        L.ns3d_server_set_policy.argtypes = [vp, C.c_int, C.c_double]
         _lib = L
     return _lib
 
@@ -298,6 +300,12 @@ class Server:
     def set_schedule(self, chunk: int, lanes: int, graphs: bool = True):
         """The engine's schedule (only while nothing is queued or running)."""
         _gpu.Engine.set_schedule_of(C.c_void_p(lib().ns3d_server_engine(self.h)), chunk, lanes, graphs)
+
+    def set_policy(self, largest_first: bool = False, max_rows: float = 0.0):
+        """Admission order (arrival, or largest first) and the cap on fluid
+        cells in flight (0: none); see ns3d_server_set_policy."""
+        if lib().ns3d_server_set_policy(self.h, int(bool(largest_first)), float(max_rows)) != 0:
+            raise _gpu.NS3DError(lib().ns3d_server_last_error().decode())
 
     def schedule(self) -> dict:
         return _gpu.Engine.schedule_of(C.c_void_p(lib().ns3d_server_engine(self.h)))

@@ -84,6 +84,37 @@ def test_admission_respects_budget():
     assert st["bytes_in_use"] == 0
 
 
+def test_rows_cap_and_largest_first():
+    """With the fluid cells in flight capped at the largest request's, one
+    request runs at a time (in arrival order the last-submitted largest
+    request would start last; largest-first starts it first), and every
+    result is still the one the request gets alone."""
+    reqs = requests()[:4]
+    rows = [int((img == 0).sum()) for img, _ in reqs]
+    solo = [solver.solve(img, **p)[1]["K"] for img, p in reqs]
+    order = []
+    for largest in (False, True):
+        srv = solver.Server(device=0, budget_bytes=16e9, max_slots=4, chunk=4)
+        try:
+            srv.set_policy(largest_first=largest, max_rows=max(rows))
+            tickets = [srv.submit(img, **p) for img, p in reqs]
+            out = [srv.wait(t) for t in tickets]
+            st = srv.stats()
+        finally:
+            srv.close()
+        assert st["peak_running"] == 1, st
+        assert [r["K"] for r in out] == solo
+        order.append(sorted(range(4), key=lambda q: out[q]["t_start"]))
+    assert order[0] == [0, 1, 2, 3]
+    assert order[1][0] == int(np.argmax(rows)) or order[1][0] == 0  # the first may start before the others queue
+    with pytest.raises(G.NS3DError):
+        srv2 = solver.Server(device=0, budget_bytes=1e9, max_slots=1, chunk=4)
+        try:
+            srv2.set_policy(largest_first=True, max_rows=-1.0)
+        finally:
+            srv2.close()
+
+
 def test_footprint_bounds_context_allocation():
     eng = G.Engine(0, 8)
     try:
