@@ -468,7 +468,7 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events (profiler runs)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="served forms: launch the engine's chunks kernel by kernel, not as HIP graphs")
-    ap.add_argument("--lanes", type=int, default=2, help="served forms: engine streams")
+    ap.add_argument("--lanes", type=int, default=3, help="served forms: engine streams")
     ap.add_argument("--max-rows", type=float, default=4e6,
                     help="served forms: cap on the fluid cells of the running requests (0: none)")
     ap.add_argument("--tune-maxit", type=int, default=60, help="autotuner: outer-iteration cap of its trials")
@@ -531,7 +531,7 @@ def main():
         it = engine_chunk(mixres["engine"])
         line["roofline"] = None if r is None else dict(
             bound="hbm", achieved=r["achieved"], peak=PEAK_HBM_GBS, unit="GB/s", frac=r["frac"], traffic=traffic,
-            kernel="bicg_A_kernel<float> batched over the requests in flight (served mixed form, one of two "
+            kernel="bicg_A_kernel<float> batched over the requests in flight (served mixed form, one of three "
                    "concurrent streams): p update + 7-point SpMV + r0.v", bytes_per_launch=r["bytes_per_launch"],
             avg_launch_us=r["avg_launch_us"], sampled_launches=r["sampled_launches"], traffic_source=tsrc,
             engine_chunks=it)

@@ -247,7 +247,10 @@ ns3d_status ns3d_engine_create(int device, int chunk, ns3d_engine** out);
 void ns3d_engine_destroy(ns3d_engine* eng);
 /* eng NULL detaches.  Attaching also gives the context two more solver
  * workspaces, so that the three momentum solves of
- * ns3d_compute_velocity_field go to the engine together. */
+ * ns3d_compute_velocity_field go to the engine together.  While a context
+ * is the only one attached to its engine, its solves run on its own stream
+ * as if detached (nothing to batch with; same arithmetic), and go to the
+ * engine again once another context attaches. */
 ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
 /* on (default): each chunk of iterations is submitted as one HIP graph per
  * engine stream (reused while the grids stay the same; the kernels read the
@@ -256,7 +259,7 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
  * round kernel by kernel.  Results are the same either way. */
 ns3d_status ns3d_engine_set_graphs(ns3d_engine* eng, int on);
 /* The engine's schedule: chunk (iterations queued per round, 1..64), lanes
- * (streams the solves are spread over, 1..4; default 2) and graphs (as
+ * (streams the solves are spread over, 1..4; default 3) and graphs (as
  * ns3d_engine_set_graphs).  Only while no solve is pending or running
  * (NS3D_ERR_INVALID otherwise).  No choice changes any result; an autotuner
  * (ns3d_amd/autotune.py) picks them by timing a request sample.  get:

@@ -106,6 +106,7 @@ struct ns3d_ctx {
   int precision = NS3D_PRECISION_FP64;
   double inner_tol = 1e-4;
   int64_t mixed_min_rows = 0;  // mixed form: smaller systems are solved in FP64
+  bool engine_used = false;     // the last solves went through the attached engine
   ns3d_params prm{};
   bool have_params = false;
   std::vector<void*> allocs;
@@ -276,16 +277,22 @@ void stream_solve(ns3d_ctx* c, const SolveJob<T>& j) {
 }
 
 // Runs the solves, each leaving its final state in its lane's hstate: with an
-// engine attached all at once (batched with the other contexts' solves),
-// else one after the other on the context's stream.  Each solve's arithmetic
-// is the same either way.
+// engine that other contexts share, all at once (batched with the other
+// contexts' solves), else one after the other on the context's stream.  Each
+// solve's arithmetic is the same either way, so a context may switch between
+// the two from one call to the next.
 template <typename T>
 void run_solves(ns3d_ctx* c, SolveJob<T>* jobs, int n) {
   if (n <= 0) return;
-  if (!c->engine) {
+  if (!c->engine || c->engine->attached.load() <= 1) {
+    // rounds the engine queued before this context's last batched solve
+    // returned may still name its lanes' state: let them drain first
+    if (c->engine && c->engine_used) c->engine->fence();
+    c->engine_used = false;
     for (int i = 0; i < n; ++i) stream_solve<T>(c, jobs[i]);
     return;
   }
+  c->engine_used = true;
   if (n > 3) throw Invalid{"run_solves: at most three solves at once"};
   EngineRequest req[3];
   EngineRequest* rp[3];
@@ -702,7 +709,10 @@ void ns3d_destroy(ns3d_ctx* c) {
   if (!c) return;
   // rounds an engine queued before this context's last solve returned may
   // still name its state buffers
-  if (c->engine) c->engine->fence();
+  if (c->engine) {
+    c->engine->fence();
+    c->engine->attached.fetch_sub(1);
+  }
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) (void)hipFree(p);
   for (Lane& L : c->ln) {
@@ -1101,7 +1111,6 @@ void ns3d_engine_destroy(ns3d_engine* e) { delete e; }
 ns3d_status ns3d_attach_engine(ns3d_ctx* c, ns3d_engine* e) {
   if (!c) return fail_invalid("ns3d_attach_engine: null context");
   if (e && e->device != c->device) return fail_invalid("ns3d_attach_engine: engine and context on different devices");
-  if (c->engine && c->engine != e) c->engine->fence();
   if (e && c->nlanes < 3 && c->sys[0].n > 0) {
     // lanes for the momentum solves to run at once
     const ns3d_status st = guarded([&] {
@@ -1111,7 +1120,15 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* c, ns3d_engine* e) {
     });
     if (st != NS3D_OK) return st;
   }
-  c->engine = e;
+  if (c->engine != e) {
+    if (c->engine) {
+      c->engine->fence();
+      c->engine->attached.fetch_sub(1);
+    }
+    if (e) e->attached.fetch_add(1);
+    c->engine = e;
+    c->engine_used = false;
+  }
   return NS3D_OK;
 }
 

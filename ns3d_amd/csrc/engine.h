@@ -76,8 +76,12 @@ struct ns3d_engine {
   static constexpr int kMaxLanes = 4;
   int device = 0;
   int chunk = 8;
-  int nlanes = 2;  // streams in use (1..kMaxLanes)
+  int nlanes = 3;  // streams in use (1..kMaxLanes)
   std::atomic<bool> graphs{true};
+  // contexts attached (ns3d_attach_engine / ns3d_destroy keep the count):
+  // with only one, its solves run on its own stream instead (nothing to
+  // batch with, and the single-solve launches are shorter)
+  std::atomic<int> attached{0};
 
  private:
   struct Group {
@@ -114,6 +118,8 @@ struct ns3d_engine {
   hipGraphExec_t chunk_graph(int lane, const int grid[2]);
   void launch_direct(int buf, const bool* on);
   void queue_round(int buf, bool restart[kGroups]);
+  // blocks until every lane of round buf has its states back on the host
+  void wait_round(int buf);
   bool process_round(int buf, bool restart[kGroups]);
   template <typename T>
   ns3d::Batch<T> batch(int g, int q) const;
@@ -157,8 +163,9 @@ struct ns3d_engine {
   std::unordered_map<long long, ns3d::EngineRequest*> live;
   long long next_serial = 1;
   bool round_on[2][kGroups] = {};      // groups with launches in the round
-  hipEvent_t round_done[2] = {};
-  hipEvent_t lane_join[2][kMaxLanes] = {};  // lane l's chunk done (lane 0 waits before the gather)
+  // lane l's gather and copy of round buf done; lanes with solves in it
+  hipEvent_t lane_done[2][kMaxLanes] = {};
+  bool lane_in[2][kMaxLanes] = {};
   // dispatch events per group: [A, B, C of the sampled iteration][start,
   // stop], then the chunk's first A and last C [start, stop]
   static constexpr int kEv = 10;

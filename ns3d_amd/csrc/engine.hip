@@ -106,8 +106,7 @@ ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) 
     d_states[b] = (BState*)dalloc(sizeof(BState) * kGroups * kEngMaxSolves);
     h_stptr[b].resize(kGroups * kEngMaxSolves);
     ECHK(hipHostMalloc((void**)&h_states[b], sizeof(BState) * kGroups * kEngMaxSolves, hipHostMallocDefault));
-    ECHK(hipEventCreateWithFlags(&round_done[b], hipEventDisableTiming));
-    for (auto& e : lane_join[b]) ECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : lane_done[b]) ECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : tev[b]) ECHK(hipEventCreate(&e));
   }
   thr = std::thread([this] { run(); });
@@ -124,8 +123,7 @@ ns3d_engine::~ns3d_engine() {
     if (st) (void)hipStreamSynchronize(st);
   for (int b = 0; b < 2; ++b) {
     if (h_states[b]) (void)hipHostFree(h_states[b]);
-    if (round_done[b]) (void)hipEventDestroy(round_done[b]);
-    for (auto& e : lane_join[b])
+    for (auto& e : lane_done[b])
       if (e) (void)hipEventDestroy(e);
     for (auto& e : tev[b])
       if (e) (void)hipEventDestroy(e);
@@ -207,6 +205,11 @@ void ns3d_engine::stats(double* out, int count) {
   v[29] = tiles_live;
   v[30] = tiles_all;
   for (int q = 0; q < count && q < 32; ++q) out[q] = v[q];
+}
+
+void ns3d_engine::wait_round(int buf) {
+  for (int l = 0; l < kMaxLanes; ++l)
+    if (lane_in[buf][l]) ECHK(hipEventSynchronize(lane_done[buf][l]));
 }
 
 void ns3d_engine::fail(const std::string& msg) {
@@ -355,12 +358,11 @@ void ns3d_engine::rebuild(int g, int buf) {
   G.dirty = false;
 }
 
-// Tables, restarts, one chunk on each lane, the join, and the gather of the
-// states of every solve in the round, into buffer buf.
+// Tables, restarts, one chunk on each lane, and on each lane the gather of
+// the states of its solves, into buffer buf.  A lane reads back only its own
+// solves' states, in its own stream order, so the lanes need not wait for
+// each other: each starts its next round as soon as its own gather is done.
 void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
-  // the other lanes start this round after the previous round's gather (on
-  // lane 0), so that gather never reads a state this round is writing
-  for (int l = 1; l < nlanes; ++l) ECHK(hipStreamWaitEvent(lane_stream[l], round_done[buf ^ 1], 0));
   for (int g = 0; g < kGroups; ++g) {
     if (grp[g].dirty) rebuild(g, buf);
     // the restart kernel acts only on solves whose status asks for it
@@ -376,13 +378,21 @@ void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
   std::vector<RoundEntry>& ord = order[buf];
   ord.clear();
   int nact = 0;
-  for (int g = 0; g < kGroups; ++g)
+  int off[kGroups + 1];
+  for (int g = 0; g < kGroups; ++g) {
+    off[g] = nact;
     for (EngineRequest* r : grp[g].active) {
       ord.push_back(RoundEntry{r->serial, r->it_queued, g, r->n, r->nnz, r->tiles});
       h_stptr[buf][nact++] = r->prec == 0 ? r->d64.w.st : r->d32.w.st;
     }
+    // the group's state pointers, on its lane (pageable source: staged
+    // before the call returns)
+    if (nact > off[g])
+      ECHK(hipMemcpyAsync(d_stptr[buf] + off[g], h_stptr[buf].data() + off[g], (nact - off[g]) * sizeof(BState*),
+                          hipMemcpyHostToDevice, lane_stream[g % kMaxLanes]));
+  }
+  off[kGroups] = nact;
   max_active = std::max(max_active, (double)nact);
-  ECHK(hipMemcpyAsync(d_stptr[buf], h_stptr[buf].data(), nact * sizeof(BState*), hipMemcpyHostToDevice, stream));
   bool* on = round_on[buf];
   int ngroups = 0;
   for (int g = 0; g < kGroups; ++g) ngroups += (on[g] = !grp[g].active.empty());
@@ -401,17 +411,23 @@ void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
   }
   n_launch += (double)chunk * 3 * ngroups;
   n_chunks += 1;
-  // join: lane 0 waits for lane 1's chunk before the gather
-  for (int l = 1; l < nlanes; ++l) {
-    ECHK(hipEventRecord(lane_join[buf][l], lane_stream[l]));
-    ECHK(hipStreamWaitEvent(stream, lane_join[buf][l], 0));
+  // per lane: gather its groups' states, copy them back, mark the lane done
+  for (int l = 0; l < kMaxLanes; ++l) {
+    lane_in[buf][l] = false;
+    for (int p = 0; p < 2; ++p) {
+      const int g = p * kMaxLanes + l;
+      const int ns = off[g + 1] - off[g];
+      if (!on[g] || ns == 0) continue;
+      const int words = ns * (int)(sizeof(BState) / 8);
+      hipLaunchKernelGGL(gather_states_kernel, dim3((words + 255) / 256), dim3(256), 0, lane_stream[l],
+                         d_stptr[buf] + off[g], d_states[buf] + off[g], ns);
+      ECHK(hipGetLastError());
+      ECHK(hipMemcpyAsync(h_states[buf] + off[g], d_states[buf] + off[g], ns * sizeof(BState), hipMemcpyDeviceToHost,
+                          lane_stream[l]));
+      lane_in[buf][l] = true;
+    }
+    if (lane_in[buf][l]) ECHK(hipEventRecord(lane_done[buf][l], lane_stream[l]));
   }
-  const int words = nact * (int)(sizeof(BState) / 8);
-  hipLaunchKernelGGL(gather_states_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, d_stptr[buf],
-                     d_states[buf], nact);
-  ECHK(hipGetLastError());
-  ECHK(hipMemcpyAsync(h_states[buf], d_states[buf], nact * sizeof(BState), hipMemcpyDeviceToHost, stream));
-  ECHK(hipEventRecord(round_done[buf], stream));
   std::lock_guard<std::mutex> lk(mu);
   rounds_queued += 1;
 }
@@ -492,7 +508,7 @@ hipGraphExec_t ns3d_engine::chunk_graph(int lane, const int grid[2]) {
 // round restarts its group, else it is flagged for the next one.  Stopped
 // solves are retired and their callers woken.  Returns whether any was.
 bool ns3d_engine::process_round(int buf, bool restart[kGroups]) {
-  ECHK(hipEventSynchronize(round_done[buf]));
+  wait_round(buf);
   bool any_done = false;
   std::lock_guard<std::mutex> lk(mu);
   const std::vector<RoundEntry>& ord = order[buf];
@@ -636,7 +652,7 @@ void ns3d_engine::run() {
       const auto h1 = std::chrono::steady_clock::now();
       bool woke = false;
       if (inflight >= 0) {
-        ECHK(hipEventSynchronize(round_done[inflight]));
+        wait_round(inflight);
         const auto h2 = std::chrono::steady_clock::now();
         woke = process_round(inflight, restart);
         const auto h3 = std::chrono::steady_clock::now();

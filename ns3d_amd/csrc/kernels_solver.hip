@@ -404,8 +404,27 @@ bicg_restart_fin(Batch<T> B) {
 // scalars for the host.
 __device__ __forceinline__ bool leader_of(int t, int t0) { return t == t0 && threadIdx.x == 0; }
 
+// Minimum waves per SIMD asked of the compiler for kernels A and B (it
+// then keeps their registers within that occupancy); 0: its own choice.
+#ifndef NS3D_A_WAVES
+#define NS3D_A_WAVES 0
+#endif
+#ifndef NS3D_B_WAVES
+#define NS3D_B_WAVES 0
+#endif
+#if NS3D_A_WAVES > 0
+#define NS3D_A_ATTR __attribute__((amdgpu_waves_per_eu(NS3D_A_WAVES)))
+#else
+#define NS3D_A_ATTR
+#endif
+#if NS3D_B_WAVES > 0
+#define NS3D_B_ATTR __attribute__((amdgpu_waves_per_eu(NS3D_B_WAVES)))
+#else
+#define NS3D_B_ATTR
+#endif
+
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) NS3D_A_ATTR
 bicg_A_kernel(Batch<T> B) {
   const int t = block_tile(B);
   if (t < 0) return;
@@ -490,7 +509,7 @@ bicg_A_kernel(Batch<T> B) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) NS3D_B_ATTR
 bicg_B_kernel(Batch<T> B) {
   const int t = block_tile(B);
   if (t < 0) return;

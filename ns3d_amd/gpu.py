@@ -317,7 +317,7 @@ class Engine:
 
     STAT_NAMES = ["chunks", "launches", "solves", "max_active", "rows_x_iters"]
 
-    def __init__(self, device: int = 0, chunk: int = 8, graphs: bool = True, lanes: int = 2):
+    def __init__(self, device: int = 0, chunk: int = 8, graphs: bool = True, lanes: int = 3):
         h = C.c_void_p()
         _check(lib().ns3d_engine_create(device, chunk, C.byref(h)))
         self.h = h
