@@ -250,7 +250,10 @@ void ns3d_engine_destroy(ns3d_engine* eng);
  * ns3d_compute_velocity_field go to the engine together.  While a context
  * is the only one attached to its engine, its solves run on its own stream
  * as if detached (nothing to batch with; same arithmetic), and go to the
- * engine again once another context attaches. */
+ * engine again once another context attaches.  While attached, the
+ * context's own kernels and copies run on the engine's context stream (one
+ * for all attached contexts, on a hardware queue apart from the engine's
+ * lanes); attach and detach first wait for the context's queued work. */
 ns3d_status ns3d_attach_engine(ns3d_ctx* ctx, ns3d_engine* eng);
 /* on (default): each chunk of iterations is submitted as one HIP graph per
  * engine stream (reused while the grids stay the same; the kernels read the

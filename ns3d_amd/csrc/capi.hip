@@ -86,7 +86,9 @@ struct Lane {
 
 struct ns3d_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // where the context's work goes: its own stream, or its engine's
+  hipStream_t own_stream = nullptr;  // the stream it created
+  hipEvent_t ev_sync = nullptr;      // marks the end of its own work so far (ctx_sync)
   int nx = 0, ny = 0, nz = 0;
   int64_t ncell = 0;
   uint8_t* img = nullptr;
@@ -132,6 +134,15 @@ struct ns3d_ctx {
 };
 
 namespace {
+
+// Waits for the work this context has queued so far.  An attached context's
+// stream is shared with the engine's other contexts, so it waits on an event
+// recorded after its own work, not for whatever the others queued since.
+hipError_t ctx_sync(ns3d_ctx* c) {
+  hipError_t e = hipEventRecord(c->ev_sync, c->stream);
+  if (e != hipSuccess) return e;
+  return hipEventSynchronize(c->ev_sync);
+}
 
 ns3d_status fail_hip(const HipFail& f) {
   char buf[512];
@@ -237,7 +248,7 @@ void stream_solve(ns3d_ctx* c, const SolveJob<T>& j) {
   const SolveDesc<T> d{j.M, j.w, j.x, j.b, j.tol, (long long)j.maxit, j.L->partials, kMaxTilesPerSolve, 0, 0, 0};
   HCHK(launch_bicg_init<T>(single_batch(d, 0, 0), s));
   HCHK(hipMemcpyAsync(hs, j.w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
-  HCHK(hipStreamSynchronize(s));
+  HCHK(ctx_sync(c));
   int64_t& hint = *j.hint;
   int64_t chunk = hint > 4 ? (hint * 3) / 4 : 4;
   int64_t launched = 0;
@@ -260,7 +271,7 @@ void stream_solve(ns3d_ctx* c, const SolveJob<T>& j) {
       HCHK(launch_bicg_C<T>(B, s, e ? e + 4 : nullptr));
     }
     HCHK(hipMemcpyAsync(hs, j.w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
-    HCHK(hipStreamSynchronize(s));
+    HCHK(ctx_sync(c));
     if (qt >= 0 && hs->it_c - done_before > qt)
       for (int k = 0; k < 3; ++k) {
         float ms = 0.f;
@@ -387,7 +398,7 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
       any = true;
     }
     if (!any) return;
-    HCHK(hipStreamSynchronize(s));
+    HCHK(ctx_sync(c));
     for (int i = 0; i < n; ++i)
       if (which[i]) {
         t[i].rr = L[i]->hresult[0];
@@ -630,6 +641,8 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
     }
     HCHK(hipGetDevice(&c->device));
     HCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = c->stream;
+    HCHK(hipEventCreateWithFlags(&c->ev_sync, hipEventDisableTiming));
     HCHK(hipEventCreate(&c->ev[0]));
     HCHK(hipEventCreate(&c->ev[1]));
     c->nx = nx; c->ny = ny; c->nz = nz; c->ncell = ncell;
@@ -650,7 +663,7 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
     HCHK(launch_scan_count(c->img, nx, ny, nz, sb, s));
     int32_t tot[4];
     HCHK(hipMemcpyAsync(tot, sb.totals, sizeof(tot), hipMemcpyDeviceToHost, s));
-    HCHK(hipStreamSynchronize(s));
+    HCHK(ctx_sync(c));
     const int np = tot[0], nu = tot[1], nv = tot[2], nw = tot[3];
     alloc_system(c, 0, nu, nx + 1, c->ug, true);
     alloc_system(c, 1, nv, nx, c->vg, true);
@@ -665,7 +678,7 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
       HCHK(launch_build_nbr(g, q, c->sys[q].cell, c->sys[q].n, c->sys[q].nbr, dnnz + q, s));
     unsigned long long hnnz[4];
     HCHK(hipMemcpyAsync(hnnz, dnnz, sizeof(hnnz), hipMemcpyDeviceToHost, s));
-    HCHK(hipStreamSynchronize(s));
+    HCHK(ctx_sync(c));
     for (int q = 0; q < 4; ++q) c->sys[q].nnz = (int64_t)hnnz[q];
     // packed neighbour distances where every distance fits (always, for the
     // monotone numbering of the index scan, unless a plane is very large)
@@ -677,7 +690,7 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
     }
     int hbad[4];
     HCHK(hipMemcpyAsync(hbad, dbad, sizeof(hbad), hipMemcpyDeviceToHost, s));
-    HCHK(hipStreamSynchronize(s));
+    HCHK(ctx_sync(c));
     for (int q = 0; q < 4; ++q)
       c->sys[q].layout = hbad[q] ? ML_INDEXED : (q == 3 ? ML_PACKED_SYM : ML_PACKED);
     for (int set = 0; set < 3; ++set)
@@ -690,7 +703,7 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
     c->lane_rows = nmax;
     alloc_lane(c, c->ln[0]);
     c->nlanes = 1;
-    HCHK(hipStreamSynchronize(s));
+    HCHK(ctx_sync(c));
   });
   if (st != NS3D_OK) {
     ns3d_destroy(c);
@@ -714,6 +727,7 @@ void ns3d_destroy(ns3d_ctx* c) {
     c->engine->attached.fetch_sub(1);
   }
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->stream = c->own_stream;
   for (void* p : c->allocs) (void)hipFree(p);
   for (Lane& L : c->ln) {
     if (L.hstate) (void)hipHostFree(L.hstate);
@@ -721,6 +735,7 @@ void ns3d_destroy(ns3d_ctx* c) {
   }
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->ev_sync) (void)hipEventDestroy(c->ev_sync);
   for (auto& e : c->tev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -741,7 +756,7 @@ ns3d_status ns3d_get_index(ns3d_ctx* c, int32_t* p, int32_t* u, int32_t* v, int3
     if (u) HCHK(hipMemcpyAsync(u, c->ug, nuc, hipMemcpyDeviceToHost, c->stream));
     if (v) HCHK(hipMemcpyAsync(v, c->vg, nc, hipMemcpyDeviceToHost, c->stream));
     if (w) HCHK(hipMemcpyAsync(w, c->wg, nc, hipMemcpyDeviceToHost, c->stream));
-    HCHK(hipStreamSynchronize(c->stream));
+    HCHK(ctx_sync(c));
   });
 }
 
@@ -762,7 +777,7 @@ ns3d_status ns3d_zero_fields(ns3d_ctx* c) {
     HCHK(launch_fill(c->p, c->sys[3].n, 0.0, c->stream));
     HCHK(launch_fill(c->pc, c->sys[3].n, 0.0, c->stream));
     HCHK(launch_fill(c->b, c->sys[3].n, 0.0, c->stream));
-    HCHK(hipStreamSynchronize(c->stream));
+    HCHK(ctx_sync(c));
   });
 }
 
@@ -771,7 +786,7 @@ ns3d_status ns3d_init_pressure_linear(ns3d_ctx* c) {
   return guarded([&] {
     HCHK(launch_init_pressure_linear(c->geo(), c->sys[3].cell, c->sys[3].n, c->p, c->prm.p0, c->prm.p1,
                                      c->stream));
-    HCHK(hipStreamSynchronize(c->stream));
+    HCHK(ctx_sync(c));
   });
 }
 
@@ -782,7 +797,7 @@ ns3d_status ns3d_set_vector(ns3d_ctx* c, int which, const double* host, int64_t 
     double* d = vec_ptr(c, which, len);
     need(n == len, "ns3d_set_vector: length mismatch");
     if (len) HCHK(hipMemcpyAsync(d, host, (size_t)len * 8, hipMemcpyHostToDevice, c->stream));
-    HCHK(hipStreamSynchronize(c->stream));
+    HCHK(ctx_sync(c));
   });
 }
 
@@ -793,7 +808,7 @@ ns3d_status ns3d_get_vector(ns3d_ctx* c, int which, double* host, int64_t n) {
     double* d = vec_ptr(c, which, len);
     need(n == len, "ns3d_get_vector: length mismatch");
     if (len) HCHK(hipMemcpyAsync(host, d, (size_t)len * 8, hipMemcpyDeviceToHost, c->stream));
-    HCHK(hipStreamSynchronize(c->stream));
+    HCHK(ctx_sync(c));
   });
 }
 
@@ -864,7 +879,7 @@ ns3d_status ns3d_compute_velocity_field(ns3d_ctx* c, double alpha, int limit, do
       bicgstab_many(c, m, sid, xs, bs, tol, maxit, info, Ls);
       for (int q = 0; q < m; ++q) finish(q, c->ln[q]);
     }
-    HCHK(hipStreamSynchronize(s));
+    HCHK(ctx_sync(c));
   });
 }
 
@@ -895,7 +910,7 @@ ns3d_status ns3d_compute_pressure_correction(ns3d_ctx* c, double tol, int64_t ma
   if (!c || !info) return fail_invalid("ns3d_compute_pressure_correction: bad arguments");
   return guarded([&] {
     bicgstab(c, 3, c->pc, c->b, tol, maxit, info);
-    HCHK(hipStreamSynchronize(c->stream));
+    HCHK(ctx_sync(c));
   });
 }
 
@@ -915,7 +930,7 @@ ns3d_status ns3d_compute_pressure_field(ns3d_ctx* c, double tol, int64_t maxit, 
     }
     info->work = work;
     if (info->iterations > 0) HCHK(hipMemcpyAsync(c->p, ptemp, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-    HCHK(hipStreamSynchronize(s));
+    HCHK(ctx_sync(c));
   });
 }
 
@@ -938,7 +953,7 @@ ns3d_status ns3d_correct_pressure(ns3d_ctx* c, double beta, double out[3]) {
     need(n > 0, "ns3d_correct_pressure: no pressure nodes");
     HCHK(launch_correct_pressure(c->p, c->pc, n, beta, c->ln[0].partials, c->ln[0].result, c->stream));
     HCHK(hipMemcpyAsync(c->ln[0].hresult, c->ln[0].result, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HCHK(hipStreamSynchronize(c->stream));
+    HCHK(ctx_sync(c));
     out[0] = beta * c->ln[0].hresult[0];
     out[1] = beta * c->ln[0].hresult[1];
     out[2] = beta * std::sqrt(c->ln[0].hresult[2]);
@@ -953,7 +968,7 @@ ns3d_status ns3d_compute_div(ns3d_ctx* c, int vel_set, double out[4]) {
     HCHK(launch_compute_div(c->geo(), c->sys[3].view(), v.u, v.v, v.w, c->prm.dx, c->prm.dy, c->prm.dz,
                             c->ln[0].partials, c->ln[0].result, c->stream));
     HCHK(hipMemcpyAsync(c->ln[0].hresult, c->ln[0].result, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HCHK(hipStreamSynchronize(c->stream));
+    HCHK(ctx_sync(c));
     out[0] = c->ln[0].hresult[0];
     out[1] = std::sqrt(c->ln[0].hresult[1]);
     out[2] = c->ln[0].hresult[2] / (double)c->sys[3].n;
@@ -967,7 +982,7 @@ ns3d_status ns3d_compute_permeability(ns3d_ctx* c, double out[5]) {
     const int border = c->prm.border;
     HCHK(launch_permeability(c->geo(), c->vel[NS3D_SET_VEL][0], c->p, border, c->ln[0].partials, c->ln[0].result, c->stream));
     HCHK(hipMemcpyAsync(c->ln[0].hresult, c->ln[0].result, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HCHK(hipStreamSynchronize(c->stream));
+    HCHK(ctx_sync(c));
     const double* r = c->ln[0].hresult;
     double vm = r[0];
     double p_in, p_out;
@@ -999,7 +1014,7 @@ ns3d_status ns3d_get_matrix(ns3d_ctx* c, int sys, double* diag, double* off, int
     if (off) HCHK(hipMemcpyAsync(off, S.coef + n, 6 * n * 8, hipMemcpyDeviceToHost, s));
     if (nbr) HCHK(hipMemcpyAsync(nbr, S.nbr, 6 * n * 4, hipMemcpyDeviceToHost, s));
     if (rowsum && S.rowsum) HCHK(hipMemcpyAsync(rowsum, S.rowsum, n * 8, hipMemcpyDeviceToHost, s));
-    HCHK(hipStreamSynchronize(s));
+    HCHK(ctx_sync(c));
   });
 }
 
@@ -1013,6 +1028,8 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
   ns3d_status st = guarded([&] {
     HCHK(hipGetDevice(&c->device));
     HCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = c->stream;
+    HCHK(hipEventCreateWithFlags(&c->ev_sync, hipEventDisableTiming));
     HCHK(hipEventCreate(&c->ev[0]));
     HCHK(hipEventCreate(&c->ev[1]));
     hipStream_t s = c->stream;
@@ -1034,7 +1051,7 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
     c->nlanes = 1;
     bicgstab(c, 3, dx, db, tol, maxit, info);
     HCHK(hipMemcpyAsync(x, dx, (size_t)n * 8, hipMemcpyDeviceToHost, s));
-    HCHK(hipStreamSynchronize(s));
+    HCHK(ctx_sync(c));
   });
   ns3d_destroy(c);
   return st;
@@ -1077,7 +1094,7 @@ ns3d_status ns3d_set_precision(ns3d_ctx* c, int mode, double inner_tol) {
     c->precision = mode;
     if (mode == NS3D_PRECISION_MIXED) c->inner_tol = inner_tol;
     for (int q = 0; q < 4; ++q) refresh_f32(c, q);
-    HCHK(hipStreamSynchronize(c->stream));
+    HCHK(ctx_sync(c));
   });
 }
 
@@ -1116,7 +1133,7 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* c, ns3d_engine* e) {
     const ns3d_status st = guarded([&] {
       for (int q = c->nlanes; q < 3; ++q) alloc_lane(c, c->ln[q]);
       c->nlanes = 3;
-      HCHK(hipStreamSynchronize(c->stream));
+      HCHK(ctx_sync(c));
     });
     if (st != NS3D_OK) return st;
   }
@@ -1125,6 +1142,11 @@ ns3d_status ns3d_attach_engine(ns3d_ctx* c, ns3d_engine* e) {
       c->engine->fence();
       c->engine->attached.fetch_sub(1);
     }
+    // the context's work moves to the engine's context stream (or back to
+    // its own) once everything queued so far is done
+    const ns3d_status st = guarded([&] { HCHK(ctx_sync(c)); });
+    if (st != NS3D_OK) return st;
+    c->stream = (e && e->ctx_stream) ? e->ctx_stream : c->own_stream;
     if (e) e->attached.fetch_add(1);
     c->engine = e;
     c->engine_used = false;
@@ -1190,7 +1212,7 @@ ns3d_status ns3d_ctx_bytes(ns3d_ctx* c, double* bytes) {
 
 ns3d_status ns3d_sync(ns3d_ctx* c) {
   if (!c) return fail_invalid("ns3d_sync: null context");
-  return guarded([&] { HCHK(hipStreamSynchronize(c->stream)); });
+  return guarded([&] { HCHK(ctx_sync(c)); });
 }
 
 }  // extern "C"

@@ -47,6 +47,19 @@ struct EngineRequest {
   bool restart_queued = false;
 };
 
+// Pinned host memory (hipHostMalloc) for the tables the dispatcher uploads:
+// a copy from it is a plain asynchronous DMA, where a copy from pageable
+// memory is staged and returns only once the stream has reached it.  Grows
+// on demand (dropping its contents), which callers do only when no copy from
+// it can still be in flight.
+struct HostBuf {
+  char* p = nullptr;
+  size_t cap = 0, size = 0;
+  void reserve(size_t n);
+  void release();
+  ~HostBuf() { release(); }
+};
+
 }  // namespace ns3d
 
 struct ns3d_engine {
@@ -82,6 +95,12 @@ struct ns3d_engine {
   // with only one, its solves run on its own stream instead (nothing to
   // batch with, and the single-solve launches are shorter)
   std::atomic<int> attached{0};
+  // One stream for the work of every attached context (assembly, residuals,
+  // corrections), created before the lanes' streams so that it has a HIP
+  // hardware queue of its own: a context's stream that shared a queue with
+  // a lane would wait behind that lane's queued chunks at every
+  // synchronisation.  Null with NS3D_CTX_STREAM=0 (each context keeps its own).
+  hipStream_t ctx_stream = nullptr;
 
  private:
   struct Group {
@@ -96,8 +115,8 @@ struct ns3d_engine {
     // reads a changed buffer (two of each: a round is queued while the
     // previous one is in flight).  The tile table carries each solve's
     // iteration base and is uploaded every round.
-    std::vector<char> h_desc[2], h_idesc[2];
-    std::vector<char> h_tiles[2], h_itiles[2];
+    ns3d::HostBuf h_desc[2], h_idesc[2];
+    ns3d::HostBuf h_tiles[2], h_itiles[2];
     std::vector<ns3d::Tile> tiles;  // current tiles (rebuilt when the active set changes)
     std::vector<char> descs;        // current descriptors, one per active solve
     int ntiles = 0;
@@ -147,7 +166,7 @@ struct ns3d_engine {
   // per round buffer: state pointers of the round's solves, gathered states,
   // the solves in gather order, the round's completion and sampled events
   ns3d::BState** d_stptr[2] = {};
-  std::vector<ns3d::BState*> h_stptr[2];
+  ns3d::HostBuf h_stptr[2];
   ns3d::BState* d_states[2] = {};
   ns3d::BState* h_states[2] = {};
   // (serials of the round's solves: a solve retired by the previous round's

@@ -75,8 +75,13 @@ SolveDesc<float>& desc_of<float>(EngineRequest* r) { return r->d32; }
 
 }  // namespace
 
+#ifndef NS3D_CTX_STREAM
+#define NS3D_CTX_STREAM 1
+#endif
+
 ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) {
   ECHK(hipSetDevice(device));
+  if (NS3D_CTX_STREAM) ECHK(hipStreamCreateWithFlags(&ctx_stream, hipStreamNonBlocking));
   // streams of the lanes in use only (each takes one of the process's HIP
   // hardware queues); more are created if the schedule asks for them
   for (int l = 0; l < nlanes; ++l) ECHK(hipStreamCreateWithFlags(&lane_stream[l], hipStreamNonBlocking));
@@ -97,14 +102,14 @@ ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) 
     g.d_tiles = tb + kTableHeader;
     g.d_itiles = (char*)dalloc(kTableHeader + tsz * kEngMaxTiles) + kTableHeader;
     for (int b = 0; b < 2; ++b) {
-      g.h_desc[b].resize(dsz * kEngMaxSolves);
-      g.h_idesc[b].resize(dsz * kEngMaxSolves);
+      g.h_desc[b].reserve(dsz * 64);
+      g.h_idesc[b].reserve(dsz * 64);
     }
   }
   for (int b = 0; b < 2; ++b) {
     d_stptr[b] = (BState**)dalloc(sizeof(BState*) * kGroups * kEngMaxSolves);
     d_states[b] = (BState*)dalloc(sizeof(BState) * kGroups * kEngMaxSolves);
-    h_stptr[b].resize(kGroups * kEngMaxSolves);
+    h_stptr[b].reserve(sizeof(BState*) * kGroups * kEngMaxSolves);
     ECHK(hipHostMalloc((void**)&h_states[b], sizeof(BState) * kGroups * kEngMaxSolves, hipHostMallocDefault));
     for (auto& e : lane_done[b]) ECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : tev[b]) ECHK(hipEventCreate(&e));
@@ -133,6 +138,24 @@ ns3d_engine::~ns3d_engine() {
   for (void* p : allocs) (void)hipFree(p);
   for (auto& st : lane_stream)
     if (st) (void)hipStreamDestroy(st);
+  if (ctx_stream) {
+    (void)hipStreamSynchronize(ctx_stream);
+    (void)hipStreamDestroy(ctx_stream);
+  }
+}
+
+void HostBuf::reserve(size_t n) {
+  if (n <= cap) return;
+  release();
+  const size_t c = std::max(n, cap + cap / 2);
+  ECHK(hipHostMalloc((void**)&p, c, hipHostMallocDefault));
+  cap = c;
+}
+
+void HostBuf::release() {
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = size = 0;
 }
 
 void ns3d_engine::set_graphs(bool on) { graphs = on; }
@@ -240,11 +263,13 @@ Batch<T> ns3d_engine::batch(int g, int q) const {
 // after the one that last used them has been processed, so no copy still in
 // flight reads a buffer that changed.
 template <typename T>
-static void tile_table(std::vector<char>& img, const std::vector<Tile>& tiles, const SolveDesc<T>* desc,
+static void tile_table(HostBuf& img, const std::vector<Tile>& tiles, const SolveDesc<T>* desc,
                        const std::vector<EngineRequest*>* reqs) {
-  img.assign(kTableHeader + tiles.size() * sizeof(TileDesc<T>), 0);
-  *(int*)img.data() = (int)tiles.size();
-  TileDesc<T>* e = (TileDesc<T>*)(img.data() + kTableHeader);
+  img.reserve(kTableHeader + tiles.size() * sizeof(TileDesc<T>));
+  img.size = kTableHeader + tiles.size() * sizeof(TileDesc<T>);
+  std::memset(img.p, 0, kTableHeader);
+  *(int*)img.p = (int)tiles.size();
+  TileDesc<T>* e = (TileDesc<T>*)(img.p + kTableHeader);
   for (size_t i = 0; i < tiles.size(); ++i) {
     const int s = tiles[i].solve;
     e[i].d = desc[s];
@@ -276,7 +301,10 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
       if (r->group == g && r->ready) ECHK(hipStreamWaitEvent(gs, r->ready, 0));
     auto init = [&](auto tag) {
       using T = decltype(tag);
-      SolveDesc<T>* hd = (SolveDesc<T>*)G.h_idesc[buf].data();
+      int nnew = 0;
+      for (EngineRequest* r : reqs) nnew += r->group == g;
+      G.h_idesc[buf].reserve(std::max(nnew, 1) * sizeof(SolveDesc<T>));
+      SolveDesc<T>* hd = (SolveDesc<T>*)G.h_idesc[buf].p;
       for (EngineRequest* r : reqs) {
         if (r->group != g) continue;
         SolveDesc<T> d = desc_of<T>(r);
@@ -286,7 +314,7 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
       if (ns == 0) return;
       tile_table(G.h_itiles[buf], tiles, hd, nullptr);
       ECHK(hipMemcpyAsync(G.d_idesc, hd, ns * sizeof(SolveDesc<T>), hipMemcpyHostToDevice, gs));
-      ECHK(hipMemcpyAsync((char*)G.d_itiles - kTableHeader, G.h_itiles[buf].data(), G.h_itiles[buf].size(),
+      ECHK(hipMemcpyAsync((char*)G.d_itiles - kTableHeader, G.h_itiles[buf].p, G.h_itiles[buf].size,
                           hipMemcpyHostToDevice, gs));
       Batch<T> B{};
       B.d = (const SolveDesc<T>*)G.d_idesc;
@@ -328,7 +356,7 @@ void ns3d_engine::upload_tiles(int g, int buf, bool restarted) {
   };
   if (g / kMaxLanes == 0) up(double{});
   else up(float{});
-  ECHK(hipMemcpyAsync(G.d_nt, G.h_tiles[buf].data(), G.h_tiles[buf].size(), hipMemcpyHostToDevice,
+  ECHK(hipMemcpyAsync(G.d_nt, G.h_tiles[buf].p, G.h_tiles[buf].size, hipMemcpyHostToDevice,
                       lane_stream[g % kMaxLanes]));
 }
 
@@ -348,7 +376,8 @@ void ns3d_engine::rebuild(int g, int buf) {
     }
     G.ntiles = (int)G.tiles.size();
     if (G.active.empty()) return;
-    SolveDesc<T>* hd = (SolveDesc<T>*)G.h_desc[buf].data();
+    G.h_desc[buf].reserve(G.active.size() * sizeof(SolveDesc<T>));
+    SolveDesc<T>* hd = (SolveDesc<T>*)G.h_desc[buf].p;
     std::copy(cur, cur + G.active.size(), hd);
     ECHK(hipMemcpyAsync(G.d_desc, hd, G.active.size() * sizeof(SolveDesc<T>), hipMemcpyHostToDevice,
                         lane_stream[g % kMaxLanes]));
@@ -383,12 +412,11 @@ void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
     off[g] = nact;
     for (EngineRequest* r : grp[g].active) {
       ord.push_back(RoundEntry{r->serial, r->it_queued, g, r->n, r->nnz, r->tiles});
-      h_stptr[buf][nact++] = r->prec == 0 ? r->d64.w.st : r->d32.w.st;
+      ((BState**)h_stptr[buf].p)[nact++] = r->prec == 0 ? r->d64.w.st : r->d32.w.st;
     }
-    // the group's state pointers, on its lane (pageable source: staged
-    // before the call returns)
+    // the group's state pointers, on its lane
     if (nact > off[g])
-      ECHK(hipMemcpyAsync(d_stptr[buf] + off[g], h_stptr[buf].data() + off[g], (nact - off[g]) * sizeof(BState*),
+      ECHK(hipMemcpyAsync(d_stptr[buf] + off[g], (BState**)h_stptr[buf].p + off[g], (nact - off[g]) * sizeof(BState*),
                           hipMemcpyHostToDevice, lane_stream[g % kMaxLanes]));
   }
   off[kGroups] = nact;
