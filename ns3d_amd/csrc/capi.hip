@@ -726,7 +726,11 @@ void ns3d_destroy(ns3d_ctx* c) {
     c->engine->fence();
     c->engine->attached.fetch_sub(1);
   }
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  // its own work done (an attached context's stream is shared)
+  if (c->stream) {
+    if (c->ev_sync) (void)ctx_sync(c);
+    else (void)hipStreamSynchronize(c->stream);
+  }
   c->stream = c->own_stream;
   for (void* p : c->allocs) (void)hipFree(p);
   for (Lane& L : c->ln) {
