@@ -90,6 +90,36 @@ def test_schedules_give_the_same_bits():
         G.Engine(device=0, chunk=4, lanes=5)
 
 
+def test_lone_context_solves_on_its_own_stream():
+    """A context alone on its engine solves on the stream its work runs on,
+    leaving the engine idle; with a second context attached, the same run goes
+    through the engine.  Both give the solo result bit for bit (mixed form:
+    FP64 defect correction around FP32 inner solves)."""
+    img = generators.rsa_spheres((18, 16, 14), 3.0, 0.3, 7)[0]
+    p = dict(PRM, method="SIMPLEC", gradp=1.0 / 18, precision=1, inner_tol=1e-4)
+    ref_rows, ref = solver.solve(img, **p)
+    eng = G.Engine(device=0, chunk=4)
+    try:
+        s = solver.Solver(img, **p)
+        s.attach(eng)
+        rows, summ = s.run()
+        s.close()
+        assert eng.stats()["solves"] == 0
+        assert summ["K"] == ref["K"] and np.array_equal(rows, ref_rows)
+        other = solver.Solver(img, **p)
+        other.attach(eng)
+        s = solver.Solver(img, **p)
+        s.attach(eng)
+        rows, summ = s.run()
+        s.close()
+        other.close()
+        assert eng.stats()["solves"] > 0
+        assert summ["K"] == ref["K"] and np.array_equal(rows, ref_rows)
+        assert summ["solver_iterations"] == ref["solver_iterations"]
+    finally:
+        eng.close()
+
+
 def test_engine_lifecycle_and_detach():
     """A context detached from an engine solves on its own stream again; an
     engine with nothing attached shuts down cleanly; bad arguments fail."""
