@@ -198,7 +198,9 @@ def mix_section(args):
             p = mix_params(img, args, 1)
             p["maxitSIMPLE"] = args.tune_maxit
             sample.append((img, p))
-        tuned = autotune.tune(sample, candidates=[(c, l, True) for c in (6, 8, 12) for l in (1, 2, 3)] + [(8, 2, False)],
+        # chunks of 12 and one lane never won a full pass; the capped sample
+        # cannot see the burst's tail, where shorter chunks help
+        tuned = autotune.tune(sample, candidates=[(c, l, True) for c in (4, 6, 8) for l in (2, 3)] + [(6, 3, False)],
                               log=log, budget_gb=args.budget_gb, slots=args.slots)
         args.chunk, args.lanes, args.no_graphs = tuned["chunk"], tuned["lanes"], not tuned["graphs"]
         log(f"  chosen: chunk {args.chunk}, lanes {args.lanes}, graphs {int(not args.no_graphs)}")

@@ -47,6 +47,10 @@ struct SysMem {
   double* rowsum = nullptr;
   int64_t last_iters = 16;   // chunk-size hint for the next solve (plain)
   int64_t last_iters32 = 16; // same for the FP32 inner solves (mixed)
+  // mixed form: FP32 sweeps on this system once failed to reduce the
+  // residual by even the inner tolerance before an FP64 finish was needed;
+  // its later solves are plain FP64
+  bool fp32_stalls = false;
   int64_t nnz = 0;           // stored entries (diagonal + existing neighbours)
   float* coef32 = nullptr;   // mixed form: FP32 copies
   float* invdiag32 = nullptr;
@@ -371,7 +375,15 @@ void bicgstab64_many(ns3d_ctx* c, int n, const int* sid, double* const* x, const
 //   tolerance max(inner_tol, tol |b| / (2 |r|)); x += d; r = b - A x.
 // A sweep that does nothing, or after which |r| has not halved, means FP32
 // has reached its accuracy on this system: the solve is finished in FP64
-// from the current iterate.  iterations = summed inner iterations; error =
+// from the current iterate.  A sweep that breaks down or makes |r| larger is
+// not applied (it is taken back), and the FP32 sweeps together use at most
+// half of maxit, so the FP64 finish always has the rest: on a large,
+// ill-conditioned system FP32 BiCGSTAB's true residual stalls far above its
+// recursive one.  When the FP32 sweeps of a solve used up their half of the
+// budget or did not reduce |r| by even the inner tolerance before the FP64
+// finish, or the finish took more than half as many iterations as those
+// sweeps, FP32 does not carry that system: its later solves are plain FP64
+// (mixed_ready).  iterations = summed inner iterations; error =
 // |b - A x| / |b| (true FP64 residual); a zero right-hand side and an
 // already-converged guess behave as in the plain solve (x = 0 with
 // iterations = maxit, and 0 iterations).  The n solves advance in lockstep
@@ -381,13 +393,16 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
                          int64_t maxit, ns3d_solve_info* info, Lane* const* L) {
   hipStream_t s = c->stream;
   struct Track {
-    double rr = 0, bb = 0, tol2 = 0, rr_old = 0, err = 0;
-    int64_t total = 0;
+    double rr = 0, bb = 0, tol2 = 0, rr_old = 0, err = 0, rr0 = 0;
+    int64_t total = 0, total32 = 0;
     int outer = 0;
     bool live = false, stagnated = false;
   };
   Track t[3];
   bool use[3] = {false, false, false};
+  // the FP32 sweeps get at most half the iteration budget, so that an FP64
+  // finish always has the other half
+  const int64_t budget32 = std::max<int64_t>(1, maxit / 2);
   auto residuals = [&](const bool* which) {
     bool any = false;
     for (int i = 0; i < n; ++i) {
@@ -421,6 +436,7 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
       continue;
     }
     t[i].tol2 = tol * tol * t[i].bb;
+    t[i].rr0 = t[i].rr;
     t[i].live = true;
   }
   for (;;) {
@@ -430,7 +446,9 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
     for (int i = 0; i < n; ++i) {
       Track& k = t[i];
       if (!use[i] || !k.live) continue;
-      if (!(k.rr > k.tol2 && k.total < maxit && k.outer < 100)) {
+      if (!(k.rr > k.tol2 && k.total < budget32 && k.outer < 100)) {
+        // out of FP32 budget or sweeps short of the tolerance: FP64 finishes
+        if (k.rr > k.tol2) k.stagnated = true;
         k.live = false;
         continue;
       }
@@ -438,7 +456,7 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
       const double rel = std::sqrt(k.rr / k.bb);
       const double tin = std::max(c->inner_tol, 0.5 * tol / rel);
       HCHK(launch_fill_f32(L[i]->d32, S.n, s));
-      jobs[m] = SolveJob<float>{sid[i], S.mat32(), L[i]->d32, L[i]->r32, L[i]->bw32, tin, maxit - k.total, L[i],
+      jobs[m] = SolveJob<float>{sid[i], S.mat32(), L[i]->d32, L[i]->r32, L[i]->bw32, tin, budget32 - k.total, L[i],
                                 &S.last_iters32};
       idx[m++] = i;
     }
@@ -453,7 +471,9 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
       info[i].restarts += (int32_t)st.restarts;
       k.total += its;
       ++k.outer;
-      if (its == 0) {  // the FP32 sweep cannot act on this residual
+      // no iteration, or a sweep that broke down (non-finite residual):
+      // its correction is not applied
+      if (its == 0 || !std::isfinite(st.rr)) {
         k.stagnated = true;
         k.live = false;
         continue;
@@ -463,11 +483,21 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
       again[i] = true;
     }
     residuals(again);
-    for (int i = 0; i < n; ++i)
-      if (again[i] && t[i].rr > 0.25 * t[i].rr_old) {  // |r| fell by less than 2x
-        t[i].stagnated = true;
-        t[i].live = false;
+    for (int i = 0; i < n; ++i) {
+      if (!again[i]) continue;
+      Track& k = t[i];
+      if (!(k.rr <= k.rr_old)) {
+        // the correction made |r| larger (FP32 sweeps past their accuracy on
+        // an ill-conditioned system): take it back
+        HCHK(launch_add_f32(x[i], L[i]->d32, c->sys[sid[i]].n, s, -1.0));
+        k.rr = k.rr_old;
+        k.stagnated = true;
+        k.live = false;
+      } else if (k.rr > 0.25 * k.rr_old) {  // |r| fell by less than 2x
+        k.stagnated = true;
+        k.live = false;
       }
+    }
   }
   // finish in FP64 from the current iterate (same tolerance, the remaining
   // iteration budget) where FP32 stagnated short of the tolerance
@@ -480,6 +510,8 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
     k.err = std::sqrt(k.rr / k.bb);
     if (k.stagnated && k.rr > k.tol2 && k.total < maxit) {
       SysMem& S = c->sys[sid[i]];
+      if (!(k.rr <= c->inner_tol * c->inner_tol * k.rr0) || k.total >= budget32) S.fp32_stalls = true;
+      k.total32 = k.total;
       fin[m] = SolveJob<double>{sid[i], S.mat64(), x[i], b[i], L[i]->bw, tol, maxit - k.total, L[i], &S.last_iters};
       idx[m++] = i;
     }
@@ -491,6 +523,9 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
     t[i].total += st.it_c;
     info[i].restarts += (int32_t)st.restarts;
     t[i].err = std::sqrt(st.rr / st.rhs_sqnorm);
+    // an FP64 finish longer than half the FP32 sweeps before it: FP32 did
+    // not carry this system
+    if (2 * st.it_c > t[i].total32) c->sys[sid[i]].fp32_stalls = true;
     c->fp64_finishes += 1;
   }
   for (int i = 0; i < n; ++i) {
@@ -506,7 +541,7 @@ bool mixed_ready(const ns3d_ctx* c, const int* sid, int n) {
   if (c->precision != NS3D_PRECISION_MIXED) return false;
   for (int i = 0; i < n; ++i) {
     const SysMem& S = c->sys[sid[i]];
-    if (S.n > 0 && (!S.coef32 || S.n < c->mixed_min_rows)) return false;
+    if (S.n > 0 && (!S.coef32 || S.n < c->mixed_min_rows || S.fp32_stalls)) return false;
   }
   return true;
 }

@@ -209,7 +209,8 @@ hipError_t launch_coef_to_f32(const double* coef, const double* invdiag, int64_t
 hipError_t launch_residual64(SMat<double> A, const double* x, const double* b, double* r, float* r32,
                              double* partials, double* result, hipStream_t s);
 // x += (double) d
-hipError_t launch_add_f32(double* x, const float* d, int64_t n, hipStream_t s);
+// x += a d (a = 1: the correction; a = -1: taking it back)
+hipError_t launch_add_f32(double* x, const float* d, int64_t n, hipStream_t s, double a = 1.0);
 hipError_t launch_fill_f32(float* x, int64_t n, hipStream_t s);
 
 // ---- fields, corrections, diagnostics (kernels_field.hip) -------------------

@@ -692,9 +692,9 @@ sums_fin(const double* partials, int nb, double* result) {
 }
 
 __global__ void __launch_bounds__(kBlock)
-add_f32_kernel(double* __restrict__ x, const float* __restrict__ d, int64_t n) {
+add_f32_kernel(double* __restrict__ x, const float* __restrict__ d, int64_t n, double a) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) x[i] += (double)d[i];
+  if (i < n) x[i] += a * (double)d[i];
 }
 
 __global__ void __launch_bounds__(kBlock) fill_f32_kernel(float* __restrict__ x, int64_t n) {
@@ -835,9 +835,9 @@ hipError_t launch_residual64(SMat<double> A, const double* x, const double* b, d
   return hipGetLastError();
 }
 
-hipError_t launch_add_f32(double* x, const float* d, int64_t n, hipStream_t s) {
+hipError_t launch_add_f32(double* x, const float* d, int64_t n, hipStream_t s, double a) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(add_f32_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, x, d, n);
+  hipLaunchKernelGGL(add_f32_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, x, d, n, a);
   return hipGetLastError();
 }
 
