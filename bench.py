@@ -273,9 +273,17 @@ def mix_section(args):
 def single_workload(args):
     from ns3d_amd import generators
     n = args.size
-    img, solid, overlap, nsph = generators.rsa_spheres((n, n, n), args.radius, 1.0 - args.porosity, args.seed)
     params = dict(dx=1.0, mu=1.0, rho=1.0, p0=1.0, gradp=1.0 / n, alpha=0.8, beta=1.0, direction=1, border=0,
                   **TOLS)
+    if getattr(args, "generator", "rsa") == "boolean":
+        # the precision configuration's image: overlapping spheres, radii U[rmin, rmax]
+        img = generators.boolean_spheres((n, n, n), args.rmin, args.rmax, args.porosity, args.seed)[0]
+        por = 1.0 - float(img.mean())
+        desc = dict(workload=f"{n}^3 Boolean spheres r=U[{args.rmin:g},{args.rmax:g}] porosity {por:.4f}, SIMPLEC, "
+                             f"flow x", image=dict(shape=[n, n, n], rmin=args.rmin, rmax=args.rmax, seed=args.seed,
+                                                   porosity=round(por, 6)))
+        return img, params, desc
+    img, solid, overlap, nsph = generators.rsa_spheres((n, n, n), args.radius, 1.0 - args.porosity, args.seed)
     desc = dict(workload=f"{n}^3 RSA sphere pack r={args.radius:g} porosity {1 - solid:.4f}, SIMPLEC, flow x",
                 image=dict(shape=[n, n, n], radius=args.radius, seed=args.seed, porosity=round(1 - solid, 6),
                            rsa_overlap_voxels=overlap, spheres=nsph))
@@ -334,7 +342,8 @@ def measured_traffic(args, kernel):
     kernel) recorded with rocprofv3 on the 128^3 image, from
     profiles/hbm_traffic_128rsa.json; None for another image."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_128rsa.json")
-    default = (args.size, args.radius, args.porosity, args.seed) == (128, 10.0, 0.75, 20261203)
+    default = (getattr(args, "generator", "rsa"), args.size, args.radius, args.porosity, args.seed) == \
+        ("rsa", 128, 10.0, 0.75, 20261203)
     if not default or not os.path.exists(path):
         return None, None
     k = json.load(open(path))["kernels"].get(kernel)
@@ -364,7 +373,7 @@ def mix_traffic(args):
 def single_section(args):
     img, params, desc = single_workload(args)
     log(f"single image: {desc['workload']}")
-    f64 = single_form(img, params, args, 0, "single fp64", args.single_steps, 1)
+    f64 = single_form(img, params, args, 0, "single fp64", args.single_steps, args.single_warmup)
     mx = single_form(img, params, args, 1, "single mixed", args.single_steps, 0)
     roof = None
     if ("f64", "pressure", "A") in f64["tot"]:
@@ -375,7 +384,7 @@ def single_section(args):
             roof = dict(bound="hbm", achieved=round(ach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
                         frac=round(ach / PEAK_HBM_GBS, 4), traffic=traffic,
                         kernel="bicg_A_kernel<double> (pressure system, packed symmetric storage): p update + "
-                               "7-point SpMV + r0.v, 128^3 image, plain FP64 path",
+                               f"7-point SpMV + r0.v, {args.size}^3 image, plain FP64 path",
                         bytes_per_launch=round(nbytes / launches), avg_launch_us=round(1e3 * ms / launches, 3),
                         traffic_source=tsrc)
     kernels = dict(f64["kernels"])
@@ -456,7 +465,12 @@ def main():
     ap.add_argument("--radius", type=float, default=10.0)
     ap.add_argument("--porosity", type=float, default=0.75)
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--generator", default="rsa", choices=["rsa", "boolean"],
+                    help="single image: RSA pack (--radius) or Boolean spheres (--rmin, --rmax)")
+    ap.add_argument("--rmin", type=float, default=6.0)
+    ap.add_argument("--rmax", type=float, default=20.0)
     ap.add_argument("--single-steps", type=int, default=1)
+    ap.add_argument("--single-warmup", type=int, default=1, help="untimed FP64 solves before the timed ones")
     ap.add_argument("--simpletol", type=float, default=1e-6)
     ap.add_argument("--maxit", type=int, default=3000, help="outer-iteration cap, single image")
     ap.add_argument("--mix-maxit", type=int, default=1000,
@@ -521,7 +535,7 @@ def main():
     else:
         s = single["out"]
         line = {"metric": METRIC, "value": s["mixed"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
-                "steps": args.single_steps, "warmup": 1, "ms_per_step": s["mixed"]["ms_per_solve"],
+                "steps": args.single_steps, "warmup": args.single_warmup, "ms_per_step": s["mixed"]["ms_per_solve"],
                 "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                 "dtype": "f64 arithmetic, f32 inner-solve storage", "data": "synthetic", "form": "single_mixed",
                 "config": {"workload": s["workload"]}}
