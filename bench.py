@@ -498,8 +498,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU runs are not part of "
-              f"this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); it has no multi-GPU mode: requests "
+              f"would shard across GPUs by image, one server per GPU", file=sys.stderr)
         sys.exit(2)
     from ns3d_amd import gpu
     if gpu.device_count() < 1:
