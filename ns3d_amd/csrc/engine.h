@@ -14,6 +14,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "kernels.h"
@@ -45,6 +46,8 @@ struct EngineRequest {
   long long it_done = 0;  // executed iterations as last read back
   long long it_queued = -1;
   bool restart_queued = false;
+  // the condition variable of the solve_many call waiting for this solve
+  std::condition_variable* waiter = nullptr;
 };
 
 // Pinned host memory (hipHostMalloc) for the tables the dispatcher uploads:
@@ -158,7 +161,12 @@ struct ns3d_engine {
   hipStream_t stream = nullptr;  // lane 0's: gathers and completion events
   std::thread thr;
   std::mutex mu;
+  // cv_work: the dispatcher waits for work; cv_done: fence() callers wait
+  // for rounds (each solve_many call waits on a condition variable of its
+  // own, woken only when one of its solves stops or the engine fails)
   std::condition_variable cv_work, cv_done;
+  std::atomic<int> fence_waiters{0};
+  std::unordered_set<std::condition_variable*> waiters;
   std::deque<ns3d::EngineRequest*> pending;
   bool stopping = false;
   std::string fatal;

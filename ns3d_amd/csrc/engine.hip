@@ -189,24 +189,31 @@ void ns3d_engine::solve_many(EngineRequest* const* reqs, int n) {
   std::unique_lock<std::mutex> lk(mu);
   if (!fatal.empty()) throw std::runtime_error("solve engine failed: " + fatal);
   if (stopping) throw std::runtime_error("solve engine is shutting down");
+  std::condition_variable cv;
   for (int i = 0; i < n; ++i) {
     reqs[i]->done = false;
+    reqs[i]->waiter = &cv;
     pending.push_back(reqs[i]);
   }
+  waiters.insert(&cv);
   cv_work.notify_one();
   auto all_done = [&] {
     for (int i = 0; i < n; ++i)
       if (!reqs[i]->done) return false;
     return true;
   };
-  cv_done.wait(lk, [&] { return all_done() || !fatal.empty(); });
+  cv.wait(lk, [&] { return all_done() || !fatal.empty(); });
+  waiters.erase(&cv);
+  for (int i = 0; i < n; ++i) reqs[i]->waiter = nullptr;
   if (!all_done()) throw std::runtime_error("solve engine failed: " + fatal);
 }
 
 void ns3d_engine::fence() {
   std::unique_lock<std::mutex> lk(mu);
   const long long target = rounds_queued;
+  fence_waiters.fetch_add(1);
   cv_done.wait(lk, [&] { return rounds_done >= target || !fatal.empty(); });
+  fence_waiters.fetch_sub(1);
 }
 
 void ns3d_engine::stats(double* out, int count) {
@@ -238,6 +245,7 @@ void ns3d_engine::wait_round(int buf) {
 void ns3d_engine::fail(const std::string& msg) {
   std::lock_guard<std::mutex> lk(mu);
   fatal = msg;
+  for (std::condition_variable* w : waiters) w->notify_one();
   cv_done.notify_all();
 }
 
@@ -621,6 +629,7 @@ bool ns3d_engine::process_round(int buf, bool restart[kGroups]) {
     live.erase(lv);
     r->result = st;
     r->done = true;
+    if (r->waiter) r->waiter->notify_one();  // under mu: the waiting call (and its cv) cannot have returned
     rows_x_iters += (double)r->n * (double)st.it_c;
     n_done += 1;
     any_done = true;
@@ -688,7 +697,8 @@ void ns3d_engine::run() {
         host_process_ms += std::chrono::duration<double, std::milli>(h3 - h2).count();
       }
       host_queue_ms += std::chrono::duration<double, std::milli>(h1 - h0).count();
-      if (woke || inflight >= 0) cv_done.notify_all();
+      (void)woke;  // process_round woke the callers of the solves it retired
+      if (inflight >= 0 && fence_waiters.load() > 0) cv_done.notify_all();
       inflight = queued ? qbuf : -1;
       if (queued) qbuf ^= 1;
     }
