@@ -171,12 +171,9 @@ struct ns3d_engine {
   bool stopping = false;
   std::string fatal;
   Group grp[kGroups];
-  // per round buffer: state pointers of the round's solves, gathered states,
-  // the solves in gather order, the round's completion and sampled events
-  ns3d::BState** d_stptr[2] = {};
-  ns3d::HostBuf h_stptr[2];
-  ns3d::BState* d_states[2] = {};
-  ns3d::BState* h_states[2] = {};
+  // per round buffer: gathered states, the solves in gather order, the
+  // round's completion and sampled events
+  ns3d::BState* h_states[2] = {};  // pinned; the lanes' gather kernels write it directly
   // (serials of the round's solves: a solve retired by the previous round's
   // results may still appear in this one, and its request is gone by then)
   struct RoundEntry {

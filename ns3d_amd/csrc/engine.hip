@@ -52,12 +52,15 @@ int graph_grid(int ntiles) {
     if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-__global__ void gather_states_kernel(BState* const* __restrict__ src, BState* __restrict__ dst, int n) {
+// The states of a group's n solves (in its descriptor table's order) into
+// dst, pinned host memory the host reads once the lane's event has fired.
+template <typename T>
+__global__ void gather_states_kernel(const SolveDesc<T>* __restrict__ desc, BState* __restrict__ dst, int n) {
   constexpr int kWords = (int)(sizeof(BState) / 8);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * kWords) return;
   const int s = i / kWords, w = i % kWords;
-  reinterpret_cast<uint64_t*>(dst + s)[w] = reinterpret_cast<const uint64_t*>(src[s])[w];
+  reinterpret_cast<uint64_t*>(dst + s)[w] = reinterpret_cast<const uint64_t*>(desc[s].w.st)[w];
 }
 
 double kernel_bytes_eng(int64_t n, int64_t nnz, int k, int vb) {
@@ -107,9 +110,6 @@ ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) 
     }
   }
   for (int b = 0; b < 2; ++b) {
-    d_stptr[b] = (BState**)dalloc(sizeof(BState*) * kGroups * kEngMaxSolves);
-    d_states[b] = (BState*)dalloc(sizeof(BState) * kGroups * kEngMaxSolves);
-    h_stptr[b].reserve(sizeof(BState*) * kGroups * kEngMaxSolves);
     ECHK(hipHostMalloc((void**)&h_states[b], sizeof(BState) * kGroups * kEngMaxSolves, hipHostMallocDefault));
     for (auto& e : lane_done[b]) ECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : tev[b]) ECHK(hipEventCreate(&e));
@@ -420,12 +420,8 @@ void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
     off[g] = nact;
     for (EngineRequest* r : grp[g].active) {
       ord.push_back(RoundEntry{r->serial, r->it_queued, g, r->n, r->nnz, r->tiles});
-      ((BState**)h_stptr[buf].p)[nact++] = r->prec == 0 ? r->d64.w.st : r->d32.w.st;
+      ++nact;
     }
-    // the group's state pointers, on its lane
-    if (nact > off[g])
-      ECHK(hipMemcpyAsync(d_stptr[buf] + off[g], (BState**)h_stptr[buf].p + off[g], (nact - off[g]) * sizeof(BState*),
-                          hipMemcpyHostToDevice, lane_stream[g % kMaxLanes]));
   }
   off[kGroups] = nact;
   max_active = std::max(max_active, (double)nact);
@@ -454,12 +450,16 @@ void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
       const int g = p * kMaxLanes + l;
       const int ns = off[g + 1] - off[g];
       if (!on[g] || ns == 0) continue;
+      // the group's descriptor table (uploaded by rebuild on this stream)
+      // is in active order, as the round's entries are
       const int words = ns * (int)(sizeof(BState) / 8);
-      hipLaunchKernelGGL(gather_states_kernel, dim3((words + 255) / 256), dim3(256), 0, lane_stream[l],
-                         d_stptr[buf] + off[g], d_states[buf] + off[g], ns);
+      if (p == 0)
+        hipLaunchKernelGGL(gather_states_kernel<double>, dim3((words + 255) / 256), dim3(256), 0, lane_stream[l],
+                           (const SolveDesc<double>*)grp[g].d_desc, h_states[buf] + off[g], ns);
+      else
+        hipLaunchKernelGGL(gather_states_kernel<float>, dim3((words + 255) / 256), dim3(256), 0, lane_stream[l],
+                           (const SolveDesc<float>*)grp[g].d_desc, h_states[buf] + off[g], ns);
       ECHK(hipGetLastError());
-      ECHK(hipMemcpyAsync(h_states[buf] + off[g], d_states[buf] + off[g], ns * sizeof(BState), hipMemcpyDeviceToHost,
-                          lane_stream[l]));
       lane_in[buf][l] = true;
     }
     if (lane_in[buf][l]) ECHK(hipEventRecord(lane_done[buf][l], lane_stream[l]));
