@@ -110,7 +110,9 @@ ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) 
     }
   }
   for (int b = 0; b < 2; ++b) {
-    ECHK(hipHostMalloc((void**)&h_states[b], sizeof(BState) * kGroups * kEngMaxSolves, hipHostMallocDefault));
+    // coherent: the gather kernels' stores reach host memory directly, with
+    // no dependence on a cache write-back at the end of the round
+    ECHK(hipHostMalloc((void**)&h_states[b], sizeof(BState) * kGroups * kEngMaxSolves, hipHostMallocCoherent));
     for (auto& e : lane_done[b]) ECHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : tev[b]) ECHK(hipEventCreate(&e));
   }
