@@ -65,6 +65,33 @@ def test_ini_fatal_errors(tmp_path, capfd, text, msg):
     assert msg in capfd.readouterr().err
 
 
+@pytest.mark.parametrize("text,msg", [
+    ("input = a.vti\nmu = 1e-400\n", "expects a number"),        # underflow is an error, as for std::stod
+    ("input = a.vti\nmu = 1e999\n", "expects a number"),
+    ("input = a.vti\nmaxitsolver = 99999999999\n", "expects a integer"),
+    ("input = a.vti\nn1 = 4 5\n", "expects a integer"),
+    ("input = a.vti\nmu =\n", "expects a number"),
+    ("input = #a.vti\n", "key 'input' has an empty value"),       # '#' opening the value starts a comment
+    ("input = a.vti\nmaxitSIMPLE = 0\n", "'maxitSIMPLE' must be > 0"),
+    ("input = a.vti\nsave_step = -1\n", "'save_step' must be > 0"),
+])
+def test_ini_fatal_edge_cases(tmp_path, capfd, text, msg):
+    rc, _, _ = read_ini(tmp_path, text)
+    assert rc == 1
+    assert msg in capfd.readouterr().err
+
+
+def test_ini_crlf_tabs_and_summary(tmp_path, capfd):
+    rc, p, names = read_ini(tmp_path, "\tinput\t=\tdir/x.vti\r\n  MU = 2.5\t# tab comment\r\n"
+                                      "method=Ideal\r\nn2 = +3\r\ngradp = 0x10\r\nvxfield = v#1.vti\n")
+    assert rc == 0
+    assert names[0] == "dir/x.vti" and names[2] == "v#1.vti"
+    assert p.mu == 2.5 and p.N2 == 3 and p.gradp == 16.0 and p.method == solver.METHODS["IDEAL"]
+    out = capfd.readouterr().out
+    assert "method         : IDEAL\n" in out and "mu (Pa.s)      : 2.5\n" in out
+    assert "dx, dy, dz (m) : 1, 1, 1\n" in out and out.rstrip().endswith("======================")
+
+
 def test_ini_porevalue_warning(tmp_path, capfd):
     rc, p, _ = read_ini(tmp_path, "input = a.vti\nporevalue = 7\n")
     assert rc == 0 and p.porevalue == 0
