@@ -135,6 +135,17 @@ int ns3d_server_set_policy(ns3d_server* s, int order, double max_rows);
 void ns3d_server_destroy(ns3d_server* s);
 const char* ns3d_server_last_error(void);
 
+/* Replays the outer loop's relaxation control (RelaxControl,
+ * NSBase.cxx:196-301, after the drivers' tracking of the best residual,
+ * SIMPLEC.cxx:155-166) over n relative mass residuals taken as outer
+ * iterations 0..n-1; after iteration i: alpha_out[i], beta_out[i] and
+ * events[i] (bits: 1 the residual jumped above 2.5x its best, 2 the 30-entry
+ * window was flat, 4 it was rising).  beta_cst as the drivers pass it
+ * (SIMPLERR 0, the others 1).  No GPU is used: for studying how a residual
+ * history drives the relaxation factors.  Returns 0, or 1 on bad arguments. */
+int ns3d_relax_replay(const double* rmr, int n, int beta_cst, double alpha0, double beta0, double deltaRMR,
+                      double* alpha_out, double* beta_out, int* events);
+
 /* Reads a .ini file with the reference's grammar (ReadInput,
  * navierstokes3D.cxx:266-463); the image path is copied to image (size cap).
  * Returns 0 on success, 1 on a fatal input error (message on stderr). */

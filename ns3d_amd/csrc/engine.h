@@ -132,7 +132,8 @@ struct ns3d_engine {
   // others' work.
   static constexpr int kGroups = 2 * kMaxLanes;
   void run();
-  void admit(std::vector<ns3d::EngineRequest*>& reqs, int buf);
+  // inflight: buffer of the round still in flight (-1: none)
+  void admit(std::vector<ns3d::EngineRequest*>& reqs, int buf, int inflight);
   void rebuild(int g, int buf);
   void upload_tiles(int g, int buf, bool restarted);
   // The instantiated graph of one chunk on lane `lane` for the given grids
@@ -182,6 +183,7 @@ struct ns3d_engine {
     int group;
     int64_t n, nnz;
     int tiles;
+    const void* st;  // the solve's device BState (a later solve may reuse it)
   };
   std::vector<RoundEntry> order[2];
   std::unordered_map<long long, ns3d::EngineRequest*> live;

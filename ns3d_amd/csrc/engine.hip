@@ -291,7 +291,7 @@ static void tile_table(HostBuf& img, const std::vector<Tile>& tiles, const Solve
   }
 }
 
-void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
+void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf, int inflight) {
   // lane: the one of the request's precision with fewer rows so far
   int64_t rows[kGroups];
   for (int g = 0; g < kGroups; ++g) rows[g] = group_rows(g);
@@ -302,13 +302,30 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf) {
       if (rows[g0 + l] < rows[r->group]) r->group = g0 + l;
     rows[r->group] += r->n;
   }
+  // A solve retired by the last processed round may still have kernels in
+  // the round in flight, on its old lane.  When a new solve reuses its state
+  // (the same context's next solve) on another lane, that lane first waits
+  // for the old lane's round: the stale kernels then see the old solve's
+  // final status, never the new solve's.  (On the same lane, stream order
+  // gives this already.)
+  auto stale_lane = [&](const EngineRequest* r) {
+    if (inflight < 0) return -1;
+    const void* st = r->prec == 0 ? (const void*)r->d64.w.st : (const void*)r->d32.w.st;
+    for (const RoundEntry& o : order[inflight])
+      if (o.st == st && o.group % kMaxLanes != r->group % kMaxLanes) return o.group % kMaxLanes;
+    return -1;
+  };
   for (int g = 0; g < kGroups; ++g) {
     Group& G = grp[g];
     hipStream_t gs = lane_stream[g % kMaxLanes];
     std::vector<Tile> tiles;
     int ns = 0;
-    for (EngineRequest* r : reqs)
-      if (r->group == g && r->ready) ECHK(hipStreamWaitEvent(gs, r->ready, 0));
+    for (EngineRequest* r : reqs) {
+      if (r->group != g) continue;
+      if (r->ready) ECHK(hipStreamWaitEvent(gs, r->ready, 0));
+      const int old = stale_lane(r);
+      if (old >= 0 && lane_in[inflight][old]) ECHK(hipStreamWaitEvent(gs, lane_done[inflight][old], 0));
+    }
     auto init = [&](auto tag) {
       using T = decltype(tag);
       int nnew = 0;
@@ -421,7 +438,8 @@ void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
   for (int g = 0; g < kGroups; ++g) {
     off[g] = nact;
     for (EngineRequest* r : grp[g].active) {
-      ord.push_back(RoundEntry{r->serial, r->it_queued, g, r->n, r->nnz, r->tiles});
+      const void* st = g / kMaxLanes == 0 ? (const void*)r->d64.w.st : (const void*)r->d32.w.st;
+      ord.push_back(RoundEntry{r->serial, r->it_queued, g, r->n, r->nnz, r->tiles, st});
       ++nact;
     }
   }
@@ -683,7 +701,7 @@ void ns3d_engine::run() {
       }
       bool queued = false;
       const auto h0 = std::chrono::steady_clock::now();
-      if (!admitted.empty()) admit(admitted, qbuf);
+      if (!admitted.empty()) admit(admitted, qbuf, inflight);
       if (any_active()) {
         queue_round(qbuf, restart);
         queued = true;

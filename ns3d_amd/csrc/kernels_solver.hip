@@ -434,11 +434,15 @@ bicg_A_kernel(Batch<T> B) {
   tile_of(B, t, tl, D, it);
   NS3D_GLOBAL BState* st = D.w.st;
   // the previous kernel's partials are loaded together with the status;
-  // launches queued after the solve stopped exit here, before any row load
-  const int status = st->status;
-  const long long done = st->it_c;
+  // launches queued after the solve stopped exit here, before any row load.
+  // The leader may store DONE / RESTART while other blocks of this launch
+  // start, so the block takes thread 0's reading: every wave of a block then
+  // makes the same decision (and the block reduction below never reads the
+  // LDS slot of a wave that left).
+  __shared__ int go;
   double pa[2];
   partials_load<2>(D.slot(kSlotC), D.t1 - D.t0, D.pstride, pa);
+  if (threadIdx.x == 0) go = st->status == BS_RUNNING && st->it_c == it;
   const bool lead = leader_of(t, D.t0);
   const int sl = (int)(it & 1);
   const int par = sl;
@@ -448,7 +452,8 @@ bicg_A_kernel(Batch<T> B) {
   const NS3D_GLOBAL T* __restrict__ vold = par ? D.w.v[1] : D.w.v[0];
   NS3D_GLOBAL T* __restrict__ pnew = par ? D.w.p[0] : D.w.p[1];
   NS3D_GLOBAL T* __restrict__ vnew = par ? D.w.v[0] : D.w.v[1];
-  if (status != BS_RUNNING || done != it) return;
+  __syncthreads();
+  if (!go) return;
   double acc = 0.0;
   bool stop = false;
   by_layout(D.S.layout, [&](auto LC) {

@@ -360,52 +360,73 @@ void NavierStokesBase::InitVelocity() {
 // NSBase.cxx:196-301 (RelaxControl).  The reference pops from a possibly
 // empty history on the first growth test; here an empty history is left
 // alone.
-void NavierStokesBase::RelaxControl(bool beta_cst) {
+int relax_control_step(RelaxState& s, bool beta_cst, double win[3], bool* window_ran) {
   const size_t size = 30;
-  bool isDecreasing = false;
-  if (RMR > old_RMR * maxconvdelta) {
-    alpha = std::max(min_alpha, alpha - relax);
-    if (!beta_cst) beta = std::max(min_beta, beta - relax);
-    old_RMR *= 1.1;
-    say("Relative Mass Residual increases too much -> new relaxation parameter alpha, beta: %g, %g\n", alpha, beta);
-    info_file << "Relative Mass Residual increases too much -> new relaxation parameter alpha, beta: " << alpha
-              << ", " << beta << std::endl;
-    if (!avgRMR.empty()) avgRMR.pop_front();
+  int ev = 0;
+  *window_ran = false;
+  if (s.RMR > s.old_RMR * s.maxconvdelta) {
+    s.alpha = std::max(s.min_alpha, s.alpha - s.relax);
+    if (!beta_cst) s.beta = std::max(s.min_beta, s.beta - s.relax);
+    s.old_RMR *= 1.1;
+    if (!s.avgRMR.empty()) s.avgRMR.pop_front();
+    ev |= RELAX_JUMP;
   }
-  if (!avgRMR.empty()) {
-    isDecreasing = avgRMR.back() > RMR;
-    say("delta RMR = %6.3e \n", RMR - avgRMR.back());
+  const bool isDecreasing = !s.avgRMR.empty() && s.avgRMR.back() > s.RMR;
+  s.avgRMR.push_back(s.RMR);
+  if (s.avgRMR.size() <= size) return ev;
+  s.avgRMR.pop_front();
+  // first and second half of the window (the middle entry goes to the second)
+  const double half = s.avgRMR.size() * 0.5;
+  double sum[2] = {0, 0};
+  int cnt[2] = {0, 0};
+  int i = 0;
+  for (double x : s.avgRMR) {
+    const int h = ++i < half ? 0 : 1;
+    sum[h] += x;
+    cnt[h] += 1;
   }
-  avgRMR.push_back(RMR);
-  if (avgRMR.size() > size) {
-    avgRMR.pop_front();
-    int i = 0, n1 = 0, n2 = 0;
-    double avg1 = 0, avg2 = 0;
-    for (double x : avgRMR) {
-      i += 1;
-      if (i < avgRMR.size() * 0.5) { avg1 += x; n1 += 1; }
-      else if (i >= avgRMR.size() * 0.5) { avg2 += x; n2 += 1; }
-    }
-    avg1 /= n1;
-    avg2 /= n2;
-    const double delta = (avg2 - avg1) / std::max(avg1, avg2);
-    say("avg RMRold = %f, avg RMRnew = %f, delta avgRMR / avgRMR_max = %f %%\n", avg1, avg2, delta * 100);
-    if (delta <= 0 && std::fabs(delta) < deltaRMR && !isDecreasing) {
-      for (int q = 0; q < 15 && !avgRMR.empty(); ++q) avgRMR.pop_front();
-      alpha = std::min(alpha0, alpha - relax * 0.25);
-      if (!beta_cst) beta = std::max(min_beta, beta - relax * 0.25);
-      say("Simulation is oscillating -> new relaxation parameter alpha, beta: %g, %g\n", alpha, beta);
-      info_file << "Simulation is oscillating -> new relaxation parameter alpha, beta: " << alpha << ", " << beta
-                << std::endl;
-    } else if (delta > 0 && !isDecreasing) {
-      for (int q = 0; q < 15 && !avgRMR.empty(); ++q) avgRMR.pop_front();
-      alpha = std::max(min_alpha, alpha - relax);
-      if (!beta_cst) beta = std::max(min_beta, beta - relax);
-      say("Simulation is diverging -> new relaxation parameter alpha, beta: %g, %g\n", alpha, beta);
-      info_file << "Simulation is diverging -> new relaxation parameter alpha, beta: " << alpha << ", " << beta
-                << std::endl;
-    }
+  const double avg1 = sum[0] / cnt[0], avg2 = sum[1] / cnt[1];
+  const double delta = (avg2 - avg1) / std::max(avg1, avg2);
+  win[0] = avg1; win[1] = avg2; win[2] = delta;
+  *window_ran = true;
+  if (isDecreasing) return ev;
+  const bool flat = delta <= 0 && std::fabs(delta) < s.deltaRMR;
+  if (!flat && !(delta > 0)) return ev;
+  for (int q = 0; q < 15 && !s.avgRMR.empty(); ++q) s.avgRMR.pop_front();
+  if (flat) {
+    // no lower clamp on alpha in this branch (NSBase.cxx:263)
+    s.alpha = std::min(s.alpha0, s.alpha - s.relax * 0.25);
+    if (!beta_cst) s.beta = std::max(s.min_beta, s.beta - s.relax * 0.25);
+    return ev | RELAX_FLAT;
   }
+  s.alpha = std::max(s.min_alpha, s.alpha - s.relax);
+  if (!beta_cst) s.beta = std::max(s.min_beta, s.beta - s.relax);
+  return ev | RELAX_RISING;
+}
+
+void NavierStokesBase::RelaxControl(bool beta_cst) {
+  RelaxState s{alpha, beta, alpha0, min_alpha, min_beta, relax, maxconvdelta, deltaRMR, RMR, old_RMR, {}};
+  s.avgRMR.swap(avgRMR);
+  const size_t before = s.avgRMR.size();
+  const double last = before ? s.avgRMR.back() : 0.0;
+  double win[3];
+  bool ran = false;
+  const int ev = relax_control_step(s, beta_cst, win, &ran);
+  alpha = s.alpha;
+  beta = s.beta;
+  old_RMR = s.old_RMR;
+  avgRMR.swap(s.avgRMR);
+  auto tell = [&](const char* what) {
+    say("%s -> new relaxation parameter alpha, beta: %g, %g\n", what, alpha, beta);
+    info_file << what << " -> new relaxation parameter alpha, beta: " << alpha << ", " << beta << std::endl;
+  };
+  if (ev & RELAX_JUMP) tell("Relative Mass Residual increases too much");
+  // against the history's last entry, if one is left after a jump's pop
+  if (before > ((ev & RELAX_JUMP) ? 1u : 0u)) say("delta RMR = %6.3e \n", RMR - last);
+  if (ran)
+    say("avg RMRold = %f, avg RMRnew = %f, delta avgRMR / avgRMR_max = %f %%\n", win[0], win[1], win[2] * 100);
+  if (ev & RELAX_FLAT) tell("Simulation is oscillating");
+  if (ev & RELAX_RISING) tell("Simulation is diverging");
 }
 
 // NSBase.cxx:304-328 (CheckConvergence); nz > 0 always holds.

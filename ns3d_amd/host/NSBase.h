@@ -111,6 +111,19 @@ class NavierStokesBase {
   void tick(Phase p);
 };
 
+// What RelaxControl reads and writes (the NavierStokesBase members of the
+// same names), so that the step can run without a GPU context.
+struct RelaxState {
+  double alpha, beta, alpha0, min_alpha, min_beta, relax, maxconvdelta, deltaRMR, RMR, old_RMR;
+  std::list<double> avgRMR;
+};
+// RelaxControl's events (a bit set): the residual jumped above maxconvdelta x
+// its best; the window of 30 residuals was flat ("oscillating"); or rising.
+enum RelaxEvent { RELAX_JUMP = 1, RELAX_FLAT = 2, RELAX_RISING = 4 };
+// One RelaxControl step on s (NSBase.cxx:196-301); returns the events.
+// avg1/avg2/delta of the window test are stored in win[3] when it ran.
+int relax_control_step(RelaxState& s, bool beta_cst, double win[3], bool* window_ran);
+
 // Throws std::runtime_error with ns3d_last_error() when st != NS3D_OK.
 void check(ns3d_status st, const char* what);
 

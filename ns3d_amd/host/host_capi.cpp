@@ -25,6 +25,31 @@ extern "C" {
 
 const char* ns3d_host_last_error(void) { return g_host_err.c_str(); }
 
+int ns3d_relax_replay(const double* rmr, int n, int beta_cst, double alpha0, double beta0, double deltaRMR,
+                      double* alpha_out, double* beta_out, int* events) {
+  if (!rmr || n < 0 || !alpha_out || !beta_out || !events) {
+    g_host_err = "ns3d_relax_replay: bad arguments";
+    return 1;
+  }
+  // the constants of the NavierStokesBase constructor (NSBase.cxx:171-179)
+  RelaxState s{alpha0, beta0, alpha0, 0.001, 0.001, 0.05, 2.5, deltaRMR, 0.0, 0.0, {}};
+  for (int it = 0; it < n; ++it) {
+    s.RMR = rmr[it];
+    // the drivers' best-residual tracking (SIMPLEC.cxx:155-166)
+    if (it > 1) {
+      if (s.RMR < s.old_RMR) s.old_RMR = s.RMR;
+    } else {
+      s.old_RMR = s.RMR;
+    }
+    double win[3];
+    bool ran = false;
+    events[it] = relax_control_step(s, beta_cst != 0, win, &ran);
+    alpha_out[it] = s.alpha;
+    beta_out[it] = s.beta;
+  }
+  return 0;
+}
+
 int ns3d_solver_create(const ns3d_run_params* p, const uint8_t* img, int nx, int ny, int nz, ns3d_solver** out) {
   if (!p || !img || !out || nx <= 0 || ny <= 0 || nz <= 0) {
     g_host_err = "ns3d_solver_create: bad arguments";

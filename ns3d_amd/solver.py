@@ -21,7 +21,7 @@ EXPORTS = ["ns3d_default_run_params", "ns3d_solver_create", "ns3d_solver_info", 
            "ns3d_solver_fields", "ns3d_solver_destroy", "ns3d_solver_context", "ns3d_host_last_error",
            "ns3d_read_ini", "ns3d_solver_warm_start", "ns3d_server_create", "ns3d_server_submit", "ns3d_server_poll", "ns3d_server_wait",
            "ns3d_server_stats", "ns3d_server_destroy", "ns3d_server_last_error", "ns3d_server_engine",
-           "ns3d_server_set_policy"]
+           "ns3d_server_set_policy", "ns3d_relax_replay"]
 
 
 class RunParams(C.Structure):
@@ -95,8 +95,26 @@ def lib():
         L.ns3d_server_engine.argtypes = [vp]
         L.ns3d_server_engine.restype = vp
         L.ns3d_server_set_policy.argtypes = [vp, C.c_int, C.c_double]
+        L.ns3d_relax_replay.argtypes = [P(C.c_double), C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        P(C.c_double), P(C.c_double), P(C.c_int)]
+        L.ns3d_relax_replay.restype = C.c_int
         _lib = L
     return _lib
+
+
+def relax_replay(rmr, beta_cst=True, alpha0=0.8, beta0=1.0, deltaRMR=1e-4):
+    """RelaxControl over a scripted sequence of relative mass residuals
+    (ns3d_relax_replay): arrays alpha, beta and events after each outer
+    iteration (events: 1 jump, 2 flat window, 4 rising window)."""
+    r = np.ascontiguousarray(rmr, np.float64)
+    a, b = np.zeros(r.size), np.zeros(r.size)
+    ev = np.zeros(r.size, np.int32)
+    P = C.POINTER
+    if lib().ns3d_relax_replay(r.ctypes.data_as(P(C.c_double)), r.size, int(beta_cst), alpha0, beta0, deltaRMR,
+                               a.ctypes.data_as(P(C.c_double)), b.ctypes.data_as(P(C.c_double)),
+                               ev.ctypes.data_as(P(C.c_int))) != 0:
+        raise _gpu.NS3DError(lib().ns3d_host_last_error().decode())
+    return a, b, ev
 
 
 def default_params(**kw) -> RunParams:

@@ -165,6 +165,12 @@ struct Ctx {
   std::vector<ResRow> rows;
   std::vector<double> timers;  // per-phase seconds: asm, vel, hat, pasm, psolve, corr, diag
   int final_it = 0;
+  // Test harness only (not in the reference): when > 0, the first outer
+  // iteration (it > 1) whose RMR falls below it is completed and the run then
+  // stops as if capped there (maxit = it + 1), so that the same run can be
+  // repeated elsewhere with maxitSIMPLE = horizon_it + 1.
+  double horizon_rmr = 0;
+  int horizon_it = -1;
   double final_K = 0;
 
   size_t g(int i, int j, int k) const { return ((size_t)i * ny + j) * nz + k; }
@@ -1161,6 +1167,7 @@ void run_driver(Ctx& c, int method, double simpletol, int maxit, int N1, int N2)
   c.alpha = c.alpha0;
   c.beta = c.beta0;
   c.avgRMR.clear();
+  c.horizon_it = -1;
   double t0;
   auto tick = [&](int slot) { double t = now_s(); c.timers[slot] += t - t0; t0 = t; };
   t0 = now_s();
@@ -1200,6 +1207,10 @@ void run_driver(Ctx& c, int method, double simpletol, int maxit, int N1, int N2)
     }
     if (method == METHOD_SIMPLEC || method == METHOD_SIMPLER) push_row(c, it, perm, c.RMR, div, reldiv);
     if (it > 1 && check_convergence(c, c.RMR, simpletol)) break;
+    if (c.horizon_rmr > 0 && it > 1 && c.horizon_it < 0 && c.RMR < c.horizon_rmr) {
+      c.horizon_it = it;
+      maxit = std::min(maxit, it + 1);
+    }
     if (method == METHOD_SIMPLEC) {
       assemble_pressure(c, PC_SIMPLEC, false, c.u, c.v, c.w, c.b); tick(3);
       compute_pressure_correction(c); tick(4);
@@ -1447,5 +1458,31 @@ int orc_solve(void* h, int method, double simpletol, int maxit, int N1, int N2, 
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
+
+// RelaxControl over a scripted residual sequence, with run_driver's tracking
+// of the best residual; alpha/beta after each iteration.
+void orc_relax_replay(const double* rmr, int n, int beta_cst, double alpha0, double beta0, double deltaRMR,
+                      double* alpha_out, double* beta_out) {
+  Ctx c;
+  c.alpha = c.alpha0 = alpha0;
+  c.beta = c.beta0 = beta0;
+  c.deltaRMR = deltaRMR;
+  for (int it = 0; it < n; ++it) {
+    c.RMR = rmr[it];
+    if (it > 1) {
+      if (c.RMR < c.old_RMR) c.old_RMR = c.RMR;
+    } else {
+      c.old_RMR = c.RMR;
+    }
+    relax_control(c, beta_cst != 0);
+    alpha_out[it] = c.alpha;
+    beta_out[it] = c.beta;
+  }
+}
+
+// Test harness: the RMR horizon of the next orc_solve (0: off) and the
+// iteration at which the last solve met it (-1: never).
+void orc_set_horizon(void* h, double rmr) { ((Ctx*)h)->horizon_rmr = rmr; }
+int orc_horizon_it(void* h) { return ((Ctx*)h)->horizon_it; }
 
 }  // extern "C"

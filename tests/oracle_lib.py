@@ -71,6 +71,10 @@ def lib():
         L.orc_solve.argtypes = [vp, i32, dbl, i32, i32, i32, P(dbl), i32, P(dbl)]
         L.orc_prepare_image.argtypes = [P(C.c_uint8), i32, i32, i32, i32, i32, P(C.c_uint8), P(i32)]
         L.orc_num_threads.restype = i32
+        L.orc_relax_replay.argtypes = [P(dbl), i32, i32, dbl, dbl, dbl, P(dbl), P(dbl)]
+        L.orc_set_horizon.argtypes = [vp, dbl]
+        L.orc_horizon_it.restype = i32
+        L.orc_horizon_it.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -213,6 +217,14 @@ class Oracle:
         lib().orc_compute_permeability(self.h, _p(out, C.c_double))
         return out
 
+    def set_horizon(self, rmr):
+        """Stop after the first outer iteration (it > 1) with RMR < rmr, as if
+        capped there; horizon_it() gives that iteration (-1: not met)."""
+        lib().orc_set_horizon(self.h, float(rmr))
+
+    def horizon_it(self):
+        return int(lib().orc_horizon_it(self.h))
+
     def solve(self, method="SIMPLEC", simpletol=1e-4, maxit=500, N1=2, N2=2, max_rows=100000):
         rows = np.zeros((max_rows, 14))
         summ = np.zeros(15)
@@ -246,3 +258,14 @@ def spmv_ell(cnt, col, val, x):
     lib().orc_spmv_ell(n, _p(cnt, C.c_int32), _p(col, C.c_int32), _p(val, C.c_double), _p(x, C.c_double),
                        _p(y, C.c_double))
     return y
+
+
+def relax_replay(rmr, beta_cst=True, alpha0=0.8, beta0=1.0, deltaRMR=1e-4):
+    """The oracle's RelaxControl over a scripted RMR sequence: (alpha, beta)
+    after each outer iteration."""
+    rmr = np.ascontiguousarray(rmr, np.float64)
+    a = np.zeros(rmr.size)
+    b = np.zeros(rmr.size)
+    lib().orc_relax_replay(_p(rmr, C.c_double), rmr.size, int(beta_cst), alpha0, beta0, deltaRMR,
+                           _p(a, C.c_double), _p(b, C.c_double))
+    return a, b

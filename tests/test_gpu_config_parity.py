@@ -1,0 +1,106 @@
+"""Parity at full size on the parity configurations (GPU): the 64^3 RSA pack
+(r = 8, porosity 0.60) with SIMPLEC, and the 128^3 RSA packs (r = 10,
+porosity 0.75 and 0.45) with all four methods along x and SIMPLEC along y
+and z -- the HIP path through the host drivers against the oracle's run of
+the same case (tests/golden/make_config_parity.py).
+
+Every case runs at simpletol 1e-6 with inner tolerances 1e-10 / 1e-10 /
+1e-6 and maxitSIMPLE = the oracle's horizon count (the first iteration
+with RMR < 1e-3, completed).  Past that horizon the stop at simpletol 1e-6
+is decided by rounding: the oracle itself, run on the image mirrored in y,
+ends 3.4e-5 away in K (64^3), so the horizon is where parity is a property
+of the two implementations and not of rounding.
+
+Bars: index arrays bit-exact (sha256); |dK|/K <= 1e-6 at the end and in
+every RES row; relative L2 <= 1e-6 of p - p_out, u, v and w.  The fields'
+relative L2 is estimated from their CountSketch (1024 buckets, +-2.2% one
+sigma on the norm), and the exact norms must agree to 1e-6 as well.
+"""
+from __future__ import annotations
+
+import glob
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import count_sketch
+from ns3d_amd import gpu as G
+from ns3d_amd import solver
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import make_config_parity as MC  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = sorted(os.path.splitext(os.path.basename(f))[0]
+                  for f in glob.glob(os.path.join(HERE, "golden", "config_parity", "*.json")))
+BAR = 1e-6
+_images = {}
+
+
+def load(case):
+    with open(os.path.join(HERE, "golden", "config_parity", case + ".json")) as f:
+        meta = json.load(f)
+    arr = np.load(os.path.join(HERE, "golden", "config_parity", case + ".npz"))
+    return meta, arr
+
+
+def raw_image(name, sha):
+    if name not in _images:
+        img, rec = MC.image(name)
+        assert rec["sha256"] == sha, f"{name}: regenerated image differs from the fixture's"
+        _images[name] = img
+    return _images[name]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("case", FIXTURES)
+def test_config_case_matches_oracle(case):
+    meta, fx = load(case)
+    print(f"[config parity] {case}: {meta['method']} dir {meta['direction']} dims {meta['dims']} "
+          f"outer its {meta['final_it']}", flush=True)
+    raw = raw_image(meta["image"], meta["recipe"]["sha256"])
+    img = G.prepare_image(raw, meta["direction"], 0)
+    assert list(img.shape) == meta["dims"]
+    prm = meta["params"]
+    # fluid-index compaction, bit-exact
+    with G.NavierStokesGPU(img, **prm) as g:
+        assert [g.np, g.nu, g.nv, g.nw] == meta["counts"]
+        assert [sha(a) for a in g.index()] == meta["index_sha256"]
+    rows, summ = solver.solve(raw, method=meta["method"], direction=meta["direction"], border=0,
+                              simpletol=meta["simpletol"], maxitSIMPLE=meta["final_it"], N1=meta["N1"],
+                              N2=meta["N2"], device=0, **prm)
+    assert summ["iterations"] == meta["final_it"] and not summ["converged"]
+    dk = abs(summ["K"] / meta["K"] - 1.0)
+    orows = fx["rows"]
+    assert rows.shape == orows.shape, (rows.shape, orows.shape)
+    if len(orows):
+        assert np.array_equal(rows[:, 0], orows[:, 0])
+        row_dk = np.max(np.abs(rows[:, 4] / orows[:, 4] - 1.0))
+        row_dq = np.max(np.abs(rows[:, 1] / orows[:, 1] - 1.0))
+    else:
+        row_dk = row_dq = 0.0
+    f = summ["fields"]
+    p_out = 0.0  # p1 = p0 - gradp * dx * nx = 0 for the unit drop
+    errs, norm_errs = {}, {}
+    for name in "puvw":
+        x = f[name] - (p_out if name == "p" else 0.0)
+        nb = np.sqrt(meta["field_sumsq"][name])
+        est = np.linalg.norm(count_sketch(x, meta["sketch_m"]) - fx["sketch_" + name]) / nb
+        errs[name] = est
+        norm_errs[name] = abs(np.linalg.norm(x) / nb - 1.0)
+    print(f"[config parity] {case}: |dK|/K {dk:.2e}, rows dK {row_dk:.2e} dQ {row_dq:.2e}, field rel L2 "
+          + " ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    assert dk <= BAR
+    assert row_dk <= BAR and row_dq <= BAR
+    for name in "puvw":
+        assert errs[name] <= BAR, (name, errs[name])
+        assert norm_errs[name] <= BAR, (name, norm_errs[name])
