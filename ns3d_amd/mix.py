@@ -3,10 +3,10 @@
 A mix follows the batching workload's recipe: per image, edges drawn
 independently per axis from a set (anisotropic boxes allowed), porosity
 uniform in a range, and the microstructure generator split evenly among
-monodisperse sphere packs (random sequential addition, radius 4-12, scaled
-down for small boxes), polydisperse overlapping spheres (Boolean model, radii
-3-15, scaled likewise) and random straight fibres (radius 3, length 40,
-isotropic axis).  Image q uses seed base_seed + q.  An image whose fluid
+monodisperse sphere packs (random sequential addition, radius uniform in
+[4, 12]), polydisperse overlapping spheres (Boolean model, radii uniform in
+[3, 15]) and random straight fibres (radius 3, length 40, isotropic axis),
+with the recipe's radii whatever the box.  Image q uses seed base_seed + q.  An image whose fluid
 does not connect the inlet to the outlet along the flow axis has no
 permeability to compute, so it is drawn again with the next seed (the seed
 used is recorded).
@@ -35,16 +35,14 @@ def percolates(img: np.ndarray, axis: int = 0) -> bool:
 def make_image(kind: str, shape, porosity: float, seed: int):
     """One image of the given generator kind ('rsa', 'boolean', 'fibres');
     returns (image, achieved porosity, description)."""
-    scale = min(shape) / 64.0  # radii of the recipe refer to boxes of ~64 or more
     rng = SplitMix64(seed ^ 0x9E3779B97F4A7C15)
     if kind == "rsa":
-        r = max(2.0, (4.0 + 8.0 * rng.uniform()) * min(1.0, scale * 1.5))
+        r = 4.0 + 8.0 * rng.uniform()
         img, solid, overlap, count = generators.rsa_spheres(shape, r, 1.0 - porosity, seed, max_reject=2000)
         return img, 1.0 - solid, dict(generator="rsa", radius=round(r, 3), spheres=count, overlap=overlap)
     if kind == "boolean":
-        rmin, rmax = 3.0 * min(1.0, scale * 1.5), 15.0 * min(1.0, scale * 1.5)
-        img, phi, count = generators.boolean_spheres(shape, max(1.5, rmin), max(2.0, rmax), porosity, seed)
-        return img, phi, dict(generator="boolean", rmin=round(rmin, 3), rmax=round(rmax, 3), spheres=count)
+        img, phi, count = generators.boolean_spheres(shape, 3.0, 15.0, porosity, seed)
+        return img, phi, dict(generator="boolean", rmin=3.0, rmax=15.0, spheres=count)
     if kind == "fibres":
         img, phi, count = generators.fibres(shape, 3.0, 40.0, porosity, seed)
         return img, phi, dict(generator="fibres", radius=3.0, length=40.0, fibres=count)
