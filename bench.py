@@ -1,35 +1,40 @@
 #!/usr/bin/env python3
 """Permeability solves/s and latency of the SIMPLE-family outer iteration on
 one MI355X: the served, batched mixed-precision form against the plain FP64
-GPU path on the same request mix.
+GPU path on the same requests.
 
-Headline workload ("mix", one step = the whole mix served once): a seeded
-request mix (ns3d_amd.mix: per image, edges per axis from {32, 48, 64, 96},
-porosity uniform in [0.35, 0.90], generators split among monodisperse RSA
-sphere packs, polydisperse Boolean spheres and random fibres; images whose
-fluid does not percolate along x are redrawn), SIMPLEC, flow along x, unit
-pressure drop, mu = rho = 1, voxel size 1, outer tolerance 1e-6 on the
-relative mass residual (at most 1000 outer iterations), inner tolerances 1e-10
-(velocity, pressure) and 1e-6 (pressure correction).  Forms, all on the same
-requests:
-  plain_fp64    one request at a time on its own context (the drop-in path),
-                contexts created (images uploaded, index scan done) before the
-                timed region;
-  served_fp64   the native server: all requests submitted at t = 0, worker
-                slots share one solve engine (batched BiCGSTAB), FP64;
-  served_mixed  the same with the mixed form (FP64 defect correction around
-                FP32-storage BiCGSTAB), FP64 rerun of a failed request --
-                the headline value, timed over `steps` passes;
-  served_mixed_poisson  the same requests arriving as a Poisson stream at a
-                fraction of the measured burst throughput, for latency under
-                load.
-Served forms include each request's upload, index scan and teardown inside
-the timed region (that is the service); plain_fp64 does not.  Every served
-permeability is compared with the plain FP64 one (|dK|/K).
+Headline workload (the batching configuration at full size): 48 seeded
+images (ns3d_amd.mix: edges per axis from {32, 48, 64, 96, 128, 160},
+porosity uniform in [0.35, 0.90], generators split among RSA sphere packs
+r U[4,12], Boolean spheres r U[3,15] and fibres r 3 L 40; images whose fluid
+does not percolate along x are redrawn), SIMPLEC, flow along x, unit
+pressure drop, mu = 1, voxel size 1, rho = 1e-3 (pore Reynolds ~1e-4: the
+Stokes limit, where the outer iteration's fixed point does not depend on
+rounding), outer tolerance 1e-6 on the relative mass residual (at most 1000
+outer iterations), inner tolerances 1e-10 (velocity, pressure) and 1e-6
+(pressure correction).
+
+  plain_fp64    every image once, one request at a time on its own context
+                (the drop-in path), contexts created before the timed solves;
+  served_mixed  the headline: a continuous stream of requests (request r is
+                image r mod 48) through the native server -- FP64 defect
+                correction around FP32-storage BiCGSTAB, batched on one solve
+                engine, requests admitted by projected device footprint and
+                a cap on fluid cells in flight, retired and backfilled as they
+                converge.  A step is --per-step completed requests; --warmup
+                steps are untimed, then --steps steps are timed (from the
+                completion ending the warm-up to the one ending the last
+                step).  Each request's upload, index scan and teardown are
+                inside (that is the service).
+
+speedup_vs_plain_fp64 compares the stream's rate with the plain path's rate
+on exactly the requests the timed region completed.  dK_gate compares every
+completed mixed request with the plain FP64 K of its image at |dK|/K <= 1e-4.
 
 Second section ("single"): one 128^3 RSA pack (r = 10, porosity 0.75), plain
 FP64 and mixed, with per-kernel timings from dispatch events -- the roofline
-of the dominant FP64 kernel, and the CPU baseline's image.
+of the dominant FP64 kernel.  The CPU baseline is the oracle on the box's
+host cores over a bounded sample of the mix.
 
 Output: one JSON line; progress on stderr.
 """
@@ -62,56 +67,80 @@ def pct(xs, q):
 # ---------------------------------------------------------------- the mix
 def mix_params(img, args, precision):
     nx = img.shape[0]
-    return dict(dx=1.0, mu=1.0, rho=1.0, p0=1.0, gradp=1.0 / nx, alpha=0.8, beta=1.0, direction=1, border=0,
+    return dict(dx=1.0, mu=1.0, rho=args.rho, p0=1.0, gradp=1.0 / nx, alpha=0.8, beta=1.0, direction=1, border=0,
                 method="SIMPLEC", simpletol=args.simpletol, maxitSIMPLE=args.mix_maxit, N1=4, N2=4, device=0,
                 precision=precision, inner_tol=args.inner_tol, mixed_min_rows=args.mixed_min_rows, **TOLS)
 
 
-def served_pass(reqs, args, precision, label, arrivals=None):
-    """All requests through one native server; arrivals: submission times
-    (seconds from the start) or None for a burst at t = 0."""
+def make_server(args):
     from ns3d_amd import solver
     srv = solver.Server(device=0, budget_bytes=args.budget_gb * 1e9, max_slots=args.slots, chunk=args.chunk,
                         fallback=solver.FALLBACK_FAILED)
-    srv.set_schedule(args.chunk, args.lanes, not getattr(args, "no_graphs", False))
-    srv.set_policy(largest_first=False, max_rows=getattr(args, "max_rows", 0.0))
+    srv.set_schedule(args.chunk, args.lanes, not args.no_graphs)
+    srv.set_policy(largest_first=False, max_rows=args.max_rows)
+    return srv
+
+
+def stream(reqs, args, precision, label, warm_steps, steps):
+    """A continuous stream of permeability requests through one native
+    server: request r is image r mod len(reqs); the server is kept fed with
+    args.backlog queued requests beyond the ones it runs, so requests join
+    the running batch as others retire.  A step is args.per_step completed
+    requests; the first warm_steps steps are untimed, the next `steps` are
+    the timed region (from the completion that ends the warm-up to the one
+    that ends the last step).  Requests still in flight after that finish
+    untimed."""
+    srv = make_server(args)
+    need = (warm_steps + steps) * args.per_step
+    done, pending = [], {}
+    nsub = 0
     try:
         t0 = time.perf_counter()
-        tickets = []
-        for q, (img, _) in enumerate(reqs):
-            if arrivals is not None:
-                dt = arrivals[q] - (time.perf_counter() - t0)
-                if dt > 0:
-                    time.sleep(dt)
-            tickets.append(srv.submit(img, **mix_params(img, args, precision)))
-        out = [srv.wait(t) for t in tickets]
-        wall = time.perf_counter() - t0
-        st = srv.stats()
+        while len(done) < need:
+            st = srv.stats()
+            while st["queued"] < args.backlog and len(done) + len(pending) < need + args.backlog:
+                img = reqs[nsub % len(reqs)][0]
+                pending[srv.submit(img, **mix_params(img, args, precision))] = nsub
+                nsub += 1
+                st = srv.stats()
+            for t in list(pending):
+                r = srv.poll(t)
+                if r is not None:
+                    done.append((pending.pop(t), r, time.perf_counter() - t0))
+            time.sleep(0.002)
         est = srv.engine_stats()
+        sst = srv.stats()
     finally:
-        srv.close()
-    lat = [r["t_end"] - r["t_submit"] for r in out]
-    log(f"  [{label}] {len(reqs)} requests in {wall:.2f} s ({len(reqs) / wall:.3f} solves/s), latency p50 "
-        f"{pct(lat, 50):.2f} s p99 {pct(lat, 99):.2f} s, peak running {st['peak_running']:.0f}, "
-        f"fallbacks {st['fallbacks']:.0f}")
-    return dict(wall=wall, results=out, latency=lat, stats=st, engine=est)
+        srv.close()  # finishes what is still in flight (untimed)
+    w = warm_steps * args.per_step
+    t_start = done[w - 1][2] if w > 0 else 0.0
+    t_end = done[need - 1][2]
+    timed = done[w:need]
+    lat = [r["t_end"] - r["t_submit"] for _, r, _ in timed]
+    wall = t_end - t_start
+    log(f"  [{label}] {len(timed)} requests timed in {wall:.2f} s ({len(timed) / wall:.3f} solves/s), latency p50 "
+        f"{pct(lat, 50):.2f} s p99 {pct(lat, 99):.2f} s, peak running {sst['peak_running']:.0f}, "
+        f"fallbacks {sst['fallbacks']:.0f}, submitted {nsub}")
+    return dict(wall=wall, timed=timed, all=done, latency=lat, stats=sst, engine=est, submitted=nsub)
 
 
 def plain_pass(reqs, args):
-    """The drop-in path: contexts created up front, then one solve at a time."""
+    """The drop-in path: contexts created up front, then one solve at a time;
+    per-request seconds and results."""
     from ns3d_amd import solver
     solvers = [solver.Solver(img, **mix_params(img, args, 0)) for img, _ in reqs]
-    out, lat = [], []
+    out, secs = [], []
     t0 = time.perf_counter()
     for s in solvers:
         t1 = time.perf_counter()
         out.append(s.run()[1])
-        lat.append(time.perf_counter() - t1)
+        secs.append(time.perf_counter() - t1)
     wall = time.perf_counter() - t0
     for s in solvers:
         s.close()
-    log(f"  [plain_fp64] {len(reqs)} requests in {wall:.2f} s ({len(reqs) / wall:.3f} solves/s)")
-    return dict(wall=wall, results=out, latency=lat)
+    log(f"  [plain_fp64] {len(reqs)} requests in {wall:.2f} s ({len(reqs) / wall:.3f} solves/s), "
+        f"{sum(int(r['converged']) for r in out)} converged")
+    return dict(wall=wall, results=out, seconds=secs)
 
 
 def engine_roofline(est, prec="f32", k="A"):
@@ -123,44 +152,35 @@ def engine_roofline(est, prec="f32", k="A"):
                 bytes_per_launch=round(nbytes / launches), achieved=round(ach, 1), frac=round(ach / PEAK_HBM_GBS, 4))
 
 
-def dk_gate(reqs, args, mixed, plain, tol=1e-4):
-    """|dK|/K of each mixed request against plain FP64, gated at tol.  The
-    outer iteration stops when the relative mass residual first dips below
-    simpletol, and the relaxation control (NSBase.cxx:196-300) lowers alpha
-    on every RMR spike; near the residual's noise floor both are decided by
-    rounding, so the stopped K of the FP64 path itself moves by up to ~1e-3
-    under a perturbation of the rounding alone.  For a request over tol the
-    FP64 path is therefore run on the image mirrored in y, in z and in both
-    (the same problem, other summation orders): the largest of those moves is
-    the FP64 path's own rounding spread, and the mixed request passes if it
-    stays within twice that (the mixed result is one more draw of the same
-    kind of perturbation, compared with the largest of three)."""
-    from ns3d_amd import solver
-    out = dict(tolerance=tol, requests=0, within_tolerance=0, within_fp64_rounding_spread=0, over=[])
-    for q, ((img, _), m, p) in enumerate(zip(reqs, mixed, plain)):
-        if not (m["converged"] and p["converged"]) or p["K"] == 0:
+def dk_gate(completed, plain, tol=1e-4):
+    """|dK|/K of every completed mixed request against the plain FP64 run of
+    its image, gated at tol.  A request that converges in FP64 but not in the
+    mixed form fails the gate; one that stalls at the outer-iteration cap in
+    both is listed (its K is not a fixed point to compare)."""
+    out = dict(tolerance=tol, requests=0, within=0, over=[], unconverged_mixed=[], stalled_both=[])
+    for r, m, _ in completed:
+        q = r % len(plain)
+        p = plain[q]
+        if not p["converged"]:
+            if not m["converged"]:
+                out["stalled_both"].append(q)
             continue
         out["requests"] += 1
+        if not m["converged"]:
+            out["unconverged_mixed"].append(q)
+            continue
         d = abs(m["K"] / p["K"] - 1)
         if d <= tol:
-            out["within_tolerance"] += 1
-            continue
-        spread = 0.0
-        for v in (img[:, ::-1, :], img[:, :, ::-1], img[:, ::-1, ::-1]):
-            v = np.ascontiguousarray(v)
-            sv = solver.Solver(v, **mix_params(v, args, 0))
-            r = sv.run()[1]
-            sv.close()
-            spread = max(spread, abs(r["K"] / p["K"] - 1))
-        log(f"  request {q}: mixed |dK|/K {d:.2e}; FP64 on mirrored copies {spread:.2e}")
-        out["within_fp64_rounding_spread"] += int(d <= 2 * spread)
-        out["over"].append(dict(request=q, dK_over_K=d, fp64_mirror_spread=spread))
-    out["passed"] = out["within_tolerance"] + out["within_fp64_rounding_spread"] == out["requests"]
+            out["within"] += 1
+        else:
+            out["over"].append(dict(request=r, image=q, dK_over_K=d))
+    out["stalled_both"] = sorted(set(out["stalled_both"]))
+    out["passed"] = out["within"] == out["requests"]
     return out
 
 
 def engine_chunk(est):
-    """Whole engine chunks (all solves in flight, both engine streams, from
+    """Whole engine chunks (all solves in flight, all engine streams, from
     the first A start to the last C stop, launch gaps included): the
     algorithmic bytes of the iterations executed over the span."""
     ms, n, nbytes = est.get("chunk", (0, 0, 0))
@@ -173,108 +193,103 @@ def engine_chunk(est):
 
 def mix_section(args):
     from ns3d_amd import mix
-    log(f"generating the request mix ({args.requests} requests, seed {args.mix_seed})")
-    reqs = mix.request_mix(args.requests, args.mix_seed, tuple(int(e) for e in args.edges.split(",")))
+    edges = tuple(int(e) for e in args.edges.split(","))
+    log(f"generating the request mix ({args.requests} requests, edges {args.edges}, seed {args.mix_seed})")
+    reqs = mix.request_mix(args.requests, args.mix_seed, edges)
     fluid = sum(d["fluid"] for _, d in reqs)
-    forms_run = set(args.mix_forms.split(","))
-    if forms_run != {"plain", "served64", "served", "poisson"}:
-        # a partial run (profiling): only the served mixed pass(es), with the
-        # engine's own sampled kernel figures for comparison
-        for q in range(args.steps):
-            res = served_pass(reqs, args, 1, f"served_mixed step {q}")
-            for prec in ("f32", "f64"):
-                for k in "ABC":
-                    r = engine_roofline(res["engine"], prec, k)
-                    if r:
-                        log(f"  engine bicg_{k}_kernel<{'float' if prec == 'f32' else 'double'}>: {r}")
+    if args.mix_forms == "stream":
+        # profiler runs: the timed stream only
+        stream(reqs, args, 1, "served_mixed", args.warmup, args.steps)
         return None
     tuned = None
     if args.autotune:
         from ns3d_amd import autotune
-        log(f"autotuning the engine schedule on the mix with each request capped at {args.tune_maxit} outer "
-            f"iterations")
+        # a held-out sample of the same recipe (another seed), each request
+        # capped at tune_maxit outer iterations
+        tune_seed = args.mix_seed + 7919
+        log(f"autotuning the engine schedule on {args.tune_requests} held-out requests (seed {tune_seed}), "
+            f"each capped at {args.tune_maxit} outer iterations")
         sample = []
-        for img, _ in reqs:
+        for img, _ in mix.request_mix(args.tune_requests, tune_seed, edges):
             p = mix_params(img, args, 1)
             p["maxitSIMPLE"] = args.tune_maxit
             sample.append((img, p))
-        # chunks of 12 and one lane never won a full pass; the capped sample
-        # cannot see the burst's tail, where shorter chunks help
         tuned = autotune.tune(sample, candidates=[(c, l, True) for c in (4, 6, 8) for l in (2, 3)] + [(6, 3, False)],
                               log=log, budget_gb=args.budget_gb, slots=args.slots)
         args.chunk, args.lanes, args.no_graphs = tuned["chunk"], tuned["lanes"], not tuned["graphs"]
         log(f"  chosen: chunk {args.chunk}, lanes {args.lanes}, graphs {int(not args.no_graphs)}")
-    for w in range(args.warmup):
-        log(f"warmup pass {w} (served mixed)")
-        served_pass(reqs, args, 1, "warmup")
-    log("plain FP64 path")
+    log("plain FP64 path, every request of the mix once")
     plain = plain_pass(reqs, args)
-    log("served FP64")
-    s64 = served_pass(reqs, args, 0, "served_fp64")
-    log(f"served mixed: {args.steps} timed pass(es)")
-    smx = [served_pass(reqs, args, 1, f"served_mixed step {q}") for q in range(args.steps)]
-    rate = args.poisson_frac * args.requests * args.steps / sum(p["wall"] for p in smx)
-    rng = np.random.default_rng(args.mix_seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs))) - 0.0
-    arrivals[0] = 0.0
-    log(f"served mixed, Poisson arrivals at {rate:.3f} requests/s")
-    spo = served_pass(reqs, args, 1, "served_mixed_poisson", arrivals=arrivals)
-    k64 = np.array([r["K"] for r in plain["results"]])
-    conv64 = [bool(r["converged"]) for r in plain["results"]]
-
-    def dk(res):
-        """|dK|/K against plain FP64 over the requests converged in both runs
-        (a request that stalls in every precision stops wherever its cap
-        falls, so its K is not a fixed point to compare)."""
-        return [abs(r["K"] / k - 1) for r, k, c in zip(res, k64, conv64) if c and r["converged"] and k != 0]
-    dk_mixed = dk(smx[-1]["results"])
-    dk_s64 = dk(s64["results"])
-    gate = dk_gate(reqs, args, smx[-1]["results"], plain["results"])
-    stalled = [q for q, (r, c) in enumerate(zip(smx[-1]["results"], conv64)) if not (c and r["converged"])]
-    n = len(reqs)
-    mixed_wall = sum(p["wall"] for p in smx) / len(smx)
-    lat_m = [x for p in smx for x in p["latency"]]
-
-    def form(wall, lat, extra=None):
-        d = dict(solves_per_s=round(n / wall, 4), ms_per_pass=round(1e3 * wall, 1),
-                 latency_s=dict(p50=round(pct(lat, 50), 3), p99=round(pct(lat, 99), 3)))
-        if extra:
-            d.update(extra)
-        return d
-
+    log(f"served mixed stream: {args.warmup} warm-up and {args.steps} timed steps of {args.per_step} completed "
+        f"requests")
+    smx = stream(reqs, args, 1, "served_mixed", args.warmup, args.steps)
+    n = len(smx["timed"])
+    value = n / smx["wall"]
+    # the plain path's rate on exactly the requests the timed region completed
+    plain_secs = sum(plain["seconds"][r % len(reqs)] for r, _, _ in smx["timed"])
+    plain_rate = n / plain_secs
+    gate = dk_gate(smx["all"], plain["results"])
+    dks = [abs(m["K"] / plain["results"][r % len(reqs)]["K"] - 1) for r, m, _ in smx["all"]
+           if m["converged"] and plain["results"][r % len(reqs)]["converged"]]
     forms = {
-        "plain_fp64": form(plain["wall"], plain["latency"], dict(
-            converged=int(sum(r["converged"] for r in plain["results"])),
-            outer_iterations=int(sum(r["iterations"] for r in plain["results"])))),
-        "served_fp64": form(s64["wall"], s64["latency"], dict(max_dK_over_K=max(dk_s64),
-                                                              peak_running=s64["stats"]["peak_running"])),
-        "served_mixed": form(mixed_wall, lat_m, dict(
-            max_dK_over_K=max(dk_mixed), median_dK_over_K=float(np.median(dk_mixed)), dK_requests=len(dk_mixed),
-            not_converged_requests=stalled,
-            fallbacks=int(sum(r["fallback"] for r in smx[-1]["results"])),
-            converged=int(sum(r["converged"] for r in smx[-1]["results"])),
-            peak_running=smx[-1]["stats"]["peak_running"],
-            peak_device_bytes=smx[-1]["stats"]["peak_bytes_in_use"],
-            dK_gate=gate, engine_A_f32=engine_roofline(smx[-1]["engine"], "f32", "A"))),
-        "served_mixed_poisson": form(spo["wall"], spo["latency"], dict(arrival_rate=round(rate, 4))),
+        "plain_fp64": dict(solves_per_s=round(len(reqs) / plain["wall"], 4), ms_per_pass=round(1e3 * plain["wall"], 1),
+                           solves_per_s_on_timed_requests=round(plain_rate, 4),
+                           converged=int(sum(r["converged"] for r in plain["results"])),
+                           outer_iterations=int(sum(r["iterations"] for r in plain["results"])),
+                           latency_s=dict(p50=round(pct(plain["seconds"], 50), 3),
+                                          p99=round(pct(plain["seconds"], 99), 3))),
+        "served_mixed": dict(solves_per_s=round(value, 4), timed_requests=n,
+                             latency_s=dict(p50=round(pct(smx["latency"], 50), 3), p99=round(pct(smx["latency"], 99), 3)),
+                             max_dK_over_K=max(dks) if dks else None,
+                             median_dK_over_K=float(np.median(dks)) if dks else None,
+                             fallbacks=int(sum(m["fallback"] for _, m, _ in smx["all"])),
+                             peak_running=smx["stats"]["peak_running"],
+                             peak_device_bytes=smx["stats"]["peak_bytes_in_use"], submitted=smx["submitted"],
+                             engine_A_f32=engine_roofline(smx["engine"], "f32", "A")),
     }
-    desc = dict(workload=f"request mix: {n} seeded images (edges {args.edges} per axis, porosity U[0.35,0.90], "
-                         f"RSA / Boolean / fibres), SIMPLEC, flow x, simpletol {args.simpletol:g}, at most "
-                         f"{args.mix_maxit} outer iterations",
-                requests=n, fluid_cells=int(fluid), mix_seed=args.mix_seed, slots=args.slots,
+    if args.poisson:
+        # latency under load: Poisson arrivals at a fraction of the stream's rate
+        rate = args.poisson_frac * value
+        rng = np.random.default_rng(args.mix_seed)
+        arrivals = np.concatenate([[0.0], np.cumsum(rng.exponential(1.0 / rate, size=len(reqs) - 1))])
+        srv = make_server(args)
+        try:
+            t0 = time.perf_counter()
+            tickets = []
+            for q, (img, _) in enumerate(reqs):
+                dt = arrivals[q] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                tickets.append(srv.submit(img, **mix_params(img, args, 1)))
+            res = [srv.wait(t) for t in tickets]
+            pwall = time.perf_counter() - t0
+        finally:
+            srv.close()
+        plat = [r["t_end"] - r["t_submit"] for r in res]
+        log(f"  [poisson {rate:.3f}/s] latency p50 {pct(plat, 50):.2f} s p99 {pct(plat, 99):.2f} s")
+        forms["served_mixed_poisson"] = dict(arrival_rate=round(rate, 4), solves_per_s=round(len(reqs) / pwall, 4),
+                                             latency_s=dict(p50=round(pct(plat, 50), 3), p99=round(pct(plat, 99), 3)))
+    desc = dict(workload=f"batching configuration: stream of {len(reqs)} seeded images (edges {args.edges} per axis, "
+                         f"porosity U[0.35,0.90], RSA r U[4,12] / Boolean r U[3,15] / fibres r 3 L 40), SIMPLEC, "
+                         f"flow x, Stokes limit (mu 1, rho {args.rho:g}, unit pressure drop), simpletol "
+                         f"{args.simpletol:g}, at most {args.mix_maxit} outer iterations; a step = "
+                         f"{args.per_step} completed requests",
+                requests=len(reqs), fluid_cells=int(fluid), mix_seed=args.mix_seed, slots=args.slots,
+                backlog=args.backlog, per_step=args.per_step,
                 schedule=dict(chunk=args.chunk, lanes=args.lanes, graphs=not args.no_graphs,
-                              autotuned=tuned is not None, max_rows_in_flight=args.max_rows), chunk=args.chunk,
+                              autotuned=tuned is not None, max_rows_in_flight=args.max_rows),
                 images=[dict(shape=d["shape"], generator=d["generator"], porosity=d["porosity"]) for _, d in reqs])
-    return dict(forms=forms, desc=desc, reqs=reqs, plain=plain, mixed_wall=mixed_wall, lat=lat_m,
-                max_dk=max(dk_mixed), gate=gate, engine=smx[-1]["engine"], autotune=tuned)
+    return dict(forms=forms, desc=desc, reqs=reqs, plain=plain, value=value, plain_rate=plain_rate,
+                ms_per_step=1e3 * smx["wall"] / args.steps, gate=gate, engine=smx["engine"], autotune=tuned,
+                max_dk=max(dks) if dks else None)
 
 
 # ---------------------------------------------------------------- single image
 def single_workload(args):
     from ns3d_amd import generators
     n = args.size
-    params = dict(dx=1.0, mu=1.0, rho=1.0, p0=1.0, gradp=1.0 / n, alpha=0.8, beta=1.0, direction=1, border=0,
-                  **TOLS)
+    params = dict(dx=1.0, mu=1.0, rho=args.rho, p0=1.0, gradp=1.0 / n, alpha=0.8, beta=1.0, direction=1,
+                  border=0, **TOLS)
     if getattr(args, "generator", "rsa") == "boolean":
         # the precision configuration's image: overlapping spheres, radii U[rmin, rmax]
         img = generators.boolean_spheres((n, n, n), args.rmin, args.rmax, args.porosity, args.seed)[0]
@@ -432,35 +447,27 @@ def cpu_baseline_mix(mixres, args):
                 sample_seconds=round(dt, 3))
 
 
-def copy_calibration(mib):
-    """A plain copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE
-    and WRITE_SIZE per dispatch give the factor from counter KiB to bytes."""
-    import torch
-    nbytes = mib << 20
-    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
-    src.fill_(1.0)
-    dst = torch.empty_like(src)
-    for _ in range(5):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes_read": nbytes, "bytes_written": nbytes, "copies": 5}), flush=True)
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1, help="timed served-mixed passes over the mix")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed served-mixed passes over the mix")
+    ap.add_argument("--steps", type=int, default=20, help="timed steps of the served mixed stream")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed steps of the stream before the timed ones")
+    ap.add_argument("--per-step", type=int, default=2, help="completed requests per step")
+    ap.add_argument("--backlog", type=int, default=2, help="stream: queued requests kept beyond the running ones")
     ap.add_argument("--section", default="all", choices=["all", "mix", "single"])
-    ap.add_argument("--requests", type=int, default=24)
+    ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--mix-seed", type=int, default=20261203)
-    ap.add_argument("--edges", default="32,48,64,96")
+    ap.add_argument("--edges", default="32,48,64,96,128,160")
+    ap.add_argument("--rho", type=float, default=1e-3,
+                    help="density: 1e-3 keeps the pore Reynolds number ~1e-4 (Stokes limit)")
     ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--chunk", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--poisson", action="store_true", help="also serve the mix once as Poisson arrivals")
     ap.add_argument("--poisson-frac", type=float, default=0.5)
-    ap.add_argument("--mix-forms", default="plain,served64,served,poisson",
-                    help="anything but the full list runs only the served mixed pass(es) (for profilers)")
+    ap.add_argument("--mix-forms", default="all", choices=["all", "stream"],
+                    help="stream: only the timed served mixed stream, no JSON line (for profilers)")
+    ap.add_argument("--tune-requests", type=int, default=12)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--radius", type=float, default=10.0)
     ap.add_argument("--porosity", type=float, default=0.75)
@@ -491,12 +498,7 @@ def main():
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="pick chunk / lanes / graphs with ns3d_amd.autotune before the mix (full runs)")
     ap.add_argument("--progress", action="store_true", help="one stderr line per outer iteration (single)")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run 5 device-to-device copies of MIB MiB (scales HBM byte counters under a profiler)")
     args = ap.parse_args()
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
-        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); it has no multi-GPU mode: requests "
               f"would shard across GPUs by image, one server per GPU", file=sys.stderr)
@@ -521,12 +523,12 @@ def main():
         value = f["served_mixed"]["solves_per_s"]
         line = {
             "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": round(1e3 * mixres["mixed_wall"], 1), "higher_is_better": True,
+            "warmup": args.warmup, "ms_per_step": round(mixres["ms_per_step"], 1), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64 arithmetic, f32 inner-solve storage",
             "data": "synthetic", "form": "served_mixed",
             "config": mixres["desc"],
             "latency_s": f["served_mixed"]["latency_s"],
-            "speedup_vs_plain_fp64": round(value / f["plain_fp64"]["solves_per_s"], 3),
+            "speedup_vs_plain_fp64": round(mixres["value"] / mixres["plain_rate"], 3),
             "max_dK_over_K_vs_fp64": mixres["max_dk"],
             "dK_gate": mixres["gate"],
             "autotune": mixres["autotune"],

@@ -146,6 +146,15 @@ const char* ns3d_server_last_error(void);
 int ns3d_relax_replay(const double* rmr, int n, int beta_cst, double alpha0, double beta0, double deltaRMR,
                       double* alpha_out, double* beta_out, int* events);
 
+/* Reads a binary image file -- XML ImageData (.vti: appended raw or base64
+ * data, zlib or none, UInt32/UInt64 headers) or legacy STRUCTURED_POINTS
+ * (.vtk) -- and binarises it as readvtkfile_uint8 does (vtkio.cxx:96-150):
+ * voxel == porevalue -> 0 (fluid), any other value -> 1.  out (may be NULL:
+ * only dims is filled) gets dims[0]*dims[1]*dims[2] bytes, C order (x
+ * slowest) as the reference's Blitz++ array.  Returns 0, or 1 with
+ * ns3d_host_last_error(). */
+int ns3d_read_image(const char* path, int porevalue, uint8_t* out, int dims[3]);
+
 /* Reads a .ini file with the reference's grammar (ReadInput,
  * navierstokes3D.cxx:266-463); the image path is copied to image (size cap).
  * Returns 0 on success, 1 on a fatal input error (message on stderr). */

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <stdexcept>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -56,9 +57,14 @@ struct SysMem {
   float* invdiag32 = nullptr;
   uint64_t* pk = nullptr;    // packed neighbour distances (layout != ML_INDEXED)
   int layout = ML_INDEXED;   // how the solver reads the matrix (MatLayout)
+  // plane-band chunk list of a large system (build_chunks; null: contiguous
+  // tiles) and the host copy of its band offsets, for the engine's tiles
+  int* chk = nullptr;
+  std::vector<int> band_off;
+  const int* band_off_host() const { return band_off.empty() ? nullptr : band_off.data(); }
   SysView view() const { return SysView{n, d0, grid, cell, nbr, coef, invdiag, rowsum}; }
-  SMat<double> mat64() const { return SMat<double>{n, layout, nbr, pk, coef, invdiag}; }
-  SMat<float> mat32() const { return SMat<float>{n, layout, nbr, pk, coef32, invdiag32}; }
+  SMat<double> mat64() const { return SMat<double>{n, layout, nbr, pk, coef, invdiag, chk}; }
+  SMat<float> mat32() const { return SMat<float>{n, layout, nbr, pk, coef32, invdiag32, chk}; }
 };
 
 // Device time of the fused BiCGSTAB kernels, per storage precision (0 FP64,
@@ -321,6 +327,7 @@ void run_solves(ns3d_ctx* c, SolveJob<T>* jobs, int n) {
     req[i].n = c->sys[j.sid].n;
     req[i].nnz = c->sys[j.sid].nnz;
     req[i].kind = j.sid == 3 ? 1 : 0;
+    req[i].band_off = c->sys[j.sid].band_off_host();
     req[i].ready = c->ev[0];
     rp[i] = &req[i];
   }
@@ -728,6 +735,25 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
     HCHK(ctx_sync(c));
     for (int q = 0; q < 4; ++q)
       c->sys[q].layout = hbad[q] ? ML_INDEXED : (q == 3 ? ML_PACKED_SYM : ML_PACKED);
+    // plane-band order for the solver kernels of large systems (the row
+    // threshold can be lowered with NS3D_CHUNK_MIN_ROWS, e.g. to test the
+    // order on small images)
+    int64_t min_rows = kChunkMinRows;
+    if (const char* e = std::getenv("NS3D_CHUNK_MIN_ROWS")) min_rows = std::max<int64_t>(1, std::atoll(e));
+    std::vector<int32_t> hcell;
+    for (int q = 0; q < 4; ++q) {
+      SysMem& S = c->sys[q];
+      if (S.n < min_rows || ny < kBands) continue;
+      hcell.resize((size_t)S.n);
+      HCHK(hipMemcpyAsync(hcell.data(), S.cell, (size_t)S.n * 4, hipMemcpyDeviceToHost, s));
+      HCHK(ctx_sync(c));
+      const std::vector<int> chk = build_chunks(hcell.data(), S.n, ny, nz, min_rows);
+      if (chk.empty()) continue;
+      S.chk = c->alloc<int>((int64_t)chk.size());
+      HCHK(hipMemcpyAsync(S.chk, chk.data(), chk.size() * 4, hipMemcpyHostToDevice, s));
+      S.band_off.assign(chk.begin(), chk.begin() + kBands + 1);
+      HCHK(ctx_sync(c));
+    }
     for (int set = 0; set < 3; ++set)
       for (int comp = 0; comp < 3; ++comp) c->vel[set][comp] = c->alloc<double>(c->sys[comp].n);
     c->p = c->alloc<double>(np);
@@ -1233,6 +1259,7 @@ ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, in
   b += 2 * 16 * std::ceil(N / kScanCells) + 64;        // scan block counts / offsets
   b += rows_all * (4 + 24 + 56 + 8 + 8) + 3 * nmax * 8;  // cell, nbr, coef, invdiag, pk; rowsum (u, v, w)
   b += 4 * kMaxTilesPerSolve * sizeof(Tile);           // tile tables
+  b += 8 * (rows_all / kBlock + 4.0 * kBands * (nx + 1)) + 4 * (4 * kChunkHeader + 64.0 * 1024);  // chunk lists
   b += 3 * 8 * (3 * nf + plane) + 24 * nf;             // velocity sets, p, pc, b
   // per lane: momentum rhs, saved field, BiCGSTAB vectors, partials, state
   const double lane = (2 + 7) * 8 * nmax + 8.0 * kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks) + 1024;

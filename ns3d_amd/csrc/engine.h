@@ -33,6 +33,9 @@ struct EngineRequest {
   SolveDesc<float> d32{};
   int64_t n = 0, nnz = 0;
   int kind = 0;
+  // the system's plane-band offsets (host; null: contiguous tiles), valid
+  // while the solve runs
+  const int* band_off = nullptr;
   hipEvent_t ready = nullptr;
   BState result{};
   bool done = false;

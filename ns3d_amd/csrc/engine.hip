@@ -287,7 +287,7 @@ static void tile_table(HostBuf& img, const std::vector<Tile>& tiles, const Solve
     e[i].end = tiles[i].end;
     e[i].it0 = reqs ? (*reqs)[s]->it_queued : 0;
     e[i].solve = s;
-    e[i].pad = 0;
+    e[i].step = tiles[i].step;
   }
 }
 
@@ -335,7 +335,7 @@ void ns3d_engine::admit(std::vector<EngineRequest*>& reqs, int buf, int inflight
       for (EngineRequest* r : reqs) {
         if (r->group != g) continue;
         SolveDesc<T> d = desc_of<T>(r);
-        make_tiles(ns, d.S.n, tiles, d.t0, d.t1);
+        make_tiles(ns, d.S.n, r->band_off, tiles, d.t0, d.t1);
         hd[ns++] = d;
       }
       if (ns == 0) return;
@@ -397,7 +397,7 @@ void ns3d_engine::rebuild(int g, int buf) {
     SolveDesc<T>* cur = (SolveDesc<T>*)G.descs.data();
     for (size_t i = 0; i < G.active.size(); ++i) {
       SolveDesc<T> d = desc_of<T>(G.active[i]);
-      make_tiles((int)i, d.S.n, G.tiles, d.t0, d.t1);
+      make_tiles((int)i, d.S.n, G.active[i]->band_off, G.tiles, d.t0, d.t1);
       G.active[i]->tiles = d.t1 - d.t0;
       cur[i] = d;
     }
@@ -690,7 +690,7 @@ void ns3d_engine::run() {
         while (!pending.empty()) {
           EngineRequest* r = pending.front();
           const int64_t n = r->prec == 0 ? r->d64.S.n : r->d32.S.n;
-          const int nt = tiles_for(n);
+          const int nt = tiles_for(n, r->band_off != nullptr);
           if (solves[r->prec] > 0 && (tiles[r->prec] + nt > kEngMaxTiles || solves[r->prec] + 1 > kEngMaxSolves))
             break;  // waits for room (first-come first-served)
           tiles[r->prec] += nt;

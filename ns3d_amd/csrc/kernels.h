@@ -87,7 +87,30 @@ struct SMat {
   const uint64_t* pk;   // [n] (ML_PACKED*)
   const T* coef;        // [7][n]
   const T* invdiag;     // [n]
+  const int* chk;       // plane-band chunk list (build_chunks), or null: contiguous tiles
 };
+
+// Plane-band order for large systems (n >= kChunkMinRows).  A plane of such a
+// system holds ~1e5 rows, so the E/W gathers of the row kernels reach one
+// plane away: with each block sweeping one contiguous tile, a row is read
+// again (as a neighbour) ~1e7 rows of traffic later and comes from HBM three
+// times.  Instead the rows are cut into kBands bands of j (one per XCD) and
+// each band into chunks of at most kBlock rows of one plane, in plane order;
+// the blocks of a band sweep its chunks together (block k of bx takes chunks
+// k, k + bx, ...), so the planes i-1, i, i+1 of a band are touched within a
+// few hundred kilobytes of that XCD's L2 traffic.  Layout of the list:
+// chk[0..kBands] = first chunk of each band (chk[kBands] = total), chk[9]
+// unused, then int2 {first row, rows} per chunk from chk + 10.
+constexpr int kBands = 8;
+#ifndef NS3D_CHUNK_MIN_ROWS
+#define NS3D_CHUNK_MIN_ROWS 4000000
+#endif
+constexpr int64_t kChunkMinRows = NS3D_CHUNK_MIN_ROWS;
+constexpr int kChunkHeader = 10;
+// The chunk list of a system from its row -> cell map (cell = (i*ny + j)*nz + k,
+// rows in ascending cell order); empty if the system stays contiguous (fewer
+// than min_rows rows, or fewer than kBands lines of j).
+std::vector<int> build_chunks(const int32_t* cell, int64_t n, int ny, int nz, int64_t min_rows);
 // BiCGSTAB vectors of storage type T; scalars and reductions are FP64.
 template <typename T>
 struct SWork {
@@ -120,10 +143,12 @@ struct SolveDesc {
   int t0, t1;
   int pad;
 };
-// A tile: rows [begin, end) of one solve's system, processed by one block.
+// A tile of one solve's system, processed by one block: rows [begin, end)
+// (step 0), or chunks begin, begin + step, ... below end of its band's
+// plane-band chunk list (step = blocks of the band).
 struct Tile {
   int solve;
-  int pad;
+  int step;
   long long begin, end;
 };
 // One entry of a batched launch's tile table: a tile with its solve's
@@ -136,7 +161,7 @@ struct TileDesc {
   long long begin, end;
   long long it0;
   int solve;
-  int pad;
+  int step;
 };
 // The operands of one launch over a set of solves of storage type T.
 // Either tables (d: one descriptor per solve, for the per-solve kernels;
@@ -171,9 +196,13 @@ constexpr int kMaxTilesPerSolve = 1024;
 #define NS3D_TILE_ROWS 1024
 #endif
 constexpr int kTileRows = NS3D_TILE_ROWS;
-int tiles_for(int64_t n);
+// (chunked: a system with a plane-band chunk list, whose tile count is a
+// multiple of kBands, at least kBands)
+int tiles_for(int64_t n, bool chunked = false);
 // Appends the tiles of solve `solve` (n rows) to out; returns its [t0, t1).
-void make_tiles(int solve, int64_t n, std::vector<Tile>& out, int& t0, int& t1);
+// band_off: the system's chunk list header (its first kBands + 1 entries,
+// host copy) for plane-band tiles, or null for contiguous ones.
+void make_tiles(int solve, int64_t n, const int* band_off, std::vector<Tile>& out, int& t0, int& t1);
 // Blocks of a launch over ntiles tiles (a multiple of the XCD count).
 int tile_grid(int ntiles);
 

@@ -194,6 +194,7 @@ struct GMat {
   const NS3D_GLOBAL uint64_t* pk;
   const NS3D_GLOBAL T* coef;
   const NS3D_GLOBAL T* invdiag;
+  const NS3D_GLOBAL int* chk;
 };
 template <typename T>
 struct GWork {
@@ -225,7 +226,7 @@ __device__ __forceinline__ NS3D_GLOBAL P* gl(P* p) {
 template <typename T>
 __device__ __forceinline__ GDesc<T> to_gdesc(const SolveDesc<T>& D) {
   GDesc<T> G;
-  G.S = GMat<T>{D.S.n, D.S.layout, gl(D.S.nbr), gl(D.S.pk), gl(D.S.coef), gl(D.S.invdiag)};
+  G.S = GMat<T>{D.S.n, D.S.layout, gl(D.S.nbr), gl(D.S.pk), gl(D.S.coef), gl(D.S.invdiag), gl(D.S.chk)};
   G.w.r = gl(D.w.r);
   G.w.r0 = gl(D.w.r0);
   G.w.p[0] = gl(D.w.p[0]);
@@ -255,19 +256,59 @@ template <typename T>
 __device__ __forceinline__ void tile_of(const Batch<T>& B, int t, Tile& tl, GDesc<T>& D, long long& it) {
   if (B.single) {
     D = to_gdesc(B.one);
-    const long long n = D.S.n, b = (long long)t * B.per;
     tl.solve = 0;
-    tl.begin = b < n ? b : n;
-    tl.end = b + B.per < n ? b + B.per : n;
+    if (D.S.chk) {
+      // plane-band tiles: band t / bx, block t % bx of it
+      const int bx = B.ntiles / kBands, band = t / bx;
+      tl.step = bx;
+      tl.begin = D.S.chk[band] + t % bx;
+      tl.end = D.S.chk[band + 1];
+    } else {
+      const long long n = D.S.n, b = (long long)t * B.per;
+      tl.step = 0;
+      tl.begin = b < n ? b : n;
+      tl.end = b + B.per < n ? b + B.per : n;
+    }
     it = B.one_it0 + B.q;
   } else {
     const TileDesc<T>* e = B.tiles + t;
     D = to_gdesc(e->d);
     tl.solve = e->solve;
+    tl.step = e->step;
     tl.begin = e->begin;
     tl.end = e->end;
     it = e->it0 + B.q;
   }
+}
+
+// The rows of a tile as one thread visits them.  Contiguous tiles (step 0):
+// rows begin + t, begin + t + kBlock, ... below end.  Plane-band tiles: the
+// chunks begin, begin + step, ... below end of the system's chunk list, each
+// {first row, rows}, thread t taking first + t when t < rows.  row() is -1
+// where this thread has no row at the current position.
+struct TileWalk {
+  long long pos, end, step;
+  // chunk descriptors as 64-bit words: first row (low half), rows (high half)
+  const NS3D_GLOBAL uint64_t* chunks;
+  __device__ __forceinline__ bool more() const { return pos < end; }
+  __device__ __forceinline__ int64_t row() const {
+    if (!step) {
+      const long long r = pos + threadIdx.x;
+      return r < end ? r : -1;
+    }
+    const uint64_t c = chunks[pos];
+    return threadIdx.x < (unsigned)(c >> 32) ? (int64_t)(uint32_t)c + threadIdx.x : -1;
+  }
+  __device__ __forceinline__ void next() { pos += step ? step : kBlock; }
+};
+template <typename T>
+__device__ __forceinline__ TileWalk walk_of(const Tile& tl, const GDesc<T>& D) {
+  TileWalk w;
+  w.pos = tl.begin;
+  w.end = tl.end;
+  w.step = tl.step;
+  w.chunks = tl.step ? (const NS3D_GLOBAL uint64_t*)(D.S.chk + kChunkHeader) : nullptr;
+  return w;
 }
 template <typename T>
 __device__ __forceinline__ GDesc<T> solve_of(const Batch<T>& B, int s) {
@@ -297,7 +338,9 @@ bicg_init_kernel(Batch<T> B) {
   by_layout(D.S.layout, [&](auto LC) {
     constexpr int L = decltype(LC)::value;
     const int64_t n = D.S.n;
-    for (int64_t row = tl.begin + threadIdx.x; row < tl.end; row += kBlock) {
+    for (TileWalk w = walk_of(tl, D); w.more(); w.next()) {
+      const int64_t row = w.row();
+      if (row < 0) continue;
       const double ax = stencil_row<T, L>(D.S, (int)row, n, [&](int q) { return (double)D.x[q]; });
       const double bv = D.b[row];
       const double r = bv - ax;
@@ -359,7 +402,9 @@ bicg_restart_kernel(Batch<T> B) {
   by_layout(D.S.layout, [&](auto LC) {
     constexpr int L = decltype(LC)::value;
     const int64_t n = D.S.n;
-    for (int64_t row = tl.begin + threadIdx.x; row < tl.end; row += kBlock) {
+    for (TileWalk w = walk_of(tl, D); w.more(); w.next()) {
+      const int64_t row = w.row();
+      if (row < 0) continue;
       const double ax = stencil_row<T, L>(D.S, (int)row, n, [&](int q) { return (double)D.x[q]; });
       const double r = (double)D.b[row] - ax;
       D.w.r[row] = (T)r;
@@ -459,10 +504,11 @@ bicg_A_kernel(Batch<T> B) {
   by_layout(D.S.layout, [&](auto LC) {
     constexpr int L = decltype(LC)::value;
     const int64_t n = D.S.n;
-    int64_t row = tl.begin + threadIdx.x;
+    TileWalk wk = walk_of(tl, D);
+    int64_t row = wk.more() ? wk.row() : -1;
     // first row in flight during the reduction
     AIn<T> cur;
-    if (row < tl.end) a_fetch<T, L>(D.S, D.w, pold, vold, (int)row, n, cur);
+    if (row >= 0) a_fetch<T, L>(D.S, D.w, pold, vold, (int)row, n, cur);
     const long long fresh_at = st->fresh_at;
     const double rho_prev = st->rho[sl ^ 1], alpha_prev = st->alpha[sl ^ 1], w_prev = st->w[sl ^ 1];
     double red[2];
@@ -491,21 +537,26 @@ bicg_A_kernel(Batch<T> B) {
       st->rho[sl] = rho;
       st->it_a = it;
     }
-    while (row < tl.end) {
-      const int64_t nxt = row + kBlock;
+    while (wk.more()) {
+      TileWalk wn = wk;
+      wn.next();
+      const int64_t nxt = wn.more() ? wn.row() : -1;
       AIn<T> nx;
-      if (NS3D_SOLVE_PIPE && nxt < tl.end) a_fetch<T, L>(D.S, D.w, pold, vold, (int)nxt, n, nx);
-      double pn[7];
+      if (NS3D_SOLVE_PIPE && nxt >= 0) a_fetch<T, L>(D.S, D.w, pold, vold, (int)nxt, n, nx);
+      if (row >= 0) {
+        double pn[7];
 #pragma unroll
-      for (int k = 0; k < 7; ++k)
-        pn[k] = (double)(T)((double)cur.r[k] + beta * ((double)cur.p[k] - om * (double)cur.v[k]));
-      const double vn = spmv_pts(cur.P, [&](int k) { return (double)cur.m[k] * pn[k]; });
-      pnew[row] = (T)pn[3];
-      vnew[row] = (T)vn;
-      acc += (double)cur.r0 * (double)(T)vn;
-      if (!NS3D_SOLVE_PIPE && nxt < tl.end) a_fetch<T, L>(D.S, D.w, pold, vold, (int)nxt, n, nx);
+        for (int k = 0; k < 7; ++k)
+          pn[k] = (double)(T)((double)cur.r[k] + beta * ((double)cur.p[k] - om * (double)cur.v[k]));
+        const double vn = spmv_pts(cur.P, [&](int k) { return (double)cur.m[k] * pn[k]; });
+        pnew[row] = (T)pn[3];
+        vnew[row] = (T)vn;
+        acc += (double)cur.r0 * (double)(T)vn;
+      }
+      if (!NS3D_SOLVE_PIPE && nxt >= 0) a_fetch<T, L>(D.S, D.w, pold, vold, (int)nxt, n, nx);
       cur = nx;
       row = nxt;
+      wk = wn;
     }
   });
   if (stop) return;
@@ -536,9 +587,10 @@ bicg_B_kernel(Batch<T> B) {
   by_layout(D.S.layout, [&](auto LC) {
     constexpr int L = decltype(LC)::value;
     const int64_t n = D.S.n;
-    int64_t row = tl.begin + threadIdx.x;
+    TileWalk wk = walk_of(tl, D);
+    int64_t row = wk.more() ? wk.row() : -1;
     BIn<T> cur;
-    if (row < tl.end) b_fetch<T, L>(D.S, D.w, vn, (int)row, n, cur);
+    if (row >= 0) b_fetch<T, L>(D.S, D.w, vn, (int)row, n, cur);
     const double rho = st->rho[sl];
     double red[1];
     partials_reduce<1>(pa, red);
@@ -547,20 +599,25 @@ bicg_B_kernel(Batch<T> B) {
       st->r0v = red[0];
       st->alpha[sl] = alpha;
     }
-    while (row < tl.end) {
-      const int64_t nxt = row + kBlock;
+    while (wk.more()) {
+      TileWalk wn = wk;
+      wn.next();
+      const int64_t nxt = wn.more() ? wn.row() : -1;
       BIn<T> nx;
-      if (NS3D_SOLVE_PIPE && nxt < tl.end) b_fetch<T, L>(D.S, D.w, vn, (int)nxt, n, nx);
-      double sv[7];
+      if (NS3D_SOLVE_PIPE && nxt >= 0) b_fetch<T, L>(D.S, D.w, vn, (int)nxt, n, nx);
+      if (row >= 0) {
+        double sv[7];
 #pragma unroll
-      for (int k = 0; k < 7; ++k) sv[k] = (double)(T)((double)cur.r[k] - alpha * (double)cur.v[k]);
-      const double tv = (double)(T)spmv_pts(cur.P, [&](int k) { return (double)cur.m[k] * sv[k]; });
-      D.w.t[row] = (T)tv;
-      a0 += tv * sv[3];
-      a1 += tv * tv;
-      if (!NS3D_SOLVE_PIPE && nxt < tl.end) b_fetch<T, L>(D.S, D.w, vn, (int)nxt, n, nx);
+        for (int k = 0; k < 7; ++k) sv[k] = (double)(T)((double)cur.r[k] - alpha * (double)cur.v[k]);
+        const double tv = (double)(T)spmv_pts(cur.P, [&](int k) { return (double)cur.m[k] * sv[k]; });
+        D.w.t[row] = (T)tv;
+        a0 += tv * sv[3];
+        a1 += tv * tv;
+      }
+      if (!NS3D_SOLVE_PIPE && nxt >= 0) b_fetch<T, L>(D.S, D.w, vn, (int)nxt, n, nx);
       cur = nx;
       row = nxt;
+      wk = wn;
     }
   });
   double v2[2] = {a0, a1};
@@ -589,9 +646,10 @@ bicg_C_kernel(Batch<T> B) {
   const NS3D_GLOBAL T* __restrict__ pn = par ? D.w.p[1] : D.w.p[0];
   const NS3D_GLOBAL T* __restrict__ vn = par ? D.w.v[1] : D.w.v[0];
   if (status != BS_RUNNING || done != it) return;
-  int64_t row = tl.begin + threadIdx.x;
+  TileWalk wk = walk_of(tl, D);
+  int64_t row = wk.more() ? wk.row() : -1;
   CIn<T> cur;
-  if (row < tl.end) c_fetch<T>(invdiag, x, D.w, pn, vn, row, cur);
+  if (row >= 0) c_fetch<T>(invdiag, x, D.w, pn, vn, row, cur);
   const double alpha = st->alpha[sl];
   double red[2];
   partials_reduce<2>(pa, red);
@@ -603,21 +661,26 @@ bicg_C_kernel(Batch<T> B) {
     st->it_c = it + 1;
   }
   double a0 = 0.0, a1 = 0.0;
-  while (row < tl.end) {
-    const int64_t nxt = row + kBlock;
+  while (wk.more()) {
+    TileWalk wn = wk;
+    wn.next();
+    const int64_t nxt = wn.more() ? wn.row() : -1;
     CIn<T> nx;
-    if (nxt < tl.end) c_fetch<T>(invdiag, x, D.w, pn, vn, nxt, nx);
-    const double md = cur.m;
-    const double y = md * (double)cur.p;
-    const double s = (double)(T)((double)cur.r - alpha * (double)cur.v);
-    const double z = md * s;
-    x[row] = (T)((double)cur.x + (alpha * y + om * z));
-    const double rn = (double)(T)(s - om * (double)cur.t);
-    D.w.r[row] = (T)rn;
-    a0 += (double)cur.r0 * rn;
-    a1 += rn * rn;
+    if (nxt >= 0) c_fetch<T>(invdiag, x, D.w, pn, vn, nxt, nx);
+    if (row >= 0) {
+      const double md = cur.m;
+      const double y = md * (double)cur.p;
+      const double s = (double)(T)((double)cur.r - alpha * (double)cur.v);
+      const double z = md * s;
+      x[row] = (T)((double)cur.x + (alpha * y + om * z));
+      const double rn = (double)(T)(s - om * (double)cur.t);
+      D.w.r[row] = (T)rn;
+      a0 += (double)cur.r0 * rn;
+      a1 += rn * rn;
+    }
     cur = nx;
     row = nxt;
+    wk = wn;
   }
   double v2[2] = {a0, a1};
   block_sums_at<2>(v2, D.slot(kSlotC), D.pstride, t - D.t0);
@@ -717,23 +780,62 @@ hipError_t launch_timed(K kernel, int nb, hipStream_t s, hipEvent_t* ev, Args...
   return hipGetLastError();
 }
 
-int tiles_for(int64_t n) {
+int tiles_for(int64_t n, bool chunked) {
   int64_t nt = (n + kTileRows - 1) / kTileRows;
   if (nt > kMaxTilesPerSolve) nt = kMaxTilesPerSolve;
   if (nt < 1) nt = 1;
+  if (chunked) nt = std::max<int64_t>(kBands, nt / kBands * kBands);
   return (int)nt;
 }
 
-void make_tiles(int solve, int64_t n, std::vector<Tile>& out, int& t0, int& t1) {
-  const int nt = tiles_for(n);
-  const int64_t per = ((n + nt - 1) / nt + kBlock - 1) / kBlock * kBlock;
+void make_tiles(int solve, int64_t n, const int* band_off, std::vector<Tile>& out, int& t0, int& t1) {
+  const int nt = tiles_for(n, band_off != nullptr);
   t0 = (int)out.size();
-  for (int k = 0; k < nt; ++k) {
-    const int64_t b = std::min<int64_t>((int64_t)k * per, n);
-    const int64_t e = std::min<int64_t>(b + per, n);
-    out.push_back(Tile{solve, 0, b, e});
+  if (band_off) {
+    const int bx = nt / kBands;
+    for (int t = 0; t < nt; ++t) {
+      const int band = t / bx;
+      out.push_back(Tile{solve, bx, (long long)band_off[band] + t % bx, (long long)band_off[band + 1]});
+    }
+  } else {
+    const int64_t per = ((n + nt - 1) / nt + kBlock - 1) / kBlock * kBlock;
+    for (int k = 0; k < nt; ++k) {
+      const int64_t b = std::min<int64_t>((int64_t)k * per, n);
+      const int64_t e = std::min<int64_t>(b + per, n);
+      out.push_back(Tile{solve, 0, b, e});
+    }
   }
   t1 = (int)out.size();
+}
+
+std::vector<int> build_chunks(const int32_t* cell, int64_t n, int ny, int nz, int64_t min_rows) {
+  std::vector<int> out;
+  if (n < min_rows || n < 1 || ny < kBands) return out;
+  // rows of one (plane, band) are contiguous: the rows follow (i, j, k)
+  // order and the band is monotone in j
+  std::vector<std::vector<int>> band(kBands);
+  const int64_t plane = (int64_t)ny * nz;
+  int64_t r = 0;
+  while (r < n) {
+    const int64_t i = cell[r] / plane;
+    const int b = (int)(((cell[r] / nz) % ny) * kBands / ny);
+    int64_t e = r + 1;
+    while (e < n && cell[e] / plane == i && (int)(((cell[e] / nz) % ny) * kBands / ny) == b) ++e;
+    for (int64_t c = r; c < e; c += kBlock) {
+      band[b].push_back((int)c);
+      band[b].push_back((int)std::min<int64_t>(kBlock, e - c));
+    }
+    r = e;
+  }
+  out.assign(kChunkHeader, 0);
+  int total = 0;
+  for (int b = 0; b < kBands; ++b) {
+    out[b] = total;
+    total += (int)band[b].size() / 2;
+  }
+  out[kBands] = total;
+  for (int b = 0; b < kBands; ++b) out.insert(out.end(), band[b].begin(), band[b].end());
+  return out;
 }
 
 int tile_grid(int ntiles) { return std::max(kXcd, (ntiles + kXcd - 1) / kXcd * kXcd); }
@@ -741,14 +843,14 @@ int tile_grid(int ntiles) { return std::max(kXcd, (ntiles + kXcd - 1) / kXcd * k
 template <typename T>
 Batch<T> single_batch(const SolveDesc<T>& d, long long it0, int q) {
   Batch<T> B{};
-  const int nt = tiles_for(d.S.n);
+  const int nt = tiles_for(d.S.n, d.S.chk != nullptr);
   B.single = 1;
   B.one = d;
   B.one.t0 = 0;
   B.one.t1 = nt;
   B.nsolves = 1;
   B.ntiles = nt;
-  B.per = ((d.S.n + nt - 1) / nt + kBlock - 1) / kBlock * kBlock;
+  B.per = d.S.chk ? 0 : ((d.S.n + nt - 1) / nt + kBlock - 1) / kBlock * kBlock;
   B.one_it0 = it0;
   B.q = q;
   return B;

@@ -8,6 +8,7 @@
 #include "../../include/ns3d_host.h"
 #include "NSBase.h"
 #include "drivers.h"
+#include "vtk_io.h"
 
 using namespace ns3d_host;
 
@@ -24,6 +25,23 @@ struct ns3d_solver {
 extern "C" {
 
 const char* ns3d_host_last_error(void) { return g_host_err.c_str(); }
+
+int ns3d_read_image(const char* path, int porevalue, uint8_t* out, int dims[3]) {
+  if (!path || !dims) {
+    g_host_err = "ns3d_read_image: bad arguments";
+    return 1;
+  }
+  try {
+    int nx = 0, ny = 0, nz = 0;
+    const std::vector<uint8_t> img = read_binary_image(path, porevalue, nx, ny, nz);
+    dims[0] = nx; dims[1] = ny; dims[2] = nz;
+    if (out) std::memcpy(out, img.data(), img.size());
+  } catch (const std::exception& e) {
+    g_host_err = e.what();
+    return 1;
+  }
+  return 0;
+}
 
 int ns3d_relax_replay(const double* rmr, int n, int beta_cst, double alpha0, double beta0, double deltaRMR,
                       double* alpha_out, double* beta_out, int* events) {

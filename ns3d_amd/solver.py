@@ -21,7 +21,7 @@ EXPORTS = ["ns3d_default_run_params", "ns3d_solver_create", "ns3d_solver_info", 
            "ns3d_solver_fields", "ns3d_solver_destroy", "ns3d_solver_context", "ns3d_host_last_error",
            "ns3d_read_ini", "ns3d_solver_warm_start", "ns3d_server_create", "ns3d_server_submit", "ns3d_server_poll", "ns3d_server_wait",
            "ns3d_server_stats", "ns3d_server_destroy", "ns3d_server_last_error", "ns3d_server_engine",
-           "ns3d_server_set_policy", "ns3d_relax_replay"]
+           "ns3d_server_set_policy", "ns3d_relax_replay", "ns3d_read_image"]
 
 
 class RunParams(C.Structure):
@@ -98,6 +98,8 @@ def lib():
         L.ns3d_relax_replay.argtypes = [P(C.c_double), C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                         P(C.c_double), P(C.c_double), P(C.c_int)]
         L.ns3d_relax_replay.restype = C.c_int
+        L.ns3d_read_image.argtypes = [C.c_char_p, C.c_int, P(C.c_uint8), P(C.c_int)]
+        L.ns3d_read_image.restype = C.c_int
         _lib = L
     return _lib
 
@@ -115,6 +117,18 @@ def relax_replay(rmr, beta_cst=True, alpha0=0.8, beta0=1.0, deltaRMR=1e-4):
                                ev.ctypes.data_as(P(C.c_int))) != 0:
         raise _gpu.NS3DError(lib().ns3d_host_last_error().decode())
     return a, b, ev
+
+
+def read_image(path, porevalue=0):
+    """A binary image file read and binarised by the host library
+    (ns3d_read_image): uint8 (nx, ny, nz), 0 = fluid, 1 = solid."""
+    dims = (C.c_int * 3)()
+    if lib().ns3d_read_image(str(path).encode(), int(porevalue), None, dims) != 0:
+        raise _gpu.NS3DError(lib().ns3d_host_last_error().decode())
+    out = np.empty(tuple(dims), np.uint8)
+    if lib().ns3d_read_image(str(path).encode(), int(porevalue), out.ctypes.data_as(C.POINTER(C.c_uint8)), dims) != 0:
+        raise _gpu.NS3DError(lib().ns3d_host_last_error().decode())
+    return out
 
 
 def default_params(**kw) -> RunParams:

@@ -13,6 +13,8 @@ It reads only data files of the reference (no code is imported or run):
   every solver report (component, iterations, status), every K line, the
   pressure-correction max/min/norm lines and the final divergence lines;
 * tests/results/RES-1-spheres_128.csv -> the RES rows as printed;
+* the INFO log's first 16 lines verbatim and the XML headers of the field
+  .vti files (up to the appended data);
 * tests/results/V{x,y,z}-1-spheres_128.vti -> the final face velocities on the
   node sets are summarised by per-component sums, sums of squares and the
   outlet-plane sum (enough to check a re-run's final fields and the outlet
@@ -106,6 +108,16 @@ def main(ref):
 
     res = os.path.join(ref, "tests", "results")
     gold["info"] = parse_info(os.path.join(res, "INFO-1-spheres_128.log"))
+    # the log's first lines as written (image summary, setup, first solves)
+    with open(os.path.join(res, "INFO-1-spheres_128.log")) as f:
+        gold["info_head"] = [next(f).rstrip("\n") for _ in range(16)]
+    # the .vti headers of the field files (XML up to the appended data)
+    heads = {}
+    for name in ("P", "Vx", "Vy", "Vz"):
+        with open(os.path.join(res, "%s-1-spheres_128.vti" % name), "rb") as f:
+            txt = f.read(2048).decode("latin-1")
+        heads[name] = txt[:txt.index("<AppendedData")]
+    gold["vti_heads"] = heads
     gold["res_header"], gold["res_rows"] = parse_res(os.path.join(res, "RES-1-spheres_128.csv"))
     gold["params"] = dict(input="spheres_128", dx=2e-6, dy=-1, dz=-1, mu=1e-3, rho=1e3, p0=1e5,
                           gradp=1e2, alpha=0.8, beta=1.0, vsolvertol=1e-5, psolvertol=1e-6,

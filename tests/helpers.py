@@ -11,6 +11,20 @@ def small_image(shape=(20, 17, 13), seed=1, porosity=0.6, rmin=1.5, rmax=3.5):
     return img
 
 
+def connected_image(shape, seed, porosity=0.62):
+    """A Boolean image whose fluid is one cluster joining inlet and outlet
+    (other clusters filled with solid), so every pressure system is
+    non-singular."""
+    from scipy import ndimage
+    img = small_image(shape, seed=seed, porosity=porosity)
+    lab, n = ndimage.label(img == 0)
+    keep = set(np.unique(lab[0])) & set(np.unique(lab[-1])) - {0}
+    assert keep, "no percolating cluster"
+    sizes = {q: int((lab == q).sum()) for q in keep}
+    big = max(sizes, key=sizes.get)
+    return np.where(lab == big, 0, 1).astype(np.uint8)
+
+
 def edge_images():
     """Degenerate and boundary cases of the index rule."""
     rng = np.random.default_rng(7)

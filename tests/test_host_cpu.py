@@ -122,3 +122,39 @@ def test_vti_roundtrip(tmp_path):
     v = rng.standard_normal((3, 4, 2, 3))
     vti.write_vti(str(tmp_path / "v.vti"), v, compress=False)
     assert np.array_equal(vti.read_vti(str(tmp_path / "v.vti")), v)
+
+
+REF_DATA = "/root/reference/data"
+
+
+@pytest.mark.parametrize("name", ["spheres_64", "spheres_128"])
+def test_product_reader_ingests_reference_images(name):
+    """The host library's reader + binarisation (readvtkfile_uint8,
+    vtkio.cxx:96-150) on the reference's own image files: shape and sha256 of
+    the C-order voxel array equal the golden values, and porevalue 1 swaps
+    fluid and solid."""
+    import hashlib
+    import json
+    path = os.path.join(REF_DATA, name + ".vti")
+    if not os.path.exists(path):
+        pytest.skip("the reference's data files are not present here")
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "spheres128_simplec.json")))["lattice"][name]
+    img = solver.read_image(path, 0)
+    assert list(img.shape) == gold["shape"]
+    assert hashlib.sha256(img.tobytes()).hexdigest() == gold["sha256"]
+    assert int(img.sum()) == gold["solid_voxels"]
+    assert np.array_equal(solver.read_image(path, 1), 1 - img)
+
+
+def test_reader_round_trip_and_errors(tmp_path):
+    """Files written by the Python codec (UInt32 headers, zlib, appended
+    base64, x fastest) read back through the C++ reader; a missing file is an
+    error with a message."""
+    rng = np.random.default_rng(3)
+    a = (rng.random((9, 7, 5)) < 0.4).astype(np.uint8)
+    for compress in (True, False):
+        f = tmp_path / f"a{int(compress)}.vti"
+        vti.write_vti(str(f), a, compress=compress)
+        assert np.array_equal(solver.read_image(f, 0), a)
+    with pytest.raises(Exception, match="nowhere"):
+        solver.read_image(tmp_path / "nowhere.vti", 0)

@@ -30,7 +30,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spl
 
 import oracle_lib as O
-from helpers import random_fields, small_image
+from helpers import connected_image, random_fields
 from ns3d_amd import generators, solver
 
 
@@ -39,20 +39,6 @@ def csr(cnt, col, val):
     mask = np.arange(7)[None, :] < cnt[:, None]
     rows = np.repeat(np.arange(n), cnt)
     return sp.csr_matrix((val[mask], (rows, col[mask])), shape=(n, n))
-
-
-def connected_image(shape, seed):
-    """A Boolean image whose fluid is one cluster joining inlet and outlet
-    (other clusters filled with solid), so every pressure system is
-    non-singular."""
-    from scipy import ndimage
-    img = small_image(shape, seed=seed, porosity=0.62)
-    lab, n = ndimage.label(img == 0)
-    keep = set(np.unique(lab[0])) & set(np.unique(lab[-1])) - {0}
-    assert keep, "no percolating cluster"
-    sizes = {q: int((lab == q).sum()) for q in keep}
-    big = max(sizes, key=sizes.get)
-    return np.where(lab == big, 0, 1).astype(np.uint8)
 
 
 def prepared(seed=3, shape=(14, 12, 11), **kw):
