@@ -107,6 +107,10 @@ def stream(reqs, args, precision, label, warm_steps, steps):
                 r = srv.poll(t)
                 if r is not None:
                     done.append((pending.pop(t), r, time.perf_counter() - t0))
+                    if len(done) % args.per_step == 0:
+                        k = len(done) // args.per_step
+                        log(f"  [{label}] step {k - warm_steps if k > warm_steps else 'warm-up ' + str(k)} done "
+                            f"at {done[-1][2]:.1f} s")
             time.sleep(0.002)
         est = srv.engine_stats()
         sst = srv.stats()
@@ -131,10 +135,11 @@ def plain_pass(reqs, args):
     solvers = [solver.Solver(img, **mix_params(img, args, 0)) for img, _ in reqs]
     out, secs = [], []
     t0 = time.perf_counter()
-    for s in solvers:
+    for q, s in enumerate(solvers):
         t1 = time.perf_counter()
         out.append(s.run()[1])
         secs.append(time.perf_counter() - t1)
+        log(f"    plain {q}: {reqs[q][1]['shape']} {out[-1]['iterations']} outer its, {secs[-1]:.2f} s")
     wall = time.perf_counter() - t0
     for s in solvers:
         s.close()
@@ -214,7 +219,7 @@ def mix_section(args):
             p = mix_params(img, args, 1)
             p["maxitSIMPLE"] = args.tune_maxit
             sample.append((img, p))
-        tuned = autotune.tune(sample, candidates=[(c, l, True) for c in (4, 6, 8) for l in (2, 3)] + [(6, 3, False)],
+        tuned = autotune.tune(sample, candidates=[(c, l, True) for c in (4, 8) for l in (2, 3)] + [(8, 3, False)],
                               log=log, budget_gb=args.budget_gb, slots=args.slots)
         args.chunk, args.lanes, args.no_graphs = tuned["chunk"], tuned["lanes"], not tuned["graphs"]
         log(f"  chosen: chunk {args.chunk}, lanes {args.lanes}, graphs {int(not args.no_graphs)}")
@@ -467,7 +472,7 @@ def main():
     ap.add_argument("--poisson-frac", type=float, default=0.5)
     ap.add_argument("--mix-forms", default="all", choices=["all", "stream"],
                     help="stream: only the timed served mixed stream, no JSON line (for profilers)")
-    ap.add_argument("--tune-requests", type=int, default=12)
+    ap.add_argument("--tune-requests", type=int, default=8)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--radius", type=float, default=10.0)
     ap.add_argument("--porosity", type=float, default=0.75)
@@ -494,7 +499,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=3, help="served forms: engine streams")
     ap.add_argument("--max-rows", type=float, default=4e6,
                     help="served forms: cap on the fluid cells of the running requests (0: none)")
-    ap.add_argument("--tune-maxit", type=int, default=60, help="autotuner: outer-iteration cap of its trials")
+    ap.add_argument("--tune-maxit", type=int, default=40, help="autotuner: outer-iteration cap of its trials")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="pick chunk / lanes / graphs with ns3d_amd.autotune before the mix (full runs)")
     ap.add_argument("--progress", action="store_true", help="one stderr line per outer iteration (single)")

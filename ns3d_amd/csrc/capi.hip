@@ -410,12 +410,13 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
   // the FP32 sweeps get at most half the iteration budget, so that an FP64
   // finish always has the other half
   const int64_t budget32 = std::max<int64_t>(1, maxit / 2);
-  auto residuals = [&](const bool* which) {
+  // |b - A x|^2 (trial: of the trial iterate x + d, x unchanged) and |b|^2
+  auto residuals = [&](const bool* which, bool trial) {
     bool any = false;
     for (int i = 0; i < n; ++i) {
       if (!which[i]) continue;
       HCHK(launch_residual64(c->sys[sid[i]].mat64(), x[i], b[i], L[i]->rres, L[i]->r32, L[i]->partials,
-                             L[i]->result, s));
+                             L[i]->result, s, trial ? L[i]->d32 : nullptr));
       HCHK(hipMemcpyAsync(L[i]->hresult, L[i]->result, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
       any = true;
     }
@@ -433,7 +434,7 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
     use[i] = c->sys[sid[i]].n > 0;
     if (!use[i]) { info[i].iterations = 0; info[i].info = 0; info[i].error = 0; }
   }
-  residuals(use);
+  residuals(use, false);
   for (int i = 0; i < n; ++i) {
     if (!use[i]) continue;
     if (t[i].bb == 0) {
@@ -485,22 +486,25 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
         k.live = false;
         continue;
       }
-      HCHK(launch_add_f32(x[i], L[i]->d32, c->sys[sid[i]].n, s));
       k.rr_old = k.rr;
       again[i] = true;
     }
-    residuals(again);
+    // the FP64 residual of x + d decides whether the correction is applied
+    residuals(again, true);
     for (int i = 0; i < n; ++i) {
       if (!again[i]) continue;
       Track& k = t[i];
       if (!(k.rr <= k.rr_old)) {
-        // the correction made |r| larger (FP32 sweeps past their accuracy on
-        // an ill-conditioned system): take it back
-        HCHK(launch_add_f32(x[i], L[i]->d32, c->sys[sid[i]].n, s, -1.0));
+        // the correction would make |r| larger, or is not finite (FP32 sweeps
+        // past their accuracy, or overflowed, on an ill-conditioned system):
+        // x stays as it was
         k.rr = k.rr_old;
         k.stagnated = true;
         k.live = false;
-      } else if (k.rr > 0.25 * k.rr_old) {  // |r| fell by less than 2x
+        continue;
+      }
+      HCHK(launch_add_f32(x[i], L[i]->d32, c->sys[sid[i]].n, s));
+      if (k.rr > 0.25 * k.rr_old) {  // |r| fell by less than 2x
         k.stagnated = true;
         k.live = false;
       }

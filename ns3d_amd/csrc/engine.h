@@ -158,6 +158,10 @@ struct ns3d_engine {
   int64_t group_rows(int g) const;
 
   static constexpr int kSampleEvery = 8;
+  // dispatch events on the sampled rounds (for stats); NS3D_ENGINE_EVENTS=0
+  // turns them off (no kernel times in the statistics), e.g. under a
+  // profiler that records every dispatch itself
+  bool events = true;
   long long rounds_started = 0;
   bool round_sampled[2] = {};
   std::unordered_map<uint64_t, hipGraphExec_t> gcache[kMaxLanes];

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -84,6 +85,7 @@ SolveDesc<float>& desc_of<float>(EngineRequest* r) { return r->d32; }
 
 ns3d_engine::ns3d_engine(int dev, int ch) : device(dev), chunk(ch > 0 ? ch : 8) {
   ECHK(hipSetDevice(device));
+  if (const char* e = std::getenv("NS3D_ENGINE_EVENTS")) events = std::atoi(e) != 0;
   if (NS3D_CTX_STREAM) ECHK(hipStreamCreateWithFlags(&ctx_stream, hipStreamNonBlocking));
   // streams of the lanes in use only (each takes one of the process's HIP
   // hardware queues); more are created if the schedule asks for them
@@ -449,7 +451,7 @@ void ns3d_engine::queue_round(int buf, bool restart[kGroups]) {
   int ngroups = 0;
   for (int g = 0; g < kGroups; ++g) ngroups += (on[g] = !grp[g].active.empty());
   const bool sampled = !graphs || rounds_started % kSampleEvery == 0;
-  round_sampled[buf] = sampled;
+  round_sampled[buf] = sampled && events;
   ++rounds_started;
   if (sampled) {
     launch_direct(buf, on);
@@ -499,7 +501,8 @@ void ns3d_engine::launch_direct(int buf, const bool* on) {
         if (!on[g]) continue;
         hipStream_t gs = lane_stream[g % kMaxLanes];
         hipEvent_t* ev = nullptr;
-        if (q == qt) ev = tev[buf] + kEv * g + 2 * k;
+        if (!events) ev = nullptr;
+        else if (q == qt) ev = tev[buf] + kEv * g + 2 * k;
         else if (span && q == 0 && k == 0) ev = tev[buf] + kEv * g + 6;
         else if (span && q == chunk - 1 && k == 2) ev = tev[buf] + kEv * g + 8;
         if (g / kMaxLanes == 0) {

@@ -191,7 +191,10 @@ template <typename T>
 Batch<T> single_batch(const SolveDesc<T>& d, long long it0, int q);
 // Tiles of one solve of n rows: between 1 and kMaxTilesPerSolve tiles of
 // a multiple of kBlock rows, at least kTileRows rows each where n allows.
-constexpr int kMaxTilesPerSolve = 1024;
+#ifndef NS3D_MAX_TILES
+#define NS3D_MAX_TILES 1024
+#endif
+constexpr int kMaxTilesPerSolve = NS3D_MAX_TILES;
 #ifndef NS3D_TILE_ROWS
 #define NS3D_TILE_ROWS 1024
 #endif
@@ -235,8 +238,10 @@ hipError_t launch_pack_neighbours(const int32_t* nbr, int n, uint64_t* pk, int* 
 hipError_t launch_coef_to_f32(const double* coef, const double* invdiag, int64_t n, float* coef32,
                               float* invdiag32, hipStream_t s);
 // r = b - A x in FP64, r32 = (float) r; result[0] = |r|^2, result[1] = |b|^2.
+// d (may be null): the residual of the trial iterate x + (double) d, x itself
+// unchanged (the mixed form applies a correction only once it is accepted).
 hipError_t launch_residual64(SMat<double> A, const double* x, const double* b, double* r, float* r32,
-                             double* partials, double* result, hipStream_t s);
+                             double* partials, double* result, hipStream_t s, const float* d = nullptr);
 // x += (double) d
 // x += a d (a = 1: the correction; a = -1: taking it back)
 hipError_t launch_add_f32(double* x, const float* d, int64_t n, hipStream_t s, double a = 1.0);

@@ -728,15 +728,19 @@ coef_to_f32_kernel(const double* __restrict__ coef, const double* __restrict__ i
 }
 
 // FP64 residual of the defect correction: r = b - A x, r32 = (float) r.
-template <int L>
+template <int L, bool TRIAL>
 __global__ void __launch_bounds__(kBlock)
-residual64_kernel(SMat<double> A, const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r,
-                  float* __restrict__ r32, double* partials) {
+residual64_kernel(SMat<double> A, const double* __restrict__ x, const float* __restrict__ d,
+                  const double* __restrict__ b, double* __restrict__ r, float* __restrict__ r32, double* partials) {
   const int64_t n = A.n;
   const RowIter rr = block_rows(n);
   double v2[2] = {0.0, 0.0};
   for (int64_t row = rr.begin + threadIdx.x; row < rr.end; row += rr.step) {
-    const double ax = stencil_row<double, L>(A, (int)row, n, [&](int q) { return x[q]; });
+    // the trial iterate x + d, as add_f32 would form it
+    const double ax = stencil_row<double, L>(A, (int)row, n, [&](int q) {
+      if constexpr (TRIAL) return x[q] + 1.0 * (double)d[q];
+      else return x[q];
+    });
     const double bv = b[row];
     const double rv = bv - ax;
     r[row] = rv;
@@ -932,12 +936,16 @@ hipError_t launch_pack_neighbours(const int32_t* nbr, int n, uint64_t* pk, int* 
 }
 
 hipError_t launch_residual64(SMat<double> A, const double* x, const double* b, double* r, float* r32, double* partials,
-                             double* result, hipStream_t s) {
+                             double* result, hipStream_t s, const float* d) {
   const int nb = reduce_blocks(A.n);
-  auto k = A.layout == ML_PACKED_SYM ? residual64_kernel<ML_PACKED_SYM>
-         : A.layout == ML_PACKED     ? residual64_kernel<ML_PACKED>
-                                     : residual64_kernel<ML_INDEXED>;
-  hipLaunchKernelGGL(k, dim3(nb), dim3(kBlock), 0, s, A, x, b, r, r32, partials);
+  auto pick = [&](auto trial) {
+    constexpr bool T = decltype(trial)::value;
+    return A.layout == ML_PACKED_SYM ? residual64_kernel<ML_PACKED_SYM, T>
+         : A.layout == ML_PACKED     ? residual64_kernel<ML_PACKED, T>
+                                     : residual64_kernel<ML_INDEXED, T>;
+  };
+  auto k = d ? pick(std::true_type{}) : pick(std::false_type{});
+  hipLaunchKernelGGL(k, dim3(nb), dim3(kBlock), 0, s, A, x, d, b, r, r32, partials);
   hipLaunchKernelGGL(sums_fin, dim3(1), dim3(kBlock), 0, s, partials, nb, result);
   return hipGetLastError();
 }

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -137,6 +138,10 @@ void ns3d_server::run_job(Job& j) {
   if (mixed && fallback >= NS3D_FALLBACK_FAILED && broken) {
     // the mixed run failed or produced non-finite values: run it again in
     // FP64 from the start
+    std::fprintf(stderr, "ns3d server: request %lld (%dx%dx%d): mixed-precision run %s; running it again in FP64\n",
+                 (long long)j.id, j.nx, j.ny, j.nz,
+                 failure.empty() ? (std::isfinite(j.res.K) ? "gave a non-finite residual" : "gave a non-finite K")
+                                 : ("failed: " + failure).c_str());
     p.precision = NS3D_PRECISION_FP64;
     attempt(p, j.res, nullptr, nullptr);
     j.res.fallback = 1;
