@@ -198,6 +198,26 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
                                   const double* b, double* x, double tol, int64_t maxit,
                                   ns3d_solve_info* info);
 
+/* ns3d_bicgstab_stencil without any device allocation: the same solve, the
+ * same kernels, bit-identical results, with diag, off, nbr, b and x in device
+ * memory of the current device and the work queued on `stream` (a
+ * hipStream_t; NULL = the null stream), which the call synchronizes.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (serving loops, harnesses that forbid hipMalloc).  The
+ * workspace (coefficients, M^-1, the BiCGSTAB vectors, partials, state) is
+ * one fixed array per GPU, held until the process ends, sized for
+ * n <= ns3d_noalloc_max_rows() (1048576 rows, 120.1 MiB); a larger n is
+ * NS3D_ERR_INVALID, as are pointers that are not device memory of the
+ * current device and neighbour rows outside [0, n) (checked on the device
+ * before the solve).  Calls that overlap on one GPU share that array and
+ * race: serialize them. */
+ns3d_status ns3d_bicgstab_stencil_noalloc(int n, const double* diag, const double* off, const int32_t* nbr,
+                                          const double* b, double* x, double tol, int64_t maxit,
+                                          ns3d_solve_info* info, void* stream);
+
+/* The row capacity of ns3d_bicgstab_stencil_noalloc's workspace. */
+ns3d_status ns3d_noalloc_max_rows(int64_t* rows);
+
 /* Linear-solve arithmetic.  NS3D_PRECISION_FP64 (default) is the reference's
  * Eigen BiCGSTAB in FP64.  NS3D_PRECISION_MIXED keeps assembly, residuals,
  * corrections and reductions in FP64 and solves each system by FP64 defect

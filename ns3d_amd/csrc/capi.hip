@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -244,57 +245,65 @@ struct SolveJob {
   int64_t* hint = nullptr;
 };
 
-// Queued BiCGSTAB iterations on the context's stream with status polling
-// between chunks.  Leaves the final solver state in the lane's hstate.  The
-// hint (iterations of the previous solve of this system) sizes the first
-// chunk and is updated on return.
-template <typename T>
-void stream_solve(ns3d_ctx* c, const SolveJob<T>& j) {
-  hipStream_t s = c->stream;
-  const SysMem& S = c->sys[j.sid];
-  const int kind = j.sid == 3 ? 1 : 0;
-  BState* hs = j.L->hstate;
-  // one solve: its descriptor travels by value in the kernel arguments
-  const SolveDesc<T> d{j.M, j.w, j.x, j.b, j.tol, (long long)j.maxit, j.L->partials, kMaxTilesPerSolve, 0, 0, 0};
+// Host loop of one queued BiCGSTAB solve on stream s: init, then chunks of
+// iterations with status polling between them.  The solver state is copied
+// into *hs after each chunk and sync() waits for it.  hint: iterations of the
+// previous solve of this system, sizes the first chunk; updated on return.
+// tev (may be null): six events that time kernels A, B, C of one iteration
+// in the middle of each chunk, after which sampled() reads them if that
+// iteration did work.
+template <typename T, typename Sync, typename Sampled>
+void queued_solve(hipStream_t s, const SolveDesc<T>& d, BState* hs, int64_t& hint, hipEvent_t* tev, Sync&& sync,
+                  Sampled&& sampled) {
   HCHK(launch_bicg_init<T>(single_batch(d, 0, 0), s));
-  HCHK(hipMemcpyAsync(hs, j.w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
-  HCHK(ctx_sync(c));
-  int64_t& hint = *j.hint;
+  HCHK(hipMemcpyAsync(hs, d.w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
+  sync();
   int64_t chunk = hint > 4 ? (hint * 3) / 4 : 4;
   int64_t launched = 0;
-  const int64_t maxit = j.maxit;
+  const int64_t maxit = d.maxit;
   const int64_t cap = 4 * (maxit > 0 ? maxit : 1) + 64;
-  const int prec = sizeof(T) == 4 ? 1 : 0;
   while (hs->status == BS_RUNNING || hs->status == BS_RESTART) {
     const long long done_before = hs->it_c;
     if (hs->status == BS_RESTART) HCHK(launch_bicg_restart<T>(single_batch(d, done_before, 0), s));
     if (chunk > maxit + 1) chunk = maxit + 1;
-    // with timing on, the kernels of one iteration in the middle of the
-    // chunk carry start/stop events, read back after the chunk's sync if
-    // that iteration did work
-    const int64_t qt = c->tm.on ? chunk / 2 : -1;
+    const int64_t qt = tev ? chunk / 2 : -1;
     for (int64_t q = 0; q < chunk; ++q) {
-      hipEvent_t* e = (q == qt) ? c->tev.data() : nullptr;
+      hipEvent_t* e = (q == qt) ? tev : nullptr;
       const Batch<T> B = single_batch(d, done_before, (int)q);
       HCHK(launch_bicg_A<T>(B, s, e));
       HCHK(launch_bicg_B<T>(B, s, e ? e + 2 : nullptr));
       HCHK(launch_bicg_C<T>(B, s, e ? e + 4 : nullptr));
     }
-    HCHK(hipMemcpyAsync(hs, j.w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
-    HCHK(ctx_sync(c));
-    if (qt >= 0 && hs->it_c - done_before > qt)
-      for (int k = 0; k < 3; ++k) {
-        float ms = 0.f;
-        HCHK(hipEventElapsedTime(&ms, c->tev[2 * k], c->tev[2 * k + 1]));
-        c->tm.ms[prec][kind][k] += ms;
-        c->tm.launches[prec][kind][k] += 1;
-        c->tm.bytes[prec][kind][k] += kernel_bytes(S.n, S.nnz, k, (int)sizeof(T));
-      }
+    HCHK(hipMemcpyAsync(hs, d.w.st, sizeof(BState), hipMemcpyDeviceToHost, s));
+    sync();
+    if (qt >= 0 && hs->it_c - done_before > qt) sampled();
     launched += chunk;
     if (launched > cap) throw Invalid{"BiCGSTAB did not stop within its iteration cap"};
     chunk = 16;
   }
   if (hs->status != BS_ZERO_RHS) hint = hs->it_c > 0 ? hs->it_c : 4;
+}
+
+// One solve queued on the context's stream.  Leaves the final solver state
+// in the lane's hstate; with timing on, one iteration per chunk is timed.
+template <typename T>
+void stream_solve(ns3d_ctx* c, const SolveJob<T>& j) {
+  const SysMem& S = c->sys[j.sid];
+  const int kind = j.sid == 3 ? 1 : 0;
+  const int prec = sizeof(T) == 4 ? 1 : 0;
+  // one solve: its descriptor travels by value in the kernel arguments
+  const SolveDesc<T> d{j.M, j.w, j.x, j.b, j.tol, (long long)j.maxit, j.L->partials, kMaxTilesPerSolve, 0, 0, 0};
+  queued_solve<T>(c->stream, d, j.L->hstate, *j.hint, c->tm.on ? c->tev.data() : nullptr,
+                  [&] { HCHK(ctx_sync(c)); },
+                  [&] {
+                    for (int k = 0; k < 3; ++k) {
+                      float ms = 0.f;
+                      HCHK(hipEventElapsedTime(&ms, c->tev[2 * k], c->tev[2 * k + 1]));
+                      c->tm.ms[prec][kind][k] += ms;
+                      c->tm.launches[prec][kind][k] += 1;
+                      c->tm.bytes[prec][kind][k] += kernel_bytes(S.n, S.nnz, k, (int)sizeof(T));
+                    }
+                  });
 }
 
 // Runs the solves, each leaving its final state in its lane's hstate: with an
@@ -339,6 +348,26 @@ void run_solves(ns3d_ctx* c, SolveJob<T>* jobs, int n) {
   }
 }
 
+// The result of a plain FP64 solve from its final state, as Eigen reports
+// it.  Returns true for a zero right-hand side, whose solution is x = 0 (the
+// caller writes it) with iterations = maxit and error = tol.
+bool solve_info(const BState& st, double tol, int64_t maxit, ns3d_solve_info& info) {
+  info.restarts = 0;
+  info.work = 0;
+  if (st.status == BS_ZERO_RHS) {
+    info.iterations = maxit;
+    info.error = tol;
+    info.info = 0;
+    return true;
+  }
+  info.iterations = st.it_c - st.ibase;
+  info.work = st.it_c;
+  info.error = std::sqrt(st.rr / st.rhs_sqnorm);
+  info.info = (info.error <= tol) ? 0 : 2;
+  info.restarts = (int32_t)st.restarts;
+  return false;
+}
+
 // n BiCGSTAB solves (system sid[i], unknown x[i] with its guess, right-hand
 // side b[i], lane L[i]; at most three, on distinct systems and lanes), Eigen
 // semantics, FP64.
@@ -359,21 +388,7 @@ void bicgstab64_many(ns3d_ctx* c, int n, const int* sid, double* const* x, const
   run_solves<double>(c, jobs, m);
   for (int q = 0; q < m; ++q) {
     const int i = idx[q];
-    const BState& st = *L[i]->hstate;
-    if (st.status == BS_ZERO_RHS) {
-      HCHK(launch_fill(x[i], c->sys[sid[i]].n, 0.0, c->stream));
-      info[i].iterations = maxit;
-      info[i].error = tol;
-      info[i].info = 0;
-      info[i].restarts = 0;
-      info[i].work = 0;
-    } else {
-      info[i].iterations = st.it_c - st.ibase;
-      info[i].work = st.it_c;
-      info[i].error = std::sqrt(st.rr / st.rhs_sqnorm);
-      info[i].info = (info[i].error <= tol) ? 0 : 2;
-      info[i].restarts = (int32_t)st.restarts;
-    }
+    if (solve_info(*L[i]->hstate, tol, maxit, info[i])) HCHK(launch_fill(x[i], c->sys[sid[i]].n, 0.0, c->stream));
   }
 }
 
@@ -1124,6 +1139,116 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
   });
   ns3d_destroy(c);
   return st;
+}
+
+}  // extern "C"
+
+// ---- the stand-alone solve without allocation --------------------------------
+// Workspace of ns3d_bicgstab_stencil_noalloc, one copy per GPU for the life of
+// the process: the matrix coefficients [7][n] and M^-1, the seven BiCGSTAB
+// vectors, the block partials, the solver state and a flag (in doubles).
+// Vector regions start on 256-byte boundaries.  At the default capacity of
+// 2^20 rows the array takes 120.1 MiB of device memory.
+#ifndef NS3D_NOALLOC_MAX_ROWS
+#define NS3D_NOALLOC_MAX_ROWS 1048576
+#endif
+namespace {
+constexpr int64_t kNoAllocVec = NS3D_NOALLOC_MAX_ROWS;  // a multiple of 32
+constexpr int64_t kNoAllocPartials = (int64_t)kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks);
+constexpr int64_t kNoAllocDoubles = 15 * kNoAllocVec + kNoAllocPartials + 64;
+static_assert(kNoAllocVec % 32 == 0, "row capacity must keep the regions 256-byte aligned");
+static_assert(sizeof(BState) <= 56 * sizeof(double), "solver state does not fit its slot");
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoAllocDevices] = {};
+
+// The device address of this GPU's scratch copy, looked up once per device
+// (with that device current); the array's name is not a device address in
+// host code.
+double* scratch_of(int dev) {
+  if (dev < 0 || dev >= kNoAllocDevices) throw Invalid{"ns3d_bicgstab_stencil_noalloc: device ordinal out of range"};
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
+// Throws unless p is device memory of device dev.
+void need_device_ptr(const void* p, int dev, const char* what) {
+  hipPointerAttribute_t a{};
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Invalid{std::string("ns3d_bicgstab_stencil_noalloc: ") + what + " is not device memory"};
+  }
+  if ((a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged) || a.device != dev)
+    throw Invalid{std::string("ns3d_bicgstab_stencil_noalloc: ") + what + " is not device memory of the current device"};
+}
+
+}  // namespace
+
+extern "C" {
+
+ns3d_status ns3d_bicgstab_stencil_noalloc(int n, const double* diag, const double* off, const int32_t* nbr,
+                                          const double* b, double* x, double tol, int64_t maxit,
+                                          ns3d_solve_info* info, void* stream) {
+  if (n <= 0 || !diag || !off || !nbr || !b || !x || !info)
+    return fail_invalid("ns3d_bicgstab_stencil_noalloc: bad arguments");
+  if (n > kNoAllocVec)
+    return fail_invalid("ns3d_bicgstab_stencil_noalloc: n = " + std::to_string(n) +
+                        " exceeds the workspace maximum of " + std::to_string(kNoAllocVec) + " rows");
+  return guarded([&] {
+    int dev = 0;
+    HCHK(hipGetDevice(&dev));
+    need_device_ptr(diag, dev, "diag");
+    need_device_ptr(off, dev, "off");
+    need_device_ptr(nbr, dev, "nbr");
+    need_device_ptr(b, dev, "b");
+    need_device_ptr(x, dev, "x");
+    hipStream_t s = (hipStream_t)stream;
+    double* base = scratch_of(dev);
+    const int64_t m = ((int64_t)n + 31) / 32 * 32;
+    double* coef = base;                      // [7][n]: diag, then E, W, N, S, T, B
+    double* invdiag = base + 7 * m;
+    double* vec = base + 8 * m;               // r, r0, p0, p1, v0, v1, t
+    double* partials = base + 15 * m;
+    BState* st = (BState*)(partials + kNoAllocPartials);
+    double* bad = partials + kNoAllocPartials + 56;
+    // neighbour rows in range, checked on the device before any solver
+    // kernel gathers through them
+    HCHK(launch_fill(bad, 1, 0.0, s));
+    HCHK(launch_check_index(nbr, (int64_t)6 * n, n, bad, s));
+    double hbad = 0.0;
+    HCHK(hipMemcpyAsync(&hbad, bad, sizeof(double), hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    if (hbad != 0.0) throw Invalid{"ns3d_bicgstab_stencil_noalloc: neighbour index out of range"};
+    HCHK(hipMemcpyAsync(coef, diag, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    HCHK(hipMemcpyAsync(coef + n, off, (size_t)6 * n * 8, hipMemcpyDeviceToDevice, s));
+    HCHK(launch_invdiag(SysView{n, 0, nullptr, nullptr, nbr, coef, invdiag, nullptr}, s));
+    const SMat<double> M{n, ML_INDEXED, nbr, nullptr, coef, invdiag, nullptr};
+    const BWork w{vec, vec + m, {vec + 2 * m, vec + 3 * m}, {vec + 4 * m, vec + 5 * m}, vec + 6 * m, st};
+    const SolveDesc<double> d{M, w, x, b, tol, (long long)maxit, partials, kMaxTilesPerSolve, 0, 0, 0};
+    BState hs{};
+    int64_t hint = 16;
+    queued_solve<double>(s, d, &hs, hint, nullptr, [&] { HCHK(hipStreamSynchronize(s)); }, [] {});
+    if (solve_info(hs, tol, maxit, *info)) HCHK(launch_fill(x, n, 0.0, s));
+    HCHK(hipStreamSynchronize(s));
+  });
+}
+
+ns3d_status ns3d_noalloc_max_rows(int64_t* rows) {
+  if (!rows) return fail_invalid("ns3d_noalloc_max_rows: null argument");
+  *rows = kNoAllocVec;
+  return NS3D_OK;
 }
 
 ns3d_status ns3d_timing(ns3d_ctx* c, int enable) {

@@ -249,6 +249,9 @@ hipError_t launch_fill_f32(float* x, int64_t n, hipStream_t s);
 
 // ---- fields, corrections, diagnostics (kernels_field.hip) -------------------
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);
+// *bad = 1.0 if an entry of idx[count] lies outside [0, n) (bad untouched
+// otherwise: the caller clears it).
+hipError_t launch_check_index(const int32_t* idx, int64_t count, int n, double* bad, hipStream_t s);
 hipError_t launch_init_pressure_linear(const Geo& g, const int32_t* pcell, int np, double* p, double p0,
                                        double p1, hipStream_t s);
 hipError_t launch_clamp(double* x, const double* temp, int64_t n, double dvmax, hipStream_t s);

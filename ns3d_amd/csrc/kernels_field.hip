@@ -12,6 +12,13 @@ __global__ void __launch_bounds__(kBlock) fill_kernel(double* x, int64_t n, doub
   if (i < n) x[i] = v;
 }
 
+// *bad = 1 if any idx[i] (i < count) is outside [0, n).  Every writer stores
+// the same value, so the flag needs no atomic.
+__global__ void __launch_bounds__(kBlock) check_index_kernel(const int32_t* idx, int64_t count, int n, double* bad) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < count && (idx[i] < 0 || idx[i] >= n)) *bad = 1.0;
+}
+
 // Initpvector, linear profile (NSBase.cxx:2671-2681).
 __global__ void __launch_bounds__(kBlock)
 init_p_kernel(Geo g, const int32_t* __restrict__ pcell, int np, double* __restrict__ p, double p0, double p1) {
@@ -196,6 +203,13 @@ __global__ void __launch_bounds__(kBlock) perm_fin(const double* partials, int n
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, x, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_check_index(const int32_t* idx, int64_t count, int n, double* bad, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(check_index_kernel, dim3((unsigned)((count + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, idx,
+                     count, n, bad);
   return hipGetLastError();
 }
 

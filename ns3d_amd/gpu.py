@@ -33,7 +33,7 @@ EXPORTS = [
     "ns3d_compute_pressure_correction", "ns3d_compute_pressure_field", "ns3d_correct_velocity",
     "ns3d_correct_pressure", "ns3d_compute_div", "ns3d_compute_permeability", "ns3d_get_matrix",
     "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_set_precision",
-    "ns3d_set_mixed_min_rows", "ns3d_sync",
+    "ns3d_set_mixed_min_rows", "ns3d_sync", "ns3d_bicgstab_stencil_noalloc", "ns3d_noalloc_max_rows",
     "ns3d_engine_create", "ns3d_engine_destroy", "ns3d_attach_engine", "ns3d_engine_stats",
     "ns3d_engine_set_graphs", "ns3d_engine_set_schedule", "ns3d_engine_get_schedule",
     "ns3d_footprint", "ns3d_ctx_bytes",
@@ -111,6 +111,8 @@ def lib():
         L.ns3d_get_matrix.argtypes = [vp, i32, P(dbl), P(dbl), P(C.c_int32), P(dbl)]
         L.ns3d_bicgstab_stencil.argtypes = [i32, P(dbl), P(dbl), P(C.c_int32), P(dbl), P(dbl), dbl, i64,
                                             P(SolveInfo)]
+        L.ns3d_bicgstab_stencil_noalloc.argtypes = [i32, vp, vp, vp, vp, vp, dbl, i64, P(SolveInfo), vp]
+        L.ns3d_noalloc_max_rows.argtypes = [P(i64)]
         L.ns3d_timing.argtypes = [vp, i32]
         L.ns3d_timing_read.argtypes = [vp, P(dbl)]
         L.ns3d_matrix_nnz.argtypes = [vp, P(i64)]
@@ -390,3 +392,29 @@ def bicgstab_stencil(diag, off, nbr, b, x0, tol, maxit):
                                        _p(nbr, C.c_int32), _p(b, C.c_double), _p(x, C.c_double), tol, maxit,
                                        C.byref(info)))
     return x, info.as_tuple()
+
+
+def noalloc_max_rows() -> int:
+    rows = C.c_int64()
+    _check(lib().ns3d_noalloc_max_rows(C.byref(rows)))
+    return rows.value
+
+
+def bicgstab_stencil_noalloc(diag, off, nbr, b, x, tol, maxit, stream=None):
+    """The stand-alone solve with no device allocation: diag, off, nbr, b and
+    x are contiguous tensors on the current GPU (float64, float64, int32,
+    float64, float64); x holds the guess and is overwritten with the
+    solution.  stream: a torch stream to queue on (default: the current
+    one)."""
+    import torch
+    n = int(diag.numel())
+    for t, name, dt, size in ((diag, "diag", torch.float64, n), (off, "off", torch.float64, 6 * n),
+                              (nbr, "nbr", torch.int32, 6 * n), (b, "b", torch.float64, n),
+                              (x, "x", torch.float64, n)):
+        if t.dtype != dt or not t.is_cuda or not t.is_contiguous() or t.numel() != size:
+            raise ValueError(f"{name}: expected a contiguous {dt} tensor of {size} elements on the GPU")
+    s = stream if stream is not None else torch.cuda.current_stream()
+    info = SolveInfo()
+    _check(lib().ns3d_bicgstab_stencil_noalloc(n, diag.data_ptr(), off.data_ptr(), nbr.data_ptr(), b.data_ptr(),
+                                               x.data_ptr(), tol, maxit, C.byref(info), s.cuda_stream))
+    return info.as_tuple()
