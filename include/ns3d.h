@@ -238,6 +238,21 @@ ns3d_status ns3d_set_precision(ns3d_ctx* ctx, int mode, double inner_tol);
  * in the mixed form.  Not part of the reference. */
 ns3d_status ns3d_set_mixed_min_rows(ns3d_ctx* ctx, int64_t rows);
 
+/* Mixed form only: pressure-type systems (p' and p) with at least `rows`
+ * rows get FP32 sweeps of conjugate gradients preconditioned by one
+ * aggregation-multigrid V-cycle (2x2x2 blocks per level, Galerkin coarse
+ * operators rebuilt after every pressure assembly) instead of Jacobi
+ * BiCGSTAB; the FP64 defect correction around them, its tolerance and its
+ * true-residual contract are unchanged.  The pressure matrices are
+ * symmetric positive (semi-)definite M-matrices, so CG applies.  0: never;
+ * default 32768.  Not part of the reference (its solver is Eigen's Jacobi
+ * BiCGSTAB, which the plain FP64 form keeps). */
+ns3d_status ns3d_set_multigrid(ns3d_ctx* ctx, int64_t rows);
+/* Levels of the pressure system's multigrid hierarchy (0 until the first
+ * multigrid sweep has built it) and their row counts (rows[kMax], up to 12
+ * entries written; rows may be NULL). */
+ns3d_status ns3d_multigrid_levels(ns3d_ctx* ctx, int* levels, int64_t* rows);
+
 /* Device time of the fused BiCGSTAB kernels (enable with ns3d_timing(ctx,
  * 1)): one iteration per queued chunk of iterations is sampled, its three
  * kernels launched with start/stop events taken from their own dispatches

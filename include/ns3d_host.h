@@ -39,6 +39,10 @@ typedef struct {
   int32_t mixed_min_rows;       /* mixed form: systems with fewer rows are solved in
                                    FP64 (ns3d_set_mixed_min_rows; 0: none) */
   double inner_tol;             /* mixed form: relative tolerance of the FP32 sweeps */
+  int32_t mg_min_rows;          /* mixed form: pressure systems from this many rows
+                                   on are swept by multigrid CG (ns3d_set_multigrid;
+                                   0: never; default 32768) */
+  int32_t pad_params;
 } ns3d_run_params;
 
 typedef struct {

@@ -93,6 +93,26 @@ struct Lane {
   double* hresult = nullptr;   // pinned copy of results
 };
 
+// The multigrid hierarchy of the pressure system (mixed form), built on the
+// first inner solve that uses it.  Levels hold device pointers into the
+// context's allocations; coef / invd are the writable views of levels >= 1.
+// Default of ns3d_set_multigrid: pressure systems from this many rows on.
+constexpr int64_t kMgDefaultMinRows = 32768;
+
+struct MgHier {
+  bool tried = false;  // build attempted (ok: it succeeded)
+  bool ok = false;
+  bool dirty = true;   // coarse operators stale (the pressure system was reassembled)
+  int nlev = 0;
+  MgLevel lv[kMgMaxLevels]{};
+  float* coef[kMgMaxLevels] = {};
+  float* invd[kMgMaxLevels] = {};
+  MgFcg G{};
+  MgState* hst = nullptr;  // pinned copy of the state
+  int64_t hint = 8;        // iterations of the last solve (sizes the first chunk)
+  int coarse_iters = 24;   // CG steps of the coarsest solve
+};
+
 }  // namespace
 
 struct ns3d_ctx {
@@ -129,6 +149,10 @@ struct ns3d_ctx {
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> tev;  // start / stop of A, B, C of the sampled iteration
   ns3d_engine* engine = nullptr;  // batched solve engine (null: solve on the context's stream)
+  // mixed form: pressure systems with at least this many rows are solved by
+  // multigrid-preconditioned flexible CG (0: never)
+  int64_t mg_min_rows = kMgDefaultMinRows;
+  MgHier mg;
 
   Geo geo() const { return Geo{nx, ny, nz, pg, ug, vg, wg}; }
 
@@ -392,6 +416,148 @@ void bicgstab64_many(ns3d_ctx* c, int n, const int* sid, double* const* x, const
   }
 }
 
+// ---- multigrid inner solve of the pressure system (mixed form) -------------
+
+float env_float(const char* name, float dflt) {
+  const char* e = std::getenv(name);
+  return e ? (float)std::atof(e) : dflt;
+}
+
+// Builds the aggregation hierarchy of the pressure system (kernels_mg.hip):
+// level l+1 holds the 2x2x2 blocks of level l's grid with at least one row,
+// down to at most kMgCoarseRows rows.  Leaves c->mg.ok false (the inner
+// solves stay Jacobi BiCGSTAB) when the system is too small, is not stored
+// packed-symmetric, or a level's neighbour distances do not fit the packed
+// format.
+void mg_build(ns3d_ctx* c) {
+  MgHier& H = c->mg;
+  H.tried = true;
+  const SysMem& S = c->sys[3];
+  if (S.layout != ML_PACKED_SYM || !S.coef32 || S.n <= kMgCoarseRows) return;
+  hipStream_t s = c->stream;
+  MgLevel lv[kMgMaxLevels]{};
+  float* coef[kMgMaxLevels] = {};
+  float* invd[kMgMaxLevels] = {};
+  MgLevel& L0 = lv[0];
+  L0.n = S.n; L0.nx = c->nx; L0.ny = c->ny; L0.nz = c->nz;
+  L0.sE = 1 + DE; L0.sN = 1 + DN; L0.sT = 1 + DT;
+  L0.grid = c->pg; L0.cell = S.cell; L0.pk = S.pk; L0.coef = S.coef32; L0.invd = S.invdiag32;
+  L0.b = c->alloc<float>(S.n);
+  L0.x = c->alloc<float>(S.n);
+  int nlev = 1;
+  int* bad = c->alloc<int>(1);
+  HCHK(hipMemsetAsync(bad, 0, sizeof(int), s));
+  while (lv[nlev - 1].n > kMgCoarseRows) {
+    if (nlev == kMgMaxLevels) return;
+    MgLevel& F = lv[nlev - 1];
+    MgLevel C{};
+    C.nx = (F.nx + 1) / 2; C.ny = (F.ny + 1) / 2; C.nz = (F.nz + 1) / 2;
+    if (C.nx == F.nx && C.ny == F.ny && C.nz == F.nz) return;
+    const int64_t ncell = (int64_t)C.nx * C.ny * C.nz;
+    int32_t* grid = c->alloc<int32_t>(ncell);
+    HCHK(launch_mg_fill_i32(grid, ncell, 0, s));
+    HCHK(launch_mg_mark(F, C, grid, s));
+    const int nb = mg_scan_blocks(ncell);
+    int32_t* counts = c->alloc<int32_t>(nb + 1);
+    HCHK(launch_mg_scan(grid, ncell, counts, nullptr, s));
+    int32_t total = 0;
+    HCHK(hipMemcpyAsync(&total, counts + nb, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HCHK(ctx_sync(c));
+    C.n = total;
+    C.grid = grid;
+    int32_t* cell = c->alloc<int32_t>(C.n);
+    C.cell = cell;
+    HCHK(launch_mg_scan(grid, ncell, counts, cell, s));
+    int32_t* agg = c->alloc<int32_t>(F.n);
+    HCHK(launch_mg_agg(F, C, agg, s));
+    F.agg = agg;
+    uint64_t* pk = c->alloc<uint64_t>(C.n);
+    C.pk = pk;
+    HCHK(launch_mg_pk(C, pk, bad, s));
+    C.sE = 1; C.sN = 2; C.sT = 3;
+    coef[nlev] = c->alloc<float>((int64_t)4 * C.n);
+    invd[nlev] = c->alloc<float>(C.n);
+    C.coef = coef[nlev];
+    C.invd = invd[nlev];
+    C.b = c->alloc<float>(C.n);
+    C.x = c->alloc<float>(C.n);
+    lv[nlev++] = C;
+  }
+  int hbad = 0;
+  HCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  HCHK(ctx_sync(c));
+  if (hbad || nlev < 2) return;
+  MgFcg& G = H.G;
+  G.f = lv[0];
+  G.p[0] = c->alloc<float>(S.n);
+  G.p[1] = c->alloc<float>(S.n);
+  G.q = c->alloc<float>(S.n);
+  G.part = c->alloc<double>((int64_t)4 * kMaxRedBlocks);
+  G.st = c->alloc<MgState>(1);
+  G.nb = reduce_blocks(S.n);
+  G.omega = env_float("NS3D_MG_OMEGA", 2.0f / 3.0f);
+  G.scale = env_float("NS3D_MG_SCALE", 1.0f);
+  H.coarse_iters = (int)env_float("NS3D_MG_COARSE_ITERS", (float)H.coarse_iters);
+  HCHK(hipHostMalloc((void**)&H.hst, sizeof(MgState), hipHostMallocDefault));
+  for (int l = 0; l < nlev; ++l) {
+    H.lv[l] = lv[l];
+    H.coef[l] = coef[l];
+    H.invd[l] = invd[l];
+  }
+  H.nlev = nlev;
+  H.ok = true;
+  H.dirty = true;
+}
+
+// Whether system sid's FP32 sweeps are multigrid-preconditioned CG (the
+// pressure system of a mixed-form context, from mg_min_rows rows on).
+bool mg_on(ns3d_ctx* c, int sid) {
+  if (sid != 3 || c->precision != NS3D_PRECISION_MIXED || c->mg_min_rows <= 0) return false;
+  const SysMem& S = c->sys[3];
+  if (S.n < c->mg_min_rows || !S.coef32) return false;
+  if (!c->mg.tried) mg_build(c);
+  return c->mg.ok;
+}
+
+// One FP32 sweep A32 x = rhs (x from 0) to a relative tolerance tol, at
+// most maxit iterations, by flexible CG with one V-cycle per iteration;
+// the final state is returned in BiCGSTAB's terms (status DONE or
+// ZERO_RHS, it_c iterations, rr = |r|^2, rhs_sqnorm = |rhs|^2).
+void mg_solve(ns3d_ctx* c, const float* rhs, float* x, double tol, int64_t maxit, BState& out) {
+  MgHier& H = c->mg;
+  hipStream_t s = c->stream;
+  if (H.dirty) {
+    for (int l = 0; l + 1 < H.nlev; ++l) HCHK(launch_mg_galerkin(H.lv[l], H.lv[l + 1], H.coef[l + 1], H.invd[l + 1], s));
+    H.dirty = false;
+  }
+  H.G.x = x;
+  HCHK(launch_mg_init(H.G, rhs, tol, (long long)maxit, s));
+  HCHK(hipMemcpyAsync(H.hst, H.G.st, sizeof(MgState), hipMemcpyDeviceToHost, s));
+  HCHK(ctx_sync(c));
+  int64_t chunk = std::max<int64_t>(2, H.hint * 3 / 4);
+  int64_t launched = 0;
+  while (H.hst->status == BS_RUNNING) {
+    if (chunk > maxit + 1) chunk = maxit + 1;
+    for (int64_t q = 0; q < chunk; ++q) HCHK(launch_mg_iteration(H.G, H.lv, H.nlev, H.coarse_iters, s));
+    HCHK(hipMemcpyAsync(H.hst, H.G.st, sizeof(MgState), hipMemcpyDeviceToHost, s));
+    HCHK(ctx_sync(c));
+    launched += chunk;
+    if (launched > 2 * maxit + 64) throw Invalid{"multigrid CG did not stop within its iteration cap"};
+    chunk = 4;
+  }
+  const MgState& st = *H.hst;
+  if (std::getenv("NS3D_MG_TRACE"))
+    std::fprintf(stderr, "mg_solve: n %d levels %d tol %.3e status %d its %lld |r0| %.6e |r| %.6e rho %.6e %.6e alpha %.6e %.6e\n",
+                 H.lv[0].n, H.nlev, tol, st.status, st.it_c, std::sqrt(st.rr0), std::sqrt(st.rr), st.rho[0], st.rho[1],
+                 st.alpha[0], st.alpha[1]);
+  if (st.status != BS_ZERO_RHS) H.hint = std::max<int64_t>(2, st.it_c);
+  out = BState{};
+  out.status = st.status == BS_ZERO_RHS ? BS_ZERO_RHS : BS_DONE;
+  out.it_c = st.it_c;
+  out.rr = st.rr;
+  out.rhs_sqnorm = st.rr0;
+}
+
 // Mixed form: FP64 defect correction around FP32 inner BiCGSTAB solves.
 //   r = b - A x (FP64); while |r| > tol |b|: solve A32 d = r32 to a relative
 //   tolerance max(inner_tol, tol |b| / (2 |r|)); x += d; r = b - A x.
@@ -464,8 +630,8 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
   }
   for (;;) {
     SolveJob<float> jobs[3];
-    int idx[3];
-    int m = 0;
+    int idx[3], all[3];
+    int m = 0, na = 0;
     for (int i = 0; i < n; ++i) {
       Track& k = t[i];
       if (!use[i] || !k.live) continue;
@@ -478,16 +644,23 @@ void bicgstab_mixed_many(ns3d_ctx* c, int n, const int* sid, double* const* x, c
       SysMem& S = c->sys[sid[i]];
       const double rel = std::sqrt(k.rr / k.bb);
       const double tin = std::max(c->inner_tol, 0.5 * tol / rel);
+      all[na++] = i;
+      if (mg_on(c, sid[i])) {
+        // the pressure system's sweep: multigrid-preconditioned CG, on the
+        // context's stream, its final state in the lane's pinned copy
+        mg_solve(c, L[i]->r32, L[i]->d32, tin, budget32 - k.total, *L[i]->hstate);
+        continue;
+      }
       HCHK(launch_fill_f32(L[i]->d32, S.n, s));
       jobs[m] = SolveJob<float>{sid[i], S.mat32(), L[i]->d32, L[i]->r32, L[i]->bw32, tin, budget32 - k.total, L[i],
                                 &S.last_iters32};
       idx[m++] = i;
     }
-    if (m == 0) break;
+    if (na == 0) break;
     run_solves<float>(c, jobs, m);
     bool again[3] = {false, false, false};
-    for (int q = 0; q < m; ++q) {
-      const int i = idx[q];
+    for (int q = 0; q < na; ++q) {
+      const int i = all[q];
       Track& k = t[i];
       const BState& st = *L[i]->hstate;
       const int64_t its = (st.status == BS_ZERO_RHS) ? 0 : st.it_c;
@@ -634,6 +807,7 @@ void refresh_f32(ns3d_ctx* c, int sid) {
     S.invdiag32 = c->alloc<float>(S.n);
   }
   HCHK(launch_coef_to_f32(S.coef, S.invdiag, S.n, S.coef32, S.invdiag32, c->stream));
+  if (sid == 3) c->mg.dirty = true;
 }
 
 void alloc_system(ns3d_ctx* c, int sid, int n, int d0, int32_t* grid, bool momentum) {
@@ -820,6 +994,7 @@ void ns3d_destroy(ns3d_ctx* c) {
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->ev_sync) (void)hipEventDestroy(c->ev_sync);
+  if (c->mg.hst) (void)hipHostFree(c->mg.hst);
   for (auto& e : c->tev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1292,6 +1467,20 @@ ns3d_status ns3d_set_precision(ns3d_ctx* c, int mode, double inner_tol) {
   });
 }
 
+ns3d_status ns3d_set_multigrid(ns3d_ctx* c, int64_t rows) {
+  if (!c || rows < 0) return fail_invalid("ns3d_set_multigrid: bad arguments");
+  c->mg_min_rows = rows;
+  return NS3D_OK;
+}
+
+ns3d_status ns3d_multigrid_levels(ns3d_ctx* c, int* levels, int64_t* rows) {
+  if (!c || !levels) return fail_invalid("ns3d_multigrid_levels: bad arguments");
+  *levels = c->mg.ok ? c->mg.nlev : 0;
+  if (rows)
+    for (int l = 0; l < *levels; ++l) rows[l] = c->mg.lv[l].n;
+  return NS3D_OK;
+}
+
 ns3d_status ns3d_set_mixed_min_rows(ns3d_ctx* c, int64_t rows) {
   if (!c || rows < 0) return fail_invalid("ns3d_set_mixed_min_rows: bad arguments");
   c->mixed_min_rows = rows;
@@ -1394,6 +1583,11 @@ ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, in
   const double lane = (2 + 7) * 8 * nmax + 8.0 * kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks) + 1024;
   b += lanes * lane + 4096;
   if (mixed) b += rows_all * (28 + 4) + lanes * nmax * (7 * 4 + 2 * 4 + 8);  // FP32 matrices; inner vectors, residual
+  // multigrid of the pressure system: level-0 vectors and aggregate map
+  // (r, z, p[2], q, agg); coarser levels hold at most N/7 cells and rows
+  // (grid 4 B per cell; cell, pk, coef, D^-1, agg, b, x 44 B per row);
+  // partials; allocation rounding of up to 10 allocations per level
+  if (mixed) b += 24 * nf + 48 * N / 7 + 32.0 * kMaxRedBlocks + 64.0 * 1024 * (8 + 10 * kMgMaxLevels);
   b += 64.0 * 1024 * (48 + 26 * (lanes - 1));          // allocation rounding (64 KiB per allocation)
   *bytes = b;
   return NS3D_OK;

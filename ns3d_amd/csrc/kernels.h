@@ -247,6 +247,74 @@ hipError_t launch_residual64(SMat<double> A, const double* x, const double* b, d
 hipError_t launch_add_f32(double* x, const float* d, int64_t n, hipStream_t s, double a = 1.0);
 hipError_t launch_fill_f32(float* x, int64_t n, hipStream_t s);
 
+// ---- aggregation multigrid for pressure systems (kernels_mg.hip) ------------
+// Rows the coarsest level may hold (its solve runs in one 1024-thread
+// work-group with the vectors in LDS), and the most levels a hierarchy has.
+constexpr int kMgCoarseRows = 2048;
+constexpr int kMgMaxLevels = 12;
+// One level: rows numbered in C order of the level's grid (nx, ny, nz);
+// level 0 is the pressure system itself (its index grid, row cells, packed
+// distances and FP32 coefficients, slots 0 / 1+DE / 1+DN / 1+DT of [7][n]),
+// coarser levels store [4][n] (diagonal, E, N, T: slots 0..3).  agg maps a
+// row to its row on the next level (null on the coarsest); b and x are the
+// level's right-hand side and result.
+struct MgLevel {
+  int n, nx, ny, nz;
+  int sE, sN, sT;
+  const int32_t* grid;
+  const int32_t* cell;
+  const uint64_t* pk;
+  const float* coef;
+  const float* invd;
+  const int32_t* agg;
+  float* b;
+  float* x;
+};
+// State of one flexible-CG solve in device memory: scalars of iteration i
+// in slot i & 1; it_c (iterations completed) is written by the update
+// kernel and read by the first restriction, which admits the iteration and
+// writes it_a, read by every later kernel of the iteration.
+struct MgState {
+  double rho[2], alpha[2];
+  double rr, rr0, tol2;
+  long long it_a, it_c, maxit;
+  int status;  // BStatus
+  int pad;
+};
+// The flexible-CG operands: level 0 (its b is the residual r, its x the
+// preconditioned residual z), the solution x, the directions p (two
+// buffers), q = A p, per-block partials ([4][kMaxRedBlocks]), the state,
+// the row kernels' block count, the smoothing weight and the coarse
+// correction's scale.
+struct MgFcg {
+  MgLevel f;
+  float* x;
+  float* p[2];
+  float* q;
+  double* part;
+  MgState* st;
+  int nb;
+  float omega, scale;
+};
+hipError_t launch_mg_fill_i32(int32_t* x, int64_t n, int32_t v, hipStream_t s);
+// flags[the coarse cell of each row of F] = 1 (flags over C's grid, zeroed).
+hipError_t launch_mg_mark(const MgLevel& F, const MgLevel& C, int32_t* flags, hipStream_t s);
+// Blocks of the flag scan over ncell cells (counts needs that many + 1).
+int mg_scan_blocks(int64_t ncell);
+// cell_or_null == null: counts = exclusive block offsets, counts[nb] = total;
+// else: grid (the flags) becomes the row of each flagged cell (-1 elsewhere)
+// and cell[row] its cell.
+hipError_t launch_mg_scan(int32_t* grid, int64_t ncell, int32_t* counts, int32_t* cell_or_null, hipStream_t s);
+hipError_t launch_mg_agg(const MgLevel& F, const MgLevel& C, int32_t* agg, hipStream_t s);
+// Packed distances of C's rows; *bad = 1 if one does not fit.
+hipError_t launch_mg_pk(const MgLevel& C, uint64_t* pk, int* bad, hipStream_t s);
+// C's coefficients ([4][C.n]) and D^-1 as the Galerkin product of F.
+hipError_t launch_mg_galerkin(const MgLevel& F, const MgLevel& C, float* coef, float* invd, hipStream_t s);
+// r = rhs, x = 0, state for tolerance tol (relative to |rhs|) and the cap.
+hipError_t launch_mg_init(const MgFcg& G, const float* rhs, double tol, long long maxit, hipStream_t s);
+// One flexible-CG iteration over levels lv[0..nlev) (nlev >= 2).
+hipError_t launch_mg_iteration(const MgFcg& G, const MgLevel* lv, int nlev, int coarse_iters, hipStream_t s);
+
 // ---- fields, corrections, diagnostics (kernels_field.hip) -------------------
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);
 // *bad = 1.0 if an entry of idx[count] lies outside [0, n) (bad untouched

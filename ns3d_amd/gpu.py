@@ -33,7 +33,7 @@ EXPORTS = [
     "ns3d_compute_pressure_correction", "ns3d_compute_pressure_field", "ns3d_correct_velocity",
     "ns3d_correct_pressure", "ns3d_compute_div", "ns3d_compute_permeability", "ns3d_get_matrix",
     "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_set_precision",
-    "ns3d_set_mixed_min_rows", "ns3d_sync", "ns3d_bicgstab_stencil_noalloc", "ns3d_noalloc_max_rows",
+    "ns3d_set_mixed_min_rows", "ns3d_set_multigrid", "ns3d_multigrid_levels", "ns3d_sync", "ns3d_bicgstab_stencil_noalloc", "ns3d_noalloc_max_rows",
     "ns3d_engine_create", "ns3d_engine_destroy", "ns3d_attach_engine", "ns3d_engine_stats",
     "ns3d_engine_set_graphs", "ns3d_engine_set_schedule", "ns3d_engine_get_schedule",
     "ns3d_footprint", "ns3d_ctx_bytes",
@@ -118,6 +118,8 @@ def lib():
         L.ns3d_matrix_nnz.argtypes = [vp, P(i64)]
         L.ns3d_set_precision.argtypes = [vp, i32, dbl]
         L.ns3d_set_mixed_min_rows.argtypes = [vp, i64]
+        L.ns3d_set_multigrid.argtypes = [vp, i64]
+        L.ns3d_multigrid_levels.argtypes = [vp, P(i32), P(i64)]
         L.ns3d_sync.argtypes = [vp]
         _lib = L
     return _lib
@@ -305,6 +307,18 @@ class NavierStokesGPU:
         """min_rows: in the mixed form, systems with fewer rows are solved in FP64."""
         _check(lib().ns3d_set_precision(self.h, int(bool(mixed)), inner_tol))
         _check(lib().ns3d_set_mixed_min_rows(self.h, int(min_rows)))
+
+    def set_multigrid(self, min_rows: int):
+        """Mixed form: pressure systems from min_rows rows on are swept by
+        multigrid-preconditioned CG (0: never)."""
+        _check(lib().ns3d_set_multigrid(self.h, int(min_rows)))
+
+    def multigrid_levels(self):
+        """Row counts of the pressure system's multigrid levels ([] until built)."""
+        lev = C.c_int32(0)
+        rows = (C.c_int64 * 12)()
+        _check(lib().ns3d_multigrid_levels(self.h, C.byref(lev), rows))
+        return [int(rows[i]) for i in range(lev.value)]
 
     def nnz(self):
         out = np.zeros(4, np.int64)

@@ -87,6 +87,7 @@ int ns3d_solver_create(const ns3d_run_params* p, const uint8_t* img, int nx, int
     if (p->precision != 0) {
       check(ns3d_set_precision(s->base->ctx, p->precision, p->inner_tol), "ns3d_set_precision");
       check(ns3d_set_mixed_min_rows(s->base->ctx, p->mixed_min_rows), "ns3d_set_mixed_min_rows");
+      check(ns3d_set_multigrid(s->base->ctx, p->mg_min_rows), "ns3d_set_multigrid");
     }
     *out = s.release();
     return 0;

@@ -34,7 +34,7 @@ class RunParams(C.Structure):
                 ("border", C.c_int32), ("method", C.c_int32), ("verbose", C.c_int32),
                 ("write_files", C.c_int32), ("device", C.c_int32), ("outdir", C.c_char_p),
                 ("name", C.c_char_p), ("precision", C.c_int32), ("mixed_min_rows", C.c_int32),
-                ("inner_tol", C.c_double)]
+                ("inner_tol", C.c_double), ("mg_min_rows", C.c_int32), ("pad_params", C.c_int32)]
 
 
 class RunResult(C.Structure):
