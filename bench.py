@@ -69,7 +69,8 @@ def mix_params(img, args, precision):
     nx = img.shape[0]
     return dict(dx=1.0, mu=1.0, rho=args.rho, p0=1.0, gradp=1.0 / nx, alpha=0.8, beta=1.0, direction=1, border=0,
                 method="SIMPLEC", simpletol=args.simpletol, maxitSIMPLE=args.mix_maxit, N1=4, N2=4, device=0,
-                precision=precision, inner_tol=args.inner_tol, mixed_min_rows=args.mixed_min_rows, **TOLS)
+                precision=precision, inner_tol=args.inner_tol, mixed_min_rows=args.mixed_min_rows,
+                mg_min_rows=args.mg_min_rows, **TOLS)
 
 
 def make_server(args):
@@ -330,7 +331,8 @@ def kernel_table(solvers):
 def single_form(img, params, args, precision, label, steps, warmup):
     from ns3d_amd import gpu, solver
     kw = dict(params, method="SIMPLEC", simpletol=args.simpletol, maxitSIMPLE=args.maxit, N1=4, N2=4, device=0,
-              precision=precision, inner_tol=args.inner_tol, verbose=2 if args.progress else 0)
+              precision=precision, inner_tol=args.inner_tol, mg_min_rows=args.mg_min_rows,
+              verbose=2 if args.progress else 0)
     for w in range(warmup):
         s = solver.Solver(img, **kw)
         summ = s.run()[1]
@@ -491,6 +493,8 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=1e-4, help="mixed form: FP32 sweep tolerance")
     ap.add_argument("--mixed-min-rows", type=int, default=0,
                     help="mixed form: systems with fewer rows are solved in FP64")
+    ap.add_argument("--mg-min-rows", type=int, default=200000,
+                    help="mixed form: pressure systems from this many rows on are swept by multigrid CG (0: never)")
     ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events (profiler runs)")

@@ -244,13 +244,14 @@ ns3d_status ns3d_set_mixed_min_rows(ns3d_ctx* ctx, int64_t rows);
  * operators rebuilt after every pressure assembly) instead of Jacobi
  * BiCGSTAB; the FP64 defect correction around them, its tolerance and its
  * true-residual contract are unchanged.  The pressure matrices are
- * symmetric positive (semi-)definite M-matrices, so CG applies.  0: never;
- * default 32768.  Not part of the reference (its solver is Eigen's Jacobi
- * BiCGSTAB, which the plain FP64 form keeps). */
+ * symmetric and definite (negated M-matrices), so CG applies.  0: never;
+ * default 200000 (smaller systems are cheaper batched in the engine).  Not
+ * part of the reference (its solver is Eigen's Jacobi BiCGSTAB, which the
+ * plain FP64 form keeps). */
 ns3d_status ns3d_set_multigrid(ns3d_ctx* ctx, int64_t rows);
 /* Levels of the pressure system's multigrid hierarchy (0 until the first
- * multigrid sweep has built it) and their row counts (rows[kMax], up to 12
- * entries written; rows may be NULL). */
+ * multigrid sweep has built it) and their row counts (up to 12 entries
+ * written to rows, which may be NULL). */
 ns3d_status ns3d_multigrid_levels(ns3d_ctx* ctx, int* levels, int64_t* rows);
 
 /* Device time of the fused BiCGSTAB kernels (enable with ns3d_timing(ctx,

@@ -41,7 +41,7 @@ typedef struct {
   double inner_tol;             /* mixed form: relative tolerance of the FP32 sweeps */
   int32_t mg_min_rows;          /* mixed form: pressure systems from this many rows
                                    on are swept by multigrid CG (ns3d_set_multigrid;
-                                   0: never; default 32768) */
+                                   0: never; default 200000) */
   int32_t pad_params;
 } ns3d_run_params;
 

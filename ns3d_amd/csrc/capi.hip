@@ -97,7 +97,7 @@ struct Lane {
 // first inner solve that uses it.  Levels hold device pointers into the
 // context's allocations; coef / invd are the writable views of levels >= 1.
 // Default of ns3d_set_multigrid: pressure systems from this many rows on.
-constexpr int64_t kMgDefaultMinRows = 32768;
+constexpr int64_t kMgDefaultMinRows = 200000;
 
 struct MgHier {
   bool tried = false;  // build attempted (ok: it succeeded)
@@ -110,7 +110,13 @@ struct MgHier {
   MgFcg G{};
   MgState* hst = nullptr;  // pinned copy of the state
   int64_t hint = 8;        // iterations of the last solve (sizes the first chunk)
-  int coarse_iters = 24;   // CG steps of the coarsest solve
+  int coarse_iters = 16;   // CG steps of the coarsest solve
+  // chunks of iterations are replayed as graphs (one per chunk length and
+  // solution vector), captured on a stream of their own; NS3D_MG_GRAPHS=0
+  // launches them kernel by kernel
+  bool graphs = true;
+  hipStream_t cap = nullptr;
+  std::vector<std::pair<std::pair<const void*, int>, hipGraphExec_t>> execs;
 };
 
 }  // namespace
@@ -498,6 +504,7 @@ void mg_build(ns3d_ctx* c) {
   G.omega = env_float("NS3D_MG_OMEGA", 2.0f / 3.0f);
   G.scale = env_float("NS3D_MG_SCALE", 1.0f);
   H.coarse_iters = (int)env_float("NS3D_MG_COARSE_ITERS", (float)H.coarse_iters);
+  H.graphs = env_float("NS3D_MG_GRAPHS", 1.0f) != 0.0f;
   HCHK(hipHostMalloc((void**)&H.hst, sizeof(MgState), hipHostMallocDefault));
   for (int l = 0; l < nlev; ++l) {
     H.lv[l] = lv[l];
@@ -519,13 +526,41 @@ bool mg_on(ns3d_ctx* c, int sid) {
   return c->mg.ok;
 }
 
+// The graph of `chunk` iterations of the current operands (built on first
+// use).  Its kernels take everything from device memory, so one graph
+// serves every solve with the same solution vector.
+hipGraphExec_t mg_graph(ns3d_ctx* c, int chunk) {
+  MgHier& H = c->mg;
+  const std::pair<const void*, int> key{H.G.x, chunk};
+  for (auto& e : H.execs)
+    if (e.first == key) return e.second;
+  if (!H.cap) HCHK(hipStreamCreateWithFlags(&H.cap, hipStreamNonBlocking));
+  HCHK(hipStreamBeginCapture(H.cap, hipStreamCaptureModeThreadLocal));
+  hipError_t err = hipSuccess;
+  for (int q = 0; q < chunk && err == hipSuccess; ++q) err = launch_mg_iteration(H.G, H.lv, H.nlev, H.coarse_iters, H.cap);
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(H.cap, &g);
+  HCHK(err);
+  HCHK(e2);
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HCHK(e3);
+  H.execs.push_back({key, ex});
+  return ex;
+}
+
 // One FP32 sweep A32 x = rhs (x from 0) to a relative tolerance tol, at
 // most maxit iterations, by flexible CG with one V-cycle per iteration;
 // the final state is returned in BiCGSTAB's terms (status DONE or
 // ZERO_RHS, it_c iterations, rr = |r|^2, rhs_sqnorm = |rhs|^2).
 void mg_solve(ns3d_ctx* c, const float* rhs, float* x, double tol, int64_t maxit, BState& out) {
   MgHier& H = c->mg;
+  // on the context's stream (an attached context's is the engine's context
+  // stream, which has a hardware queue of its own: a stream sharing a
+  // lane's queue would wait behind that lane's chunks)
   hipStream_t s = c->stream;
+  auto sync = [&] { HCHK(ctx_sync(c)); };
   if (H.dirty) {
     for (int l = 0; l + 1 < H.nlev; ++l) HCHK(launch_mg_galerkin(H.lv[l], H.lv[l + 1], H.coef[l + 1], H.invd[l + 1], s));
     H.dirty = false;
@@ -533,14 +568,20 @@ void mg_solve(ns3d_ctx* c, const float* rhs, float* x, double tol, int64_t maxit
   H.G.x = x;
   HCHK(launch_mg_init(H.G, rhs, tol, (long long)maxit, s));
   HCHK(hipMemcpyAsync(H.hst, H.G.st, sizeof(MgState), hipMemcpyDeviceToHost, s));
-  HCHK(ctx_sync(c));
-  int64_t chunk = std::max<int64_t>(2, H.hint * 3 / 4);
+  sync();
+  // chunks of 2, 4, 8 or 16 iterations: the first sized by the last solve,
+  // then 4 between polls
+  int chunk = 2;
+  while (chunk < 16 && 2 * chunk <= H.hint * 3 / 4) chunk *= 2;
   int64_t launched = 0;
   while (H.hst->status == BS_RUNNING) {
-    if (chunk > maxit + 1) chunk = maxit + 1;
-    for (int64_t q = 0; q < chunk; ++q) HCHK(launch_mg_iteration(H.G, H.lv, H.nlev, H.coarse_iters, s));
+    if (H.graphs) {
+      HCHK(hipGraphLaunch(mg_graph(c, chunk), s));
+    } else {
+      for (int q = 0; q < chunk; ++q) HCHK(launch_mg_iteration(H.G, H.lv, H.nlev, H.coarse_iters, s));
+    }
     HCHK(hipMemcpyAsync(H.hst, H.G.st, sizeof(MgState), hipMemcpyDeviceToHost, s));
-    HCHK(ctx_sync(c));
+    sync();
     launched += chunk;
     if (launched > 2 * maxit + 64) throw Invalid{"multigrid CG did not stop within its iteration cap"};
     chunk = 4;
@@ -995,6 +1036,8 @@ void ns3d_destroy(ns3d_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   if (c->ev_sync) (void)hipEventDestroy(c->ev_sync);
   if (c->mg.hst) (void)hipHostFree(c->mg.hst);
+  for (auto& e : c->mg.execs) (void)hipGraphExecDestroy(e.second);
+  if (c->mg.cap) (void)hipStreamDestroy(c->mg.cap);
   for (auto& e : c->tev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -248,9 +248,11 @@ hipError_t launch_add_f32(double* x, const float* d, int64_t n, hipStream_t s, d
 hipError_t launch_fill_f32(float* x, int64_t n, hipStream_t s);
 
 // ---- aggregation multigrid for pressure systems (kernels_mg.hip) ------------
-// Rows the coarsest level may hold (its solve runs in one 1024-thread
-// work-group with the vectors in LDS), and the most levels a hierarchy has.
-constexpr int kMgCoarseRows = 2048;
+// Rows the coarsest level may hold (its solve runs in one work-group of
+// kMgCoarseThreads threads, two rows each), and the most levels a hierarchy
+// has.
+constexpr int kMgCoarseRows = 512;
+constexpr int kMgCoarseThreads = 256;
 constexpr int kMgMaxLevels = 12;
 // One level: rows numbered in C order of the level's grid (nx, ny, nz);
 // level 0 is the pressure system itself (its index grid, row cells, packed

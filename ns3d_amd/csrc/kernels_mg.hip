@@ -20,7 +20,7 @@
 //   pre-smoothing   x1 = w D^-1 b                        (one Jacobi step)
 //   restriction     b' = P^T (b - A x1)                  (mg_restrict)
 //   coarse solve    e = V-cycle(l+1, b'), or on the coarsest level a few
-//                   Jacobi-preconditioned CG steps in one work-group
+//                   Jacobi-preconditioned CG steps in one work-group (FP32)
 //   correction and  x2 = x1 + s P e;  x3 = x2 + w D^-1 (b - A x2)
 //   post-smoothing                                      (mg_prolong)
 // x1 and x2 are never stored: each kernel recomputes them at the seven
@@ -257,10 +257,13 @@ __global__ void __launch_bounds__(kBlock) mg_galerkin_kernel(MgLevel F, MgLevel 
 
 // ---- V-cycle ------------------------------------------------------------------
 
-// b' = P^T (b - A w D^-1 b) on the coarse rows of C.  The first restriction
-// of an iteration (TOP) also admits it: it sums the r.r partials of the
-// previous update (or of init) and stops the solve when converged or at the
-// cap; later launches of a stopped solve exit at once.
+// b' = P^T (b - A w D^-1 b) on the coarse rows of C: eight consecutive
+// threads per coarse row, one per child block position (a, b, c), each the
+// residual of its fine row (0 where there is none), summed over the eight
+// lanes by a fixed butterfly.  The first restriction of an iteration (TOP)
+// also admits it: it sums the r.r partials of the previous update (or of
+// init) and stops the solve when converged or at the cap; later launches of
+// a stopped solve exit at once.
 template <bool TOP>
 __global__ void __launch_bounds__(kBlock) mg_restrict_kernel(MgLevel F, MgLevel C, MgFcg G) {
   MgState* st = G.st;
@@ -293,21 +296,22 @@ __global__ void __launch_bounds__(kBlock) mg_restrict_kernel(MgLevel F, MgLevel 
   } else {
     if (!mg_admit(st, it)) return;
   }
-  const int64_t R = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (R >= C.n) return;
-  const int64_t cc = C.cell[R];
-  const int K = (int)(cc % C.nz);
-  const int64_t t = cc / C.nz;
-  const int J = (int)(t % C.ny), I = (int)(t / C.ny);
-  const float w = G.omega;
-  float acc = 0.f;
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int c = 0; c < 2; ++c) {
-        const int i = 2 * I + a, j = 2 * J + b, k = 2 * K + c;
-        if (i >= F.nx || j >= F.ny || k >= F.nz) continue;
-        const int r = F.grid[((int64_t)i * F.ny + j) * F.nz + k];
-        if (r < 0) continue;
+  const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t R = g >> 3;
+  const int ch = (int)(g & 7);
+  // rows past the end still take part in the butterfly (whole groups of
+  // eight lie on one side of C.n, so they only meet each other)
+  float res = 0.f;
+  if (R < C.n) {
+    const int64_t cc = C.cell[R];
+    const int K = (int)(cc % C.nz);
+    const int64_t t = cc / C.nz;
+    const int J = (int)(t % C.ny), I = (int)(t / C.ny);
+    const int i = 2 * I + (ch >> 2), j = 2 * J + ((ch >> 1) & 1), k = 2 * K + (ch & 1);
+    if (i < F.nx && j < F.ny && k < F.nz) {
+      const int r = F.grid[((int64_t)i * F.ny + j) * F.nz + k];
+      if (r >= 0) {
+        const float w = G.omega;
         MgPts P;
         mg_row(F, r, P);
         float ax = 0.f;
@@ -316,38 +320,46 @@ __global__ void __launch_bounds__(kBlock) mg_restrict_kernel(MgLevel F, MgLevel 
           const int q = P.q[m];
           ax += P.c[m] * (w * F.invd[q] * F.b[q]);
         }
-        acc += F.b[r] - ax;
+        res = F.b[r] - ax;
       }
-  C.b[R] = acc;
+    }
+  }
+  res += __shfl_xor(res, 1, 64);
+  res += __shfl_xor(res, 2, 64);
+  res += __shfl_xor(res, 4, 64);
+  if (ch == 0 && R < C.n) C.b[R] = res;
 }
 
-// Block sum of one double over a 1024-thread block, valid in every thread.
-__device__ __forceinline__ double mg_bsum(double v) {
-  __shared__ double sh[16];
-  v = wave_reduce<RSUM>(v);
+// Block sum of one float over a kMgCoarseThreads-thread block, valid in
+// every thread (fixed order).
+__device__ __forceinline__ float mg_bsum(float v) {
+  constexpr int kW = kMgCoarseThreads / 64;
+  __shared__ float sh[kW];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  double s = sh[0];
+  float s = sh[0];
 #pragma unroll
-  for (int q = 1; q < 16; ++q) s += sh[q];
+  for (int q = 1; q < kW; ++q) s += sh[q];
   return s;
 }
 
 // Coarsest level: `iters` Jacobi-preconditioned CG steps from x = 0 in one
-// 1024-thread work-group, the vectors in LDS (at most kMgCoarseRows rows).
-__global__ void __launch_bounds__(1024) mg_coarse_kernel(MgLevel L, MgFcg G, int iters) {
+// work-group, the direction vector in LDS (at most kMgCoarseRows rows).
+__global__ void __launch_bounds__(kMgCoarseThreads) mg_coarse_kernel(MgLevel L, MgFcg G, int iters) {
   long long it;
   if (!mg_admit(G.st, it)) return;
-  constexpr int kPer = kMgCoarseRows / 1024;
+  constexpr int kPer = kMgCoarseRows / kMgCoarseThreads;
   __shared__ float ps[kMgCoarseRows];
   const int n = L.n;
   MgPts P[kPer];
   float x[kPer], r[kPer], z[kPer], p[kPer], dg[kPer];
-  double rz = 0.0;
+  float rz = 0.f;
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int row = threadIdx.x + 1024 * u;
+    const int row = threadIdx.x + kMgCoarseThreads * u;
     x[u] = 0.f;
     if (row < n) {
       mg_row(L, row, P[u]);
@@ -362,45 +374,45 @@ __global__ void __launch_bounds__(1024) mg_coarse_kernel(MgLevel L, MgFcg G, int
     z[u] = dg[u] * r[u];
     p[u] = z[u];
     if (row < n) ps[row] = p[u];
-    rz += (double)r[u] * z[u];
+    rz += r[u] * z[u];
   }
   rz = mg_bsum(rz);  // (its barriers also publish ps)
-  for (int k = 0; k < iters && rz != 0.0; ++k) {
+  for (int k = 0; k < iters && rz != 0.f; ++k) {
     float q[kPer];
-    double pq = 0.0;
+    float pq = 0.f;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-      float s = 0.f;
+      float sv = 0.f;
 #pragma unroll
-      for (int m = 0; m < 7; ++m) s += P[u].c[m] * ps[P[u].q[m]];
-      q[u] = s;
-      pq += (double)p[u] * s;
+      for (int m = 0; m < 7; ++m) sv += P[u].c[m] * ps[P[u].q[m]];
+      q[u] = sv;
+      pq += p[u] * sv;
     }
     pq = mg_bsum(pq);
-    if (!(rz / pq > 0.0)) break;  // (also a zero or non-finite p.q)
-    const float al = (float)(rz / pq);
-    double rzn = 0.0;
+    const float al = rz / pq;
+    if (!(al > 0.f && al < __builtin_huge_valf())) break;  // breakdown (or p.q = 0)
+    float rzn = 0.f;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       x[u] += al * p[u];
       r[u] -= al * q[u];
       z[u] = dg[u] * r[u];
-      rzn += (double)r[u] * z[u];
+      rzn += r[u] * z[u];
     }
     rzn = mg_bsum(rzn);
-    const float be = (float)(rzn / rz);
+    const float be = rzn / rz;
     rz = rzn;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       p[u] = z[u] + be * p[u];
-      const int row = threadIdx.x + 1024 * u;
+      const int row = threadIdx.x + kMgCoarseThreads * u;
       if (row < n) ps[row] = p[u];
     }
     __syncthreads();
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int row = threadIdx.x + 1024 * u;
+    const int row = threadIdx.x + kMgCoarseThreads * u;
     if (row < n) L.x[row] = x[u];
   }
 }
@@ -606,11 +618,11 @@ hipError_t launch_mg_init(const MgFcg& G, const float* rhs, double tol, long lon
 hipError_t launch_mg_iteration(const MgFcg& G, const MgLevel* lv, int nlev, int coarse_iters, hipStream_t s) {
   // down: restrict 0 -> 1 -> ... -> nlev-1
   for (int l = 0; l + 1 < nlev; ++l) {
-    const dim3 grid(blocks_for(lv[l + 1].n));
+    const dim3 grid(blocks_for(8 * (int64_t)lv[l + 1].n));
     if (l == 0) hipLaunchKernelGGL(mg_restrict_kernel<true>, grid, dim3(kBlock), 0, s, lv[0], lv[1], G);
     else hipLaunchKernelGGL(mg_restrict_kernel<false>, grid, dim3(kBlock), 0, s, lv[l], lv[l + 1], G);
   }
-  hipLaunchKernelGGL(mg_coarse_kernel, dim3(1), dim3(1024), 0, s, lv[nlev - 1], G, coarse_iters);
+  hipLaunchKernelGGL(mg_coarse_kernel, dim3(1), dim3(kMgCoarseThreads), 0, s, lv[nlev - 1], G, coarse_iters);
   for (int l = nlev - 2; l >= 0; --l) {
     if (l == 0)
       hipLaunchKernelGGL(mg_prolong_kernel<true>, dim3(G.nb), dim3(kBlock), 0, s, lv[0], lv[1], G);

@@ -186,7 +186,7 @@ extern "C" void ns3d_default_run_params(ns3d_run_params* p) {
   p->write_files = 1;
   p->device = -1;
   p->inner_tol = 1e-4;
-  p->mg_min_rows = 32768;
+  p->mg_min_rows = 200000;
 }
 
 extern "C" int ns3d_read_ini(const char* path, ns3d_run_params* p, char* image, int cap) {
