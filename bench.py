@@ -495,7 +495,7 @@ def main():
                     help="mixed form: systems with fewer rows are solved in FP64")
     ap.add_argument("--mg-min-rows", type=int, default=200000,
                     help="mixed form: pressure systems from this many rows on are swept by multigrid CG (0: never)")
-    ap.add_argument("--cpu-iters", type=int, default=8)
+    ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events (profiler runs)")
     ap.add_argument("--no-graphs", action="store_true",
@@ -507,6 +507,8 @@ def main():
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="pick chunk / lanes / graphs with ns3d_amd.autotune before the mix (full runs)")
     ap.add_argument("--progress", action="store_true", help="one stderr line per outer iteration (single)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the library's plain device copy of MIB MiB (counter calibration) and print its GB/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); it has no multi-GPU mode: requests "
@@ -515,6 +517,13 @@ def main():
     from ns3d_amd import gpu
     if gpu.device_count() < 1:
         raise SystemExit("no GPU visible")
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        print(json.dumps({"copy_bytes": nbytes, "copy_GBs": round(gpu.copy_bandwidth(nbytes, 10), 1)}), flush=True)
+        return
+    # the achievable streaming rate the roofline fractions are read against
+    copy_gbs = gpu.copy_bandwidth(1 << 30, 20)
+    log(f"device copy (1 GiB read + written): {copy_gbs:.0f} GB/s")
 
     mixres = mix_section(args) if args.section in ("all", "mix") else None
     if args.section == "mix" and mixres is None:
@@ -571,6 +580,7 @@ def main():
         line["single_image"] = single["out"]
         line["kernels"] = single["kernels"]
     line["cpu_baseline"] = cpu
+    line["device_copy_GBs"] = round(copy_gbs, 1)
     print(json.dumps(line), flush=True)
 
 

@@ -253,6 +253,21 @@ ns3d_status ns3d_set_multigrid(ns3d_ctx* ctx, int64_t rows);
  * multigrid sweep has built it) and their row counts (up to 12 entries
  * written to rows, which may be NULL). */
 ns3d_status ns3d_multigrid_levels(ns3d_ctx* ctx, int* levels, int64_t* rows);
+/* Level `level` (1 .. levels-1) of that hierarchy as it is on the device
+ * (for checks of the aggregation and the Galerkin operators): the linear
+ * cell of each row on the level's grid (each dimension of level l+1 is
+ * (d+1)/2 of level l's) into cell[n], and the FP32 diagonal, E, N and T
+ * couplings into coef[4][n] (either may be NULL).  Not part of the
+ * reference. */
+ns3d_status ns3d_multigrid_level(ns3d_ctx* ctx, int level, int32_t* cell, float* coef);
+
+/* Measurement: a plain device copy of `bytes` (read once, written once) run
+ * `reps` times on `device` (-1: current), timed with events; *gbs = bytes
+ * read + written per second / 1e9.  The achievable streaming bandwidth that
+ * the kernels' HBM fractions are read against, and (under rocprofv3 --pmc)
+ * the known-size transfer that calibrates FETCH_SIZE / WRITE_SIZE.
+ * Allocates and frees its own buffers.  Not part of the reference. */
+ns3d_status ns3d_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs);
 
 /* Device time of the fused BiCGSTAB kernels (enable with ns3d_timing(ctx,
  * 1)): one iteration per queued chunk of iterations is sampled, its three

@@ -1524,6 +1524,47 @@ ns3d_status ns3d_multigrid_levels(ns3d_ctx* c, int* levels, int64_t* rows) {
   return NS3D_OK;
 }
 
+ns3d_status ns3d_multigrid_level(ns3d_ctx* c, int level, int32_t* cell, float* coef) {
+  if (!c || !c->mg.ok || level < 1 || level >= c->mg.nlev) return fail_invalid("ns3d_multigrid_level: no such level");
+  return guarded([&] {
+    const MgLevel& L = c->mg.lv[level];
+    if (cell) HCHK(hipMemcpyAsync(cell, L.cell, (size_t)L.n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (coef) HCHK(hipMemcpyAsync(coef, L.coef, (size_t)L.n * 16, hipMemcpyDeviceToHost, c->stream));
+    HCHK(ctx_sync(c));
+  });
+}
+
+ns3d_status ns3d_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs) {
+  if (bytes < 16 || reps < 1 || !gbs) return fail_invalid("ns3d_copy_bandwidth: bad arguments");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const ns3d_status st = guarded([&] {
+    if (device >= 0) HCHK(hipSetDevice(device));
+    const int64_t words = bytes / 16;
+    HCHK(hipMalloc(&a, (size_t)words * 16));
+    HCHK(hipMalloc(&b, (size_t)words * 16));
+    HCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HCHK(hipEventCreate(&e0));
+    HCHK(hipEventCreate(&e1));
+    HCHK(hipMemsetAsync(a, 1, (size_t)words * 16, s));
+    HCHK(launch_copy16(a, b, words, s));  // warm-up
+    HCHK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) HCHK(launch_copy16(a, b, words, s));
+    HCHK(hipEventRecord(e1, s));
+    HCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HCHK(hipEventElapsedTime(&ms, e0, e1));
+    *gbs = 2.0 * 16.0 * (double)words * reps / (ms * 1e-3) / 1e9;
+  });
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return st;
+}
+
 ns3d_status ns3d_set_mixed_min_rows(ns3d_ctx* c, int64_t rows) {
   if (!c || rows < 0) return fail_invalid("ns3d_set_mixed_min_rows: bad arguments");
   c->mixed_min_rows = rows;

@@ -319,6 +319,8 @@ hipError_t launch_mg_iteration(const MgFcg& G, const MgLevel* lv, int nlev, int 
 
 // ---- fields, corrections, diagnostics (kernels_field.hip) -------------------
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);
+// dst = src, `words` 16-byte words.
+hipError_t launch_copy16(const void* src, void* dst, int64_t words, hipStream_t s);
 // *bad = 1.0 if an entry of idx[count] lies outside [0, n) (bad untouched
 // otherwise: the caller clears it).
 hipError_t launch_check_index(const int32_t* idx, int64_t count, int n, double* bad, hipStream_t s);

@@ -1,6 +1,8 @@
 // kernels_field.hip -- velocity/pressure corrections, divergence diagnostics,
 // the outlet-flux reduction behind the Darcy permeability, and small vector
 // kernels.  Reductions are deterministic (fixed block order) and FP64.
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace ns3d {
@@ -198,7 +200,22 @@ __global__ void __launch_bounds__(kBlock) perm_fin(const double* partials, int n
     for (int t = 0; t < 5; ++t) result[t] = out[t];
 }
 
+// dst = src in 16-byte words, grid-stride (the copy that calibrates HBM
+// counters and measures the achievable streaming bandwidth).
+__global__ void __launch_bounds__(kBlock) copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                        int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+hipError_t launch_copy16(const void* src, void* dst, int64_t words, hipStream_t s) {
+  if (words <= 0) return hipSuccess;
+  const int64_t nb = std::min<int64_t>((words + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, (const uint4*)src, (uint4*)dst, words);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s) {
   if (n <= 0) return hipSuccess;

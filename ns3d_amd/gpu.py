@@ -33,7 +33,7 @@ EXPORTS = [
     "ns3d_compute_pressure_correction", "ns3d_compute_pressure_field", "ns3d_correct_velocity",
     "ns3d_correct_pressure", "ns3d_compute_div", "ns3d_compute_permeability", "ns3d_get_matrix",
     "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_set_precision",
-    "ns3d_set_mixed_min_rows", "ns3d_set_multigrid", "ns3d_multigrid_levels", "ns3d_sync", "ns3d_bicgstab_stencil_noalloc", "ns3d_noalloc_max_rows",
+    "ns3d_set_mixed_min_rows", "ns3d_set_multigrid", "ns3d_multigrid_levels", "ns3d_multigrid_level", "ns3d_copy_bandwidth", "ns3d_sync", "ns3d_bicgstab_stencil_noalloc", "ns3d_noalloc_max_rows",
     "ns3d_engine_create", "ns3d_engine_destroy", "ns3d_attach_engine", "ns3d_engine_stats",
     "ns3d_engine_set_graphs", "ns3d_engine_set_schedule", "ns3d_engine_get_schedule",
     "ns3d_footprint", "ns3d_ctx_bytes",
@@ -120,6 +120,8 @@ def lib():
         L.ns3d_set_mixed_min_rows.argtypes = [vp, i64]
         L.ns3d_set_multigrid.argtypes = [vp, i64]
         L.ns3d_multigrid_levels.argtypes = [vp, P(i32), P(i64)]
+        L.ns3d_multigrid_level.argtypes = [vp, i32, P(C.c_int32), P(C.c_float)]
+        L.ns3d_copy_bandwidth.argtypes = [i32, i64, i32, P(dbl)]
         L.ns3d_sync.argtypes = [vp]
         _lib = L
     return _lib
@@ -320,6 +322,15 @@ class NavierStokesGPU:
         _check(lib().ns3d_multigrid_levels(self.h, C.byref(lev), rows))
         return [int(rows[i]) for i in range(lev.value)]
 
+    def multigrid_level(self, level):
+        """(cells, coef[4][n]) of hierarchy level `level` >= 1 as on the device."""
+        n = self.multigrid_levels()[level]
+        cell = np.empty(n, np.int32)
+        coef = np.empty((4, n), np.float32)
+        _check(lib().ns3d_multigrid_level(self.h, int(level), cell.ctypes.data_as(C.POINTER(C.c_int32)),
+                                          coef.ctypes.data_as(C.POINTER(C.c_float))))
+        return cell, coef
+
     def nnz(self):
         out = np.zeros(4, np.int64)
         _check(lib().ns3d_matrix_nnz(self.h, _p(out, C.c_int64)))
@@ -383,6 +394,14 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+def copy_bandwidth(nbytes: int, reps: int = 20, device: int = 0) -> float:
+    """GB/s (read + written) of a plain device copy of nbytes (the library's
+    own copy kernel)."""
+    out = C.c_double(0.0)
+    _check(lib().ns3d_copy_bandwidth(int(device), int(nbytes), int(reps), C.byref(out)))
+    return out.value
 
 
 def footprint(shape, nfluid, mixed=False, engine=False):
