@@ -1673,7 +1673,8 @@ ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, in
   // partials; allocation rounding of up to 10 allocations per level
   if (mixed) b += 24 * nf + 48 * N / 7 + 32.0 * kMaxRedBlocks + 64.0 * 1024 * (8 + 10 * kMgMaxLevels);
   b += 64.0 * 1024 * (48 + 26 * (lanes - 1));          // allocation rounding (64 KiB per allocation)
-  *bytes = b;
+  // whole bytes: sums and differences of footprints stay exact in FP64
+  *bytes = std::ceil(b);
   return NS3D_OK;
 }
 
