@@ -335,9 +335,10 @@ ns3d_status ns3d_engine_get_schedule(ns3d_engine* eng, int out[3]);
 ns3d_status ns3d_engine_stats(ns3d_engine* eng, double out[32]);
 
 /* Upper bound of the device bytes a context of an nx x ny x nz image with
- * nfluid fluid voxels allocates (mixed != 0: with the mixed form's buffers;
- * lanes 1: alone, 3: attached to an engine), for admission control;
- * ns3d_ctx_bytes gives what a context did request. */
+ * nfluid fluid voxels allocates (mixed != 0: with the mixed form's buffers,
+ * the multigrid hierarchy included when the image could reach the default
+ * ns3d_set_multigrid threshold; lanes 1: alone, 3: attached to an engine),
+ * for admission control; ns3d_ctx_bytes gives what a context did request. */
 ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, int lanes, double* bytes);
 ns3d_status ns3d_ctx_bytes(ns3d_ctx* ctx, double* bytes);
 

@@ -1667,11 +1667,17 @@ ns3d_status ns3d_footprint(int nx, int ny, int nz, int64_t nfluid, int mixed, in
   const double lane = (2 + 7) * 8 * nmax + 8.0 * kPartialSlots * std::max(kMaxTilesPerSolve, kMaxRedBlocks) + 1024;
   b += lanes * lane + 4096;
   if (mixed) b += rows_all * (28 + 4) + lanes * nmax * (7 * 4 + 2 * 4 + 8);  // FP32 matrices; inner vectors, residual
-  // multigrid of the pressure system: level-0 vectors and aggregate map
-  // (r, z, p[2], q, agg); coarser levels hold at most N/7 cells and rows
-  // (grid 4 B per cell; cell, pk, coef, D^-1, agg, b, x 44 B per row);
-  // partials; allocation rounding of up to 10 allocations per level
-  if (mixed) b += 24 * nf + 48 * N / 7 + 32.0 * kMaxRedBlocks + 64.0 * 1024 * (8 + 10 * kMgMaxLevels);
+  // multigrid of the pressure system (from the default threshold on, where
+  // the pressure rows may reach it): level-0 vectors and aggregate map (r,
+  // z, p[2], q, agg); coarser levels hold at most N/7 cells and rows (grid
+  // 4 B per cell; cell, pk, coef, D^-1, agg, b, x 44 B per row); partials;
+  // allocation rounding of ten allocations per level (each level has at
+  // most 1/4 of the rows of the one above, down to kMgCoarseRows)
+  if (mixed && nf >= (double)kMgDefaultMinRows) {
+    int levels = 2;
+    for (double r = nf; r > kMgCoarseRows && levels < kMgMaxLevels; r /= 4) ++levels;
+    b += 24 * nf + 48 * N / 7 + 32.0 * kMaxRedBlocks + 64.0 * 1024 * (8 + 10 * levels);
+  }
   b += 64.0 * 1024 * (48 + 26 * (lanes - 1));          // allocation rounding (64 KiB per allocation)
   // whole bytes: sums and differences of footprints stay exact in FP64
   *bytes = std::ceil(b);

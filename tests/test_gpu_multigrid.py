@@ -234,3 +234,23 @@ def test_served_multigrid_requests_equal_solo():
         assert b["fallback"] == 0, q
         assert a["K"] == b["K"] and a["iterations"] == b["iterations"], (q, a["K"], b["K"])
         assert a["solver_iterations"] == b["solver_iterations"], q
+
+
+def test_footprint_bounds_multigrid_context():
+    """The admission projection covers the hierarchy, built at the first
+    pressure solve of a mixed context whose system reaches the default
+    threshold."""
+    img = generators.rsa_spheres((72, 66, 60), 4.0, 0.2, 31)[0]
+    nfluid = int((img == 0).sum())
+    g = prepared(img)
+    try:
+        g.set_precision(True, 1e-4)
+        assert g.np >= 200000, g.np
+        g.compute_pressure_correction(1e-6, 5000)
+        assert len(g.multigrid_levels()) >= 4
+        used = g.device_bytes()
+        bound = G.footprint(img.shape, nfluid, True)
+        print(f"[multigrid] footprint: used {used / 1e6:.1f} MB, bound {bound / 1e6:.1f} MB")
+        assert used <= bound <= 1.5 * used + 8e6, (used, bound)
+    finally:
+        g.close()
