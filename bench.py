@@ -454,6 +454,55 @@ def cpu_baseline_mix(mixres, args):
                 sample_seconds=round(dt, 3))
 
 
+def cpu_runs(args):
+    """The CPU-baseline plan's whole runs, each beside the GPU paths on the
+    same case: (1) the parity configuration's 64^3 RSA pack (r = 8, solid
+    fraction 0.40 target) to convergence at simpletol 1e-6 with the tight
+    inner tolerances, the oracle on every host thread; (2) the reference's
+    own timed case, tests/compare_algorithms.ini on the 64^3 sphere lattice
+    with SIMPLEC (README.md:177-183: 5.435 s, 48 outer iterations, K
+    3.28386e-12 on 4 threads), the oracle on 4 threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import oracle_lib
+    import make_config_parity as MC
+    from ns3d_amd import generators, solver
+    out = {}
+    host_threads = oracle_lib.lib().orc_num_threads()
+    raw, rec = MC.image("c1")
+    prm = dict(MC.params(raw.shape[0]))
+    c1 = dict(case="64^3 RSA r=8 porosity %.4f, SIMPLEC, flow x, simpletol 1e-6, inner 1e-10/1e-10/1e-6"
+              % rec["achieved_porosity"])
+    for label, kw in (("gpu_fp64", dict(precision=0)), ("gpu_mixed", dict(precision=1))):
+        t0 = time.perf_counter()
+        s = solver.solve(raw, method="SIMPLEC", simpletol=1e-6, maxitSIMPLE=3000, N1=4, N2=4, device=0, **prm,
+                         **kw)[1]
+        c1[label] = dict(seconds=round(time.perf_counter() - t0, 3), iterations=s["iterations"], K=s["K"],
+                         converged=s["converged"])
+        log(f"  C1 {label}: {c1[label]}")
+    o = oracle_lib.Oracle(raw, **prm)
+    s = o.solve("SIMPLEC", 1e-6, 3000, 4, 4)[1]
+    c1["cpu_oracle"] = dict(seconds=round(s["seconds"], 3), iterations=s["iterations"], K=s["K"], threads=host_threads)
+    log(f"  C1 cpu: {c1['cpu_oracle']}")
+    out["c1_converged"] = c1
+    lat = generators.sphere_lattice(64, 8, 3)
+    ini = dict(dx=2e-6, mu=1e-3, rho=1e3, p0=1e5, gradp=1e2, alpha=0.8, beta=1.0, vsolvertol=1e-5, psolvertol=1e-6,
+               pcsolvertol=1e-3, deltaRMR=1e-4, maxitsolver=500)
+    rd = dict(case="tests/compare_algorithms.ini, 64^3 sphere lattice, SIMPLEC (README.md:177-183)",
+              readme=dict(seconds=5.435, iterations=48, K=3.28386e-12, threads=4))
+    oracle_lib.lib().orc_set_threads(4)
+    o = oracle_lib.Oracle(lat, **ini)
+    s = o.solve("SIMPLEC", 1e-4, 500, 2, 2)[1]
+    rd["cpu_oracle_4_threads"] = dict(seconds=round(s["seconds"], 3), iterations=s["iterations"], K=s["K"])
+    oracle_lib.lib().orc_set_threads(host_threads)
+    t0 = time.perf_counter()
+    g = solver.solve(lat, method="SIMPLEC", simpletol=1e-4, maxitSIMPLE=500, N1=2, N2=2, device=0, **ini)[1]
+    rd["gpu_fp64"] = dict(seconds=round(time.perf_counter() - t0, 3), iterations=g["iterations"], K=g["K"])
+    log(f"  README case: {rd}")
+    out["readme_64_simplec"] = rd
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -461,7 +510,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5, help="untimed steps of the stream before the timed ones")
     ap.add_argument("--per-step", type=int, default=2, help="completed requests per step")
     ap.add_argument("--backlog", type=int, default=2, help="stream: queued requests kept beyond the running ones")
-    ap.add_argument("--section", default="all", choices=["all", "mix", "single"])
+    ap.add_argument("--section", default="all", choices=["all", "mix", "single", "cpu"])
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--mix-seed", type=int, default=20261203)
     ap.add_argument("--edges", default="32,48,64,96,128,160")
@@ -525,6 +574,10 @@ def main():
     copy_gbs = gpu.copy_bandwidth(1 << 30, 20)
     log(f"device copy (1 GiB read + written): {copy_gbs:.0f} GB/s")
 
+    if args.section == "cpu":
+        # the CPU-baseline plan's whole runs (not part of the default run)
+        print(json.dumps({"cpu_runs": cpu_runs(args), "device_copy_GBs": round(copy_gbs, 1)}), flush=True)
+        return
     mixres = mix_section(args) if args.section in ("all", "mix") else None
     if args.section == "mix" and mixres is None:
         return  # partial (profiling) run: no JSON line

@@ -1458,6 +1458,7 @@ int orc_solve(void* h, int method, double simpletol, int maxit, int N1, int N2, 
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 
 // RelaxControl over a scripted residual sequence, with run_driver's tracking
 // of the best residual; alpha/beta after each iteration.

@@ -71,6 +71,8 @@ def lib():
         L.orc_solve.argtypes = [vp, i32, dbl, i32, i32, i32, P(dbl), i32, P(dbl)]
         L.orc_prepare_image.argtypes = [P(C.c_uint8), i32, i32, i32, i32, i32, P(C.c_uint8), P(i32)]
         L.orc_num_threads.restype = i32
+        L.orc_set_threads.argtypes = [i32]
+        L.orc_set_threads.restype = None
         L.orc_relax_replay.argtypes = [P(dbl), i32, i32, dbl, dbl, dbl, P(dbl), P(dbl)]
         L.orc_set_horizon.argtypes = [vp, dbl]
         L.orc_horizon_it.restype = i32
