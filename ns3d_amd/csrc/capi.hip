@@ -115,6 +115,11 @@ struct MgHier {
   // solution vector), captured on a stream of their own; NS3D_MG_GRAPHS=0
   // launches them kernel by kernel
   bool graphs = true;
+  // sampled kernel times (with the context's timing on): events of the
+  // finest level's four kernels, and per kernel milliseconds, launches and
+  // algorithmic bytes
+  hipEvent_t tev[8] = {};
+  double ms[4] = {}, launches[4] = {}, bytes[4] = {};
   hipStream_t cap = nullptr;
   std::vector<std::pair<std::pair<const void*, int>, hipGraphExec_t>> execs;
 };
@@ -574,14 +579,37 @@ void mg_solve(ns3d_ctx* c, const float* rhs, float* x, double tol, int64_t maxit
   int chunk = 2;
   while (chunk < 16 && 2 * chunk <= H.hint * 3 / 4) chunk *= 2;
   int64_t launched = 0;
+  // with timing on, the first chunk goes kernel by kernel and its first
+  // iteration's finest-level kernels carry dispatch events
+  bool sample = c->tm.on;
+  if (sample && !H.tev[0])
+    for (auto& e : H.tev) HCHK(hipEventCreate(&e));
   while (H.hst->status == BS_RUNNING) {
-    if (H.graphs) {
+    const bool direct = !H.graphs || sample;
+    if (!direct) {
       HCHK(hipGraphLaunch(mg_graph(c, chunk), s));
     } else {
-      for (int q = 0; q < chunk; ++q) HCHK(launch_mg_iteration(H.G, H.lv, H.nlev, H.coarse_iters, s));
+      for (int q = 0; q < chunk; ++q)
+        HCHK(launch_mg_iteration(H.G, H.lv, H.nlev, H.coarse_iters, s, (sample && q == 0) ? H.tev : nullptr));
     }
     HCHK(hipMemcpyAsync(H.hst, H.G.st, sizeof(MgState), hipMemcpyDeviceToHost, s));
     sync();
+    if (sample && H.hst->it_c > 0) {
+      // the sampled iteration ran: its kernels' times and algorithmic bytes
+      const MgLevel& F = H.lv[0];
+      const double n = F.n, nnz = (double)c->sys[3].nnz, nc = H.lv[1].n;
+      const double cells = (double)F.nx * F.ny * F.nz;
+      const double b[4] = {nnz * 8 + 8 * n + 4 * cells + 4 * nc, nnz * 8 + 20 * n + 4 * nc, nnz * 8 + 16 * n,
+                           24 * n};
+      for (int k = 0; k < 4; ++k) {
+        float ms = 0.f;
+        HCHK(hipEventElapsedTime(&ms, H.tev[2 * k], H.tev[2 * k + 1]));
+        H.ms[k] += ms;
+        H.launches[k] += 1;
+        H.bytes[k] += b[k];
+      }
+    }
+    sample = false;
     launched += chunk;
     if (launched > 2 * maxit + 64) throw Invalid{"multigrid CG did not stop within its iteration cap"};
     chunk = 4;
@@ -1037,6 +1065,8 @@ void ns3d_destroy(ns3d_ctx* c) {
   if (c->ev_sync) (void)hipEventDestroy(c->ev_sync);
   if (c->mg.hst) (void)hipHostFree(c->mg.hst);
   for (auto& e : c->mg.execs) (void)hipGraphExecDestroy(e.second);
+  for (auto& e : c->mg.tev)
+    if (e) (void)hipEventDestroy(e);
   if (c->mg.cap) (void)hipStreamDestroy(c->mg.cap);
   for (auto& e : c->tev)
     if (e) (void)hipEventDestroy(e);
@@ -1563,6 +1593,16 @@ ns3d_status ns3d_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs
   if (a) (void)hipFree(a);
   if (b) (void)hipFree(b);
   return st;
+}
+
+ns3d_status ns3d_multigrid_timing(ns3d_ctx* c, double out[12]) {
+  if (!c || !out) return fail_invalid("ns3d_multigrid_timing: bad arguments");
+  for (int k = 0; k < 4; ++k) {
+    out[3 * k] = c->mg.ms[k];
+    out[3 * k + 1] = c->mg.launches[k];
+    out[3 * k + 2] = c->mg.bytes[k];
+  }
+  return NS3D_OK;
 }
 
 ns3d_status ns3d_set_mixed_min_rows(ns3d_ctx* c, int64_t rows) {

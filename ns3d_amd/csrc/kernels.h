@@ -314,8 +314,12 @@ hipError_t launch_mg_pk(const MgLevel& C, uint64_t* pk, int* bad, hipStream_t s)
 hipError_t launch_mg_galerkin(const MgLevel& F, const MgLevel& C, float* coef, float* invd, hipStream_t s);
 // r = rhs, x = 0, state for tolerance tol (relative to |rhs|) and the cap.
 hipError_t launch_mg_init(const MgFcg& G, const float* rhs, double tol, long long maxit, hipStream_t s);
-// One flexible-CG iteration over levels lv[0..nlev) (nlev >= 2).
-hipError_t launch_mg_iteration(const MgFcg& G, const MgLevel* lv, int nlev, int coarse_iters, hipStream_t s);
+// One flexible-CG iteration over levels lv[0..nlev) (nlev >= 2).  ev (may
+// be null): {start, stop} events for each of the finest level's four
+// kernels (first restriction, last prolongation, direction, update), taken
+// from their own dispatches.
+hipError_t launch_mg_iteration(const MgFcg& G, const MgLevel* lv, int nlev, int coarse_iters, hipStream_t s,
+                               const hipEvent_t* ev = nullptr);
 
 // ---- fields, corrections, diagnostics (kernels_field.hip) -------------------
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);

@@ -204,24 +204,15 @@ __global__ void __launch_bounds__(kBlock) perm_fin(const double* partials, int n
 // counters and measures the achievable streaming bandwidth).
 __global__ void __launch_bounds__(kBlock) copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                         int64_t n) {
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  // four independent 16-byte loads in flight per thread
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-    dst[i] = a;
-    dst[i + stride] = b;
-    dst[i + 2 * stride] = c;
-    dst[i + 3 * stride] = d;
-  }
-  for (; i < n; i += stride) dst[i] = src[i];
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    dst[i] = src[i];
 }
 
 }  // namespace
 
 hipError_t launch_copy16(const void* src, void* dst, int64_t words, hipStream_t s) {
   if (words <= 0) return hipSuccess;
-  const int64_t nb = std::min<int64_t>((words + kBlock - 1) / kBlock, 2048);
+  const int64_t nb = std::min<int64_t>((words + kBlock - 1) / kBlock, 8192);
   hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, (const uint4*)src, (uint4*)dst, words);
   return hipGetLastError();
 }

@@ -41,6 +41,8 @@
 // replay it) and launches queued after the solve stopped exit at once.
 // Reductions are fixed-order (per-block partials summed in block order by
 // every block of the next kernel): the result does not depend on timing.
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 
 #include "kernels.h"
@@ -615,23 +617,32 @@ hipError_t launch_mg_init(const MgFcg& G, const float* rhs, double tol, long lon
   return hipGetLastError();
 }
 
-hipError_t launch_mg_iteration(const MgFcg& G, const MgLevel* lv, int nlev, int coarse_iters, hipStream_t s) {
+// A launch with start / stop events taken from its own dispatch when ev is
+// given.
+template <typename K, typename... Args>
+void mg_launch(K kernel, dim3 grid, dim3 block, hipStream_t s, const hipEvent_t* ev, Args... args) {
+  if (ev) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev[0], ev[1], 0, args...);
+  else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+
+hipError_t launch_mg_iteration(const MgFcg& G, const MgLevel* lv, int nlev, int coarse_iters, hipStream_t s,
+                               const hipEvent_t* ev) {
   // down: restrict 0 -> 1 -> ... -> nlev-1
   for (int l = 0; l + 1 < nlev; ++l) {
     const dim3 grid(blocks_for(8 * (int64_t)lv[l + 1].n));
-    if (l == 0) hipLaunchKernelGGL(mg_restrict_kernel<true>, grid, dim3(kBlock), 0, s, lv[0], lv[1], G);
+    if (l == 0) mg_launch(mg_restrict_kernel<true>, grid, dim3(kBlock), s, ev, lv[0], lv[1], G);
     else hipLaunchKernelGGL(mg_restrict_kernel<false>, grid, dim3(kBlock), 0, s, lv[l], lv[l + 1], G);
   }
   hipLaunchKernelGGL(mg_coarse_kernel, dim3(1), dim3(kMgCoarseThreads), 0, s, lv[nlev - 1], G, coarse_iters);
   for (int l = nlev - 2; l >= 0; --l) {
     if (l == 0)
-      hipLaunchKernelGGL(mg_prolong_kernel<true>, dim3(G.nb), dim3(kBlock), 0, s, lv[0], lv[1], G);
+      mg_launch(mg_prolong_kernel<true>, dim3(G.nb), dim3(kBlock), s, ev ? ev + 2 : nullptr, lv[0], lv[1], G);
     else
       hipLaunchKernelGGL(mg_prolong_kernel<false>, dim3(blocks_for(lv[l].n)), dim3(kBlock), 0, s, lv[l], lv[l + 1],
                          G);
   }
-  hipLaunchKernelGGL(mg_dir_kernel, dim3(G.nb), dim3(kBlock), 0, s, G);
-  hipLaunchKernelGGL(mg_update_kernel, dim3(G.nb), dim3(kBlock), 0, s, G);
+  mg_launch(mg_dir_kernel, dim3(G.nb), dim3(kBlock), s, ev ? ev + 4 : nullptr, G);
+  mg_launch(mg_update_kernel, dim3(G.nb), dim3(kBlock), s, ev ? ev + 6 : nullptr, G);
   return hipGetLastError();
 }
 

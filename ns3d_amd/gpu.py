@@ -33,7 +33,7 @@ EXPORTS = [
     "ns3d_compute_pressure_correction", "ns3d_compute_pressure_field", "ns3d_correct_velocity",
     "ns3d_correct_pressure", "ns3d_compute_div", "ns3d_compute_permeability", "ns3d_get_matrix",
     "ns3d_bicgstab_stencil", "ns3d_timing", "ns3d_timing_read", "ns3d_matrix_nnz", "ns3d_set_precision",
-    "ns3d_set_mixed_min_rows", "ns3d_set_multigrid", "ns3d_multigrid_levels", "ns3d_multigrid_level", "ns3d_copy_bandwidth", "ns3d_sync", "ns3d_bicgstab_stencil_noalloc", "ns3d_noalloc_max_rows",
+    "ns3d_set_mixed_min_rows", "ns3d_set_multigrid", "ns3d_multigrid_levels", "ns3d_multigrid_level", "ns3d_multigrid_timing", "ns3d_copy_bandwidth", "ns3d_sync", "ns3d_bicgstab_stencil_noalloc", "ns3d_noalloc_max_rows",
     "ns3d_engine_create", "ns3d_engine_destroy", "ns3d_attach_engine", "ns3d_engine_stats",
     "ns3d_engine_set_graphs", "ns3d_engine_set_schedule", "ns3d_engine_get_schedule",
     "ns3d_footprint", "ns3d_ctx_bytes",
@@ -122,6 +122,7 @@ def lib():
         L.ns3d_multigrid_levels.argtypes = [vp, P(i32), P(i64)]
         L.ns3d_multigrid_level.argtypes = [vp, i32, P(C.c_int32), P(C.c_float)]
         L.ns3d_copy_bandwidth.argtypes = [i32, i64, i32, P(dbl)]
+        L.ns3d_multigrid_timing.argtypes = [vp, P(dbl)]
         L.ns3d_sync.argtypes = [vp]
         _lib = L
     return _lib

@@ -224,6 +224,14 @@ class Solver:
                     res[(prec, kind, name)] = (float(out[o]), float(out[o + 1]), float(out[o + 2]))
         return res
 
+    def multigrid_timing(self):
+        """{kernel: (ms, launches, bytes)} of the multigrid sweeps' sampled
+        finest-level kernels (restrict, prolong, dir, update)."""
+        out = np.zeros(12)
+        _gpu._check(_gpu.lib().ns3d_multigrid_timing(self.context(), out.ctypes.data_as(C.POINTER(C.c_double))))
+        return {name: (float(out[3 * k]), float(out[3 * k + 1]), float(out[3 * k + 2]))
+                for k, name in enumerate(("restrict", "prolong", "dir", "update"))}
+
     def nnz(self):
         out = np.zeros(4, np.int64)
         _gpu._check(_gpu.lib().ns3d_matrix_nnz(self.context(), out.ctypes.data_as(C.POINTER(C.c_int64))))
