@@ -164,6 +164,10 @@ struct ns3d_ctx {
   // multigrid-preconditioned flexible CG (0: never)
   int64_t mg_min_rows = kMgDefaultMinRows;
   MgHier mg;
+  // BiCGSTAB restart threshold factor (0: Eigen's epsilon^2).  Set from
+  // NS3D_RESTART_EPS2 at creation: tests raise it to drive solves through
+  // the restart path, which Eigen's threshold reaches only rarely
+  double restart_eps2 = 0.0;
 
   Geo geo() const { return Geo{nx, ny, nz, pg, ug, vg, wg}; }
 
@@ -327,7 +331,8 @@ void stream_solve(ns3d_ctx* c, const SolveJob<T>& j) {
   const int kind = j.sid == 3 ? 1 : 0;
   const int prec = sizeof(T) == 4 ? 1 : 0;
   // one solve: its descriptor travels by value in the kernel arguments
-  const SolveDesc<T> d{j.M, j.w, j.x, j.b, j.tol, (long long)j.maxit, j.L->partials, kMaxTilesPerSolve, 0, 0, 0};
+  const SolveDesc<T> d{j.M, j.w, j.x, j.b, j.tol, (long long)j.maxit, j.L->partials, kMaxTilesPerSolve, 0, 0, 0,
+                       c->restart_eps2};
   queued_solve<T>(c->stream, d, j.L->hstate, *j.hint, c->tm.on ? c->tev.data() : nullptr,
                   [&] { HCHK(ctx_sync(c)); },
                   [&] {
@@ -364,7 +369,8 @@ void run_solves(ns3d_ctx* c, SolveJob<T>* jobs, int n) {
   HCHK(hipEventRecord(c->ev[0], c->stream));
   for (int i = 0; i < n; ++i) {
     const SolveJob<T>& j = jobs[i];
-    const SolveDesc<T> d{j.M, j.w, j.x, j.b, j.tol, (long long)j.maxit, j.L->partials, kMaxTilesPerSolve, 0, 0, 0};
+    const SolveDesc<T> d{j.M, j.w, j.x, j.b, j.tol, (long long)j.maxit, j.L->partials, kMaxTilesPerSolve, 0, 0, 0,
+                         c->restart_eps2};
     req[i].prec = sizeof(T) == 4 ? 1 : 0;
     if constexpr (sizeof(T) == 4) req[i].d32 = d;
     else req[i].d64 = d;
@@ -944,6 +950,7 @@ ns3d_status ns3d_create(const uint8_t* img, int nx, int ny, int nz, int device, 
       HCHK(hipSetDevice(device));
     }
     HCHK(hipGetDevice(&c->device));
+    if (const char* e = std::getenv("NS3D_RESTART_EPS2")) c->restart_eps2 = std::max(0.0, std::atof(e));
     HCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->own_stream = c->stream;
     HCHK(hipEventCreateWithFlags(&c->ev_sync, hipEventDisableTiming));
@@ -1359,6 +1366,7 @@ ns3d_status ns3d_bicgstab_stencil(int n, const double* diag, const double* off, 
   ns3d_ctx* c = new ns3d_ctx();
   ns3d_status st = guarded([&] {
     HCHK(hipGetDevice(&c->device));
+    if (const char* e = std::getenv("NS3D_RESTART_EPS2")) c->restart_eps2 = std::max(0.0, std::atof(e));
     HCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->own_stream = c->stream;
     HCHK(hipEventCreateWithFlags(&c->ev_sync, hipEventDisableTiming));

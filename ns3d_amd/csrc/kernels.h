@@ -61,6 +61,7 @@ struct BState {
   double rho_fresh;     // rho of iteration fresh_at, set by init / restart
   double r0_sqnorm, rhs_sqnorm, tol2, tol;
   double rr, r0v, ts, tt;
+  double eps2;          // restart threshold factor (SolveDesc::eps2, or Eigen's epsilon^2)
   long long it_a, it_c, ibase, fresh_at, maxit, restarts;
   int status;
   int pad;
@@ -142,6 +143,8 @@ struct SolveDesc {
   int pstride;
   int t0, t1;
   int pad;
+  // restart threshold: |rho| < eps2 |r0|^2 restarts (0: Eigen's epsilon^2)
+  double eps2;
 };
 // A tile of one solve's system, processed by one block: rows [begin, end)
 // (step 0), or chunks begin, begin + step, ... below end of its band's

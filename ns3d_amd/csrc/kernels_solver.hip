@@ -216,6 +216,7 @@ struct GDesc {
   NS3D_GLOBAL double* partials;
   int pstride;
   int t0, t1;
+  double eps2;
   // slot q of this solve's partials (tile-local index)
   __device__ __forceinline__ NS3D_GLOBAL double* slot(int q) const { return partials + (int64_t)q * pstride; }
 };
@@ -243,6 +244,7 @@ __device__ __forceinline__ GDesc<T> to_gdesc(const SolveDesc<T>& D) {
   G.pstride = D.pstride;
   G.t0 = D.t0;
   G.t1 = D.t1;
+  G.eps2 = D.eps2;
   return G;
 }
 template <typename T>
@@ -367,6 +369,7 @@ bicg_init_fin(Batch<T> B) {
   NS3D_GLOBAL BState* st = D.w.st;
   st->tol = D.tol;
   st->maxit = D.maxit;
+  st->eps2 = D.eps2 > 0 ? D.eps2 : kEps * kEps;
   st->r0_sqnorm = out[0];
   st->rhs_sqnorm = out[1];
   st->it_a = 0;
@@ -523,7 +526,7 @@ bicg_A_kernel(Batch<T> B) {
         stop = true;
         return;
       }
-      if (fabs(r0r) < kEps * kEps * st->r0_sqnorm) {
+      if (fabs(r0r) < st->eps2 * st->r0_sqnorm) {
         if (lead) { st->rr = rr2; st->status = BS_RESTART; }
         stop = true;
         return;
