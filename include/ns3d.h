@@ -263,11 +263,11 @@ ns3d_status ns3d_multigrid_level(ns3d_ctx* ctx, int level, int32_t* cell, float*
 /* Device time of the multigrid sweeps' finest-level kernels (with
  * ns3d_timing(ctx, 1): the first iteration of each sweep is launched with
  * dispatch events).  out[3k..3k+2] = milliseconds, sampled launches and
- * algorithmic bytes of kernel k: 0 the first restriction (nnz*8 + 8n + 4 per
- * grid cell + 4 per coarse row), 1 the last prolongation with smoothing
- * (nnz*8 + 20n + 4 per coarse row), 2 the direction update with A p (nnz*8 +
- * 16n), 3 the x / r update (24n); n pressure rows, nnz their stored
- * entries.  Not part of the reference. */
+ * algorithmic bytes of kernel k: 0 the pre-smoothed residual of the first
+ * restriction (nnz*8 + 12n), 1 the last prolongation with smoothing (nnz*8
+ * + 20n + 4 per coarse row), 2 the direction update with A p (nnz*8 + 16n),
+ * 3 the x / r update (24n); n pressure rows, nnz their stored entries.  Not
+ * part of the reference. */
 ns3d_status ns3d_multigrid_timing(ns3d_ctx* ctx, double out[12]);
 
 /* Measurement: a plain device copy of `bytes` (read once, written once) run

@@ -604,9 +604,7 @@ void mg_solve(ns3d_ctx* c, const float* rhs, float* x, double tol, int64_t maxit
       // the sampled iteration ran: its kernels' times and algorithmic bytes
       const MgLevel& F = H.lv[0];
       const double n = F.n, nnz = (double)c->sys[3].nnz, nc = H.lv[1].n;
-      const double cells = (double)F.nx * F.ny * F.nz;
-      const double b[4] = {nnz * 8 + 8 * n + 4 * cells + 4 * nc, nnz * 8 + 20 * n + 4 * nc, nnz * 8 + 16 * n,
-                           24 * n};
+      const double b[4] = {nnz * 8 + 12 * n, nnz * 8 + 20 * n + 4 * nc, nnz * 8 + 16 * n, 24 * n};
       for (int k = 0; k < 4; ++k) {
         float ms = 0.f;
         HCHK(hipEventElapsedTime(&ms, H.tev[2 * k], H.tev[2 * k + 1]));

@@ -18,7 +18,8 @@
 //
 // V-cycle on level l for right-hand side b (x = 0 to start):
 //   pre-smoothing   x1 = w D^-1 b                        (one Jacobi step)
-//   restriction     b' = P^T (b - A x1)                  (mg_restrict)
+//   restriction     b' = P^T (b - A x1)                  (mg_restrict; on the
+//                   finest level mg_resid0 + mg_gather0, which admits the iteration)
 //   coarse solve    e = V-cycle(l+1, b'), or on the coarsest level a few
 //                   Jacobi-preconditioned CG steps in one work-group (FP32)
 //   correction and  x2 = x1 + s P e;  x3 = x2 + w D^-1 (b - A x2)
@@ -259,45 +260,59 @@ __global__ void __launch_bounds__(kBlock) mg_galerkin_kernel(MgLevel F, MgLevel 
 
 // ---- V-cycle ------------------------------------------------------------------
 
-// b' = P^T (b - A w D^-1 b) on the coarse rows of C: eight consecutive
-// threads per coarse row, one per child block position (a, b, c), each the
-// residual of its fine row (0 where there is none), summed over the eight
-// lanes by a fixed butterfly.  The first restriction of an iteration (TOP)
-// also admits it: it sums the r.r partials of the previous update (or of
-// init) and stops the solve when converged or at the cap; later launches of
-// a stopped solve exit at once.
-template <bool TOP>
-__global__ void __launch_bounds__(kBlock) mg_restrict_kernel(MgLevel F, MgLevel C, MgFcg G) {
+// Admission of an iteration by its first kernel (block-uniform): sums the
+// r.r partials of the previous update (or of init) and stops the solve when
+// converged or at the cap; false when the solve has stopped.
+__device__ __forceinline__ bool mg_admit_first(const MgFcg& G, long long& it) {
   MgState* st = G.st;
-  long long it;
-  if (TOP) {
-    __shared__ int go;
-    __shared__ long long sit;
-    if (threadIdx.x == 0) {
-      go = st->status == BS_RUNNING;
-      sit = st->it_c;
-    }
-    __syncthreads();
-    if (!go) return;
-    it = sit;
-    double pa[1], rr[1];
-    partials_load<1>(G.part + kSlotRR * kMaxRedBlocks, G.nb, kMaxRedBlocks, pa);
-    partials_reduce<1>(pa, rr);
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (!(rr[0] > st->tol2 && it < st->maxit)) {
-      if (lead) {
-        st->rr = rr[0];
-        st->status = BS_DONE;
-      }
-      return;
-    }
+  __shared__ int go;
+  __shared__ long long sit;
+  if (threadIdx.x == 0) {
+    go = st->status == BS_RUNNING;
+    sit = st->it_c;
+  }
+  __syncthreads();
+  if (!go) return false;
+  it = sit;
+  double pa[1], rr[1];
+  partials_load<1>(G.part + kSlotRR * kMaxRedBlocks, G.nb, kMaxRedBlocks, pa);
+  partials_reduce<1>(pa, rr);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!(rr[0] > st->tol2 && it < st->maxit)) {
     if (lead) {
       st->rr = rr[0];
-      st->it_a = it;
+      st->status = BS_DONE;
     }
-  } else {
-    if (!mg_admit(st, it)) return;
+    return false;
   }
+  if (lead) {
+    st->rr = rr[0];
+    st->it_a = it;
+  }
+  return true;
+}
+
+// The residual of one fine row after the pre-smoothing step from zero:
+// b_r - sum_m c_m (w D^-1 b)_q.
+__device__ __forceinline__ float mg_presmoothed_residual(const MgLevel& F, int r, float w) {
+  MgPts P;
+  mg_row(F, r, P);
+  float ax = 0.f;
+#pragma unroll
+  for (int m = 0; m < 7; ++m) {
+    const int q = P.q[m];
+    ax += P.c[m] * (w * F.invd[q] * F.b[q]);
+  }
+  return F.b[r] - ax;
+}
+
+// b' = P^T (b - A w D^-1 b) on the coarse rows of C (levels below the
+// finest): eight consecutive threads per coarse row, one per child block
+// position (a, b, c), each the residual of its fine row (0 where there is
+// none), summed over the eight lanes by a fixed butterfly.
+__global__ void __launch_bounds__(kBlock) mg_restrict_kernel(MgLevel F, MgLevel C, MgFcg G) {
+  long long it;
+  if (!mg_admit(G.st, it)) return;
   const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t R = g >> 3;
   const int ch = (int)(g & 7);
@@ -312,18 +327,44 @@ __global__ void __launch_bounds__(kBlock) mg_restrict_kernel(MgLevel F, MgLevel 
     const int i = 2 * I + (ch >> 2), j = 2 * J + ((ch >> 1) & 1), k = 2 * K + (ch & 1);
     if (i < F.nx && j < F.ny && k < F.nz) {
       const int r = F.grid[((int64_t)i * F.ny + j) * F.nz + k];
-      if (r >= 0) {
-        const float w = G.omega;
-        MgPts P;
-        mg_row(F, r, P);
-        float ax = 0.f;
-#pragma unroll
-        for (int m = 0; m < 7; ++m) {
-          const int q = P.q[m];
-          ax += P.c[m] * (w * F.invd[q] * F.b[q]);
-        }
-        res = F.b[r] - ax;
-      }
+      if (r >= 0) res = mg_presmoothed_residual(F, r, G.omega);
+    }
+  }
+  res += __shfl_xor(res, 1, 64);
+  res += __shfl_xor(res, 2, 64);
+  res += __shfl_xor(res, 4, 64);
+  if (ch == 0 && R < C.n) C.b[R] = res;
+}
+
+// The finest level's restriction in two passes: the pre-smoothed residual
+// of every fine row into F.x (free until the last prolongation writes z),
+// row-parallel over the solve's blocks (this kernel admits the iteration),
+// then the children's sum per coarse row in the fixed butterfly order of
+// mg_restrict_kernel (same bits).
+__global__ void __launch_bounds__(kBlock) mg_resid0_kernel(MgLevel F, MgFcg G) {
+  long long it;
+  if (!mg_admit_first(G, it)) return;
+  const RowIter rr = block_rows(F.n);
+  for (int64_t r = rr.begin + threadIdx.x; r < rr.end; r += rr.step)
+    F.x[r] = mg_presmoothed_residual(F, (int)r, G.omega);
+}
+
+__global__ void __launch_bounds__(kBlock) mg_gather0_kernel(MgLevel F, MgLevel C, MgFcg G) {
+  long long it;
+  if (!mg_admit(G.st, it)) return;
+  const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t R = g >> 3;
+  const int ch = (int)(g & 7);
+  float res = 0.f;
+  if (R < C.n) {
+    const int64_t cc = C.cell[R];
+    const int K = (int)(cc % C.nz);
+    const int64_t t = cc / C.nz;
+    const int J = (int)(t % C.ny), I = (int)(t / C.ny);
+    const int i = 2 * I + (ch >> 2), j = 2 * J + ((ch >> 1) & 1), k = 2 * K + (ch & 1);
+    if (i < F.nx && j < F.ny && k < F.nz) {
+      const int r = F.grid[((int64_t)i * F.ny + j) * F.nz + k];
+      if (r >= 0) res = F.x[r];
     }
   }
   res += __shfl_xor(res, 1, 64);
@@ -630,8 +671,12 @@ hipError_t launch_mg_iteration(const MgFcg& G, const MgLevel* lv, int nlev, int 
   // down: restrict 0 -> 1 -> ... -> nlev-1
   for (int l = 0; l + 1 < nlev; ++l) {
     const dim3 grid(blocks_for(8 * (int64_t)lv[l + 1].n));
-    if (l == 0) mg_launch(mg_restrict_kernel<true>, grid, dim3(kBlock), s, ev, lv[0], lv[1], G);
-    else hipLaunchKernelGGL(mg_restrict_kernel<false>, grid, dim3(kBlock), 0, s, lv[l], lv[l + 1], G);
+    if (l == 0) {
+      mg_launch(mg_resid0_kernel, dim3(G.nb), dim3(kBlock), s, ev, lv[0], G);
+      hipLaunchKernelGGL(mg_gather0_kernel, grid, dim3(kBlock), 0, s, lv[0], lv[1], G);
+    } else {
+      hipLaunchKernelGGL(mg_restrict_kernel, grid, dim3(kBlock), 0, s, lv[l], lv[l + 1], G);
+    }
   }
   hipLaunchKernelGGL(mg_coarse_kernel, dim3(1), dim3(kMgCoarseThreads), 0, s, lv[nlev - 1], G, coarse_iters);
   for (int l = nlev - 2; l >= 0; --l) {
