@@ -559,7 +559,7 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="served forms: launch the engine's chunks kernel by kernel, not as HIP graphs")
     ap.add_argument("--lanes", type=int, default=3, help="served forms: engine streams")
-    ap.add_argument("--max-rows", type=float, default=4e6,
+    ap.add_argument("--max-rows", type=float, default=2e6,
                     help="served forms: cap on the fluid cells of the running requests (0: none)")
     ap.add_argument("--tune-maxit", type=int, default=40, help="autotuner: outer-iteration cap of its trials")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
