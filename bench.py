@@ -512,14 +512,151 @@ def cpu_runs(args):
     return out
 
 
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def field_errors(mixed, fp64):
+    return {k: rel_l2(mixed[k], fp64[k]) for k in "puvw"}
+
+
+def connected_p_rows(raw, direction=1):
+    """Mask over the pressure rows (C order of the prepared image) of the
+    fluid 6-connected to the inlet or outlet plane."""
+    import scipy.ndimage as ndi
+    from ns3d_amd import gpu
+    img = gpu.prepare_image(raw, direction, 0)
+    lab, _ = ndi.label(img == 0)
+    keep = np.setdiff1d(np.union1d(np.unique(lab[0]), np.unique(lab[-1])), [0])
+    with gpu.NavierStokesGPU(img) as g:
+        cells = g.index()[0] >= 0
+    return np.isin(lab[cells], keep)
+
+
+def precision_runs(args):
+    """The precision configuration run to convergence in the mixed form: one
+    384^3 Boolean image (radii U[6, 20], porosity 0.70, flow along x).
+
+    Run 1, Stokes limit (rho = --rho): the mixed form to simpletol.  The
+    plain FP64 path runs to a horizon of --prec-horizon outer iterations (its
+    converged solve of this image takes hours) and is compared with the mixed
+    form at the same iteration: |dK|/K from the RES rows, the fields p, u, v,
+    w by relative L2 against a mixed run capped at the horizon.
+
+    Run 2, Forchheimer points: rho = 1 and the gradients that give pore
+    Reynolds numbers of about --forch-re under Darcy's law with run 1's K
+    (Re = rho (v / phi) d / mu, v the Darcy velocity, d = rmin + rmax the
+    mean sphere diameter).  The mixed form runs every point to simpletol,
+    all on one engine; plain FP64 and mixed capped at --forch-horizon are
+    compared as in run 1; -grad p = mu / k1 v + rho / k2 v^2 is fitted to
+    the converged points.  Field errors: p, u, v, w over their node sets, and
+    p over the fluid connected to the inlet or outlet (p_connected)."""
+    from ns3d_amd import forchheimer, generators, solver
+    n = args.size
+    img = generators.boolean_spheres((n, n, n), args.rmin, args.rmax, args.porosity, args.seed)[0]
+    phi = 1.0 - float(img.mean())
+    d = args.rmin + args.rmax
+    log(f"precision configuration: {n}^3 Boolean spheres r=U[{args.rmin:g},{args.rmax:g}], porosity {phi:.4f}")
+    base = dict(dx=1.0, mu=1.0, p0=1.0, gradp=1.0 / n, alpha=0.8, beta=1.0, direction=1, border=0, method="SIMPLEC",
+                simpletol=args.simpletol, N1=4, N2=4, device=0, inner_tol=args.inner_tol,
+                mg_min_rows=args.mg_min_rows, verbose=2 if args.progress else 0, **TOLS)
+    out = dict(image=dict(shape=[n, n, n], generator="boolean", rmin=args.rmin, rmax=args.rmax, seed=args.seed,
+                          porosity=round(phi, 6)),
+               simpletol=args.simpletol, inner_tol=args.inner_tol)
+    parts = set(args.prec_parts.split(","))
+
+    conn = []
+
+    def horizon_pair(params, h, label):
+        rows64, s64 = solver.solve(img, **dict(params, precision=0, maxitSIMPLE=h))
+        log(f"  [{label} fp64 x{h}] K {s64['K']:.10e}, {s64['seconds']:.1f} s")
+        rowsm, sm = solver.solve(img, **dict(params, precision=1, maxitSIMPLE=h))
+        log(f"  [{label} mixed x{h}] K {sm['K']:.10e}, {sm['seconds']:.1f} s")
+        err = field_errors(sm["fields"], s64["fields"])
+        if not conn:
+            conn.append(connected_p_rows(img))
+        # pressure in pockets cut off from the inlet and outlet is fixed only up to a constant
+        err["p_connected"] = rel_l2(sm["fields"]["p"][conn[0]], s64["fields"]["p"][conn[0]])
+        return dict(outer_iterations=h, K_fp64=s64["K"], K_mixed=sm["K"], dK_over_K=abs(sm["K"] / s64["K"] - 1),
+                    field_rel_l2=err,
+                    seconds_fp64=round(s64["seconds"], 2), seconds_mixed=round(sm["seconds"], 2),
+                    pressure_iterations_fp64=s64["solver_work"][3], pressure_iterations_mixed=sm["solver_work"][3])
+
+    k_stokes = None
+    if "stokes" in parts:
+        prm = dict(base, rho=args.rho, maxitSIMPLE=args.maxit)
+        s = solver.Solver(img, **dict(prm, precision=1))
+        try:
+            rows, sm = s.run()
+        finally:
+            s.close()
+        log(f"  [stokes mixed] {sm['iterations']} outer its, converged {sm['converged']}, K {sm['K']:.10e}, "
+            f"{sm['seconds']:.1f} s")
+        k_stokes = sm["K"]
+        run1 = dict(rho=args.rho, gradp=1.0 / n, mixed=dict(
+            converged=int(sm["converged"]), outer_iterations=int(sm["iterations"]), K=sm["K"],
+            seconds=round(sm["seconds"], 2), RMR=float(rows[-1, 3]), solver_work=sm["solver_work"],
+            pore_Re=args.rho * (sm["v_moy"] / phi) * d / 1.0))
+        h = min(args.prec_horizon, int(sm["iterations"]))
+        run1["horizon"] = horizon_pair(prm, h, "stokes")
+        # the converged mixed K against its own value at the horizon: how far the horizon is from the end
+        run1["horizon"]["K_mixed_from_rows"] = float(rows[h - 1, 4])
+        out["stokes"] = run1
+    elif "stokes_horizon" in parts:
+        out["stokes"] = dict(rho=args.rho, gradp=1.0 / n,
+                             horizon=horizon_pair(dict(base, rho=args.rho), args.prec_horizon, "stokes"))
+    if "forch" in parts:
+        if k_stokes is None:
+            k_stokes = args.forch_k
+        if not k_stokes:
+            raise SystemExit("the Forchheimer points need run 1's K (--prec-parts stokes,forch or --forch-k)")
+        res = [float(x) for x in args.forch_re.split(",")]
+        gradps = [r * phi / (k_stokes * d) for r in res]  # mu = rho = 1
+        prm = dict(base, rho=1.0, maxitSIMPLE=args.maxit, precision=1)
+        points = []
+        for r, g in zip(res, gradps):
+            s = solver.Solver(img, **dict(prm, gradp=g))
+            try:
+                sm = s.run()[1]
+            finally:
+                s.close()
+            v = float(sm["v_moy"])
+            points.append(dict(target_pore_Re=r, gradp=g, v_moy=v, K=float(sm["K"]), K_apparent=v / g,
+                               pore_Re=v / phi * d, converged=int(sm["converged"]), iterations=int(sm["iterations"]),
+                               seconds=round(sm["seconds"], 2)))
+            p = points[-1]
+            log(f"  [forchheimer Re~{r:g}] gradp {g:.4e}: {p['iterations']} outer its, converged {p['converged']}, "
+                f"K_app {p['K_apparent']:.6e}, pore Re {p['pore_Re']:.2f}, {p['seconds']:.1f} s")
+        good = [p for p in points if p["converged"] and p["v_moy"] > 0]
+        fit = forchheimer.fit([p["gradp"] for p in good], [p["v_moy"] for p in good], 1.0, 1.0) \
+            if len(good) >= 2 else None
+        run2 = dict(rho=1.0, K_stokes=k_stokes, mean_diameter=d, points=points, fit=fit, horizons=[])
+        for r, g in zip(res, gradps):
+            hp = horizon_pair(dict(base, rho=1.0, gradp=g), args.forch_horizon, f"forchheimer Re~{r:g}")
+            hp["target_pore_Re"] = r
+            run2["horizons"].append(hp)
+        out["forchheimer"] = run2
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed steps of the served mixed stream")
+    ap.add_argument("--steps", type=int, default=24,
+                    help="timed steps of the served mixed stream (24 x 2 requests: one cycle of the 48 images)")
     ap.add_argument("--warmup", type=int, default=5, help="untimed steps of the stream before the timed ones")
     ap.add_argument("--per-step", type=int, default=2, help="completed requests per step")
     ap.add_argument("--backlog", type=int, default=2, help="stream: queued requests kept beyond the running ones")
-    ap.add_argument("--section", default="all", choices=["all", "mix", "single", "cpu"])
+    ap.add_argument("--section", default="all", choices=["all", "mix", "single", "cpu", "precision"],
+                    help="precision: the 384^3 precision configuration run to convergence (not in the default run)")
+    ap.add_argument("--prec-parts", default="stokes,forch",
+                    help="precision: stokes (run 1), stokes_horizon (its FP64 / mixed comparison only), forch (run 2)")
+    ap.add_argument("--prec-horizon", type=int, default=30,
+                    help="precision: outer iterations of the FP64 / mixed comparison of run 1")
+    ap.add_argument("--forch-re", default="1,3,10", help="precision: target pore Reynolds numbers of run 2")
+    ap.add_argument("--forch-horizon", type=int, default=10,
+                    help="precision: outer iterations of the FP64 / mixed comparison of each point of run 2")
+    ap.add_argument("--forch-k", type=float, default=0.0, help="precision: run 1's K when run 2 runs alone")
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--mix-seed", type=int, default=20261203)
     ap.add_argument("--edges", default="32,48,64,96,128,160")
@@ -583,6 +720,10 @@ def main():
     copy_gbs = gpu.copy_bandwidth(1 << 30, 20)
     log(f"device copy (1 GiB read + written): {copy_gbs:.0f} GB/s")
 
+    if args.section == "precision":
+        print(json.dumps({"precision_config": precision_runs(args), "device_copy_GBs": round(copy_gbs, 1)}),
+              flush=True)
+        return
     if args.section == "cpu":
         # the CPU-baseline plan's whole runs (not part of the default run)
         print(json.dumps({"cpu_runs": cpu_runs(args), "device_copy_GBs": round(copy_gbs, 1)}), flush=True)
