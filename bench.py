@@ -639,6 +639,106 @@ def precision_runs(args):
     return out
 
 
+def serving_runs(args):
+    """The serving workload on a bounded sample: the first --serve-requests
+    requests of the serving recipe (ns3d_amd.mix.serving_mix: cubic edge
+    log-uniform in [48, 512], Boolean spheres r U[6, 20], porosity U[0.30,
+    0.90], random flow direction and method) whose edge is at most
+    --serve-max-edge, Stokes limit (rho --rho), simpletol --simpletol.
+
+    (1) Each request alone on the plain FP64 path: its K and seconds.
+    (2) The mixed form served with FP64 fallback on non-convergence
+    (FALLBACK_UNCONVERGED: an unconverged or failed mixed run is continued
+    in FP64 from its fields): all requests arriving at once (saturating
+    load), then Poisson arrivals at the fractions --serve-loads of that
+    burst's rate.  Each load reports solves/s (completed requests over the
+    time from the first arrival to the last completion), p50/p99 latency
+    (arrival to result) and |dK|/K of every result against its FP64 K,
+    gated at 1e-4 like the headline."""
+    from ns3d_amd import mix, solver
+    reqs = mix.serving_mix(args.serve_requests, args.mix_seed, max_edge=args.serve_max_edge)
+    skipped = reqs[-1][1]["skipped_before"]
+    log(f"serving sample: {len(reqs)} requests of edge <= {args.serve_max_edge} ({skipped} larger ones of the "
+        f"recipe skipped), {sum(d['fluid'] for _, d in reqs)} fluid cells")
+
+    def params(img, d, precision):
+        p = mix_params(img, args, precision)
+        # the pressure drop is unit along the request's own flow axis
+        p.update(method=d["method"], direction=d["direction"], gradp=1.0 / img.shape[d["direction"] - 1])
+        return p
+
+    plain = []
+    for q, (img, d) in enumerate(reqs):
+        s = solver.Solver(img, **params(img, d, 0))
+        try:
+            t1 = time.perf_counter()
+            sm = s.run()[1]
+            sec = time.perf_counter() - t1
+        finally:
+            s.close()
+        plain.append(dict(K=sm["K"], converged=int(sm["converged"]), iterations=int(sm["iterations"]), seconds=sec))
+        log(f"    plain {q}: {d['shape'][0]}^3 {d['method']} dir {d['direction']} {sm['iterations']} outer its, "
+            f"{sec:.2f} s")
+
+    def serve(arrivals, label):
+        srv = solver.Server(device=0, budget_bytes=args.budget_gb * 1e9, max_slots=args.slots, chunk=args.chunk,
+                            fallback=solver.FALLBACK_UNCONVERGED)
+        try:
+            srv.set_schedule(args.chunk, args.lanes, not args.no_graphs)
+            srv.set_policy(largest_first=False, max_rows=args.max_rows)
+            t0 = time.perf_counter()
+            tickets = []
+            for q, (img, d) in enumerate(reqs):
+                dt = arrivals[q] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                tickets.append(srv.submit(img, **params(img, d, 1)))
+            res = [srv.wait(t) for t in tickets]
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        finally:
+            srv.close()
+        lat = [r["t_end"] - r["t_submit"] for r in res]
+        dks, over, stalled = [], [], []
+        for q, (r, p) in enumerate(zip(res, plain)):
+            if not p["converged"]:
+                stalled.append(q)
+                continue
+            dk = abs(r["K"] / p["K"] - 1) if r["converged"] else float("inf")
+            dks.append(dk)
+            if not dk <= 1e-4:
+                over.append(dict(request=q, dK_over_K=dk, converged=int(r["converged"])))
+        out = dict(solves_per_s=round(len(res) / wall, 4), wall_s=round(wall, 2),
+                   latency_s=dict(p50=round(pct(lat, 50), 3), p99=round(pct(lat, 99), 3)),
+                   gate=dict(tolerance=1e-4, requests=len(dks), within=len(dks) - len(over), over=over,
+                             fp64_unconverged=stalled, passed=not over),
+                   max_dK_over_K=max(dks) if dks else None, fallbacks=int(st["fallbacks"]),
+                   peak_running=int(st["peak_running"]))
+        log(f"  [{label}] {len(res)} requests in {wall:.2f} s ({out['solves_per_s']} solves/s), latency p50 "
+            f"{out['latency_s']['p50']} s p99 {out['latency_s']['p99']} s, max dK/K {out['max_dK_over_K']}, "
+            f"fallbacks {out['fallbacks']}")
+        return out
+
+    loads = {}
+    loads["burst"] = serve([0.0] * len(reqs), "burst")
+    rng = np.random.default_rng(args.mix_seed)
+    for f in (float(x) for x in args.serve_loads.split(",")):
+        rate = f * loads["burst"]["solves_per_s"]
+        arr = np.concatenate([[0.0], np.cumsum(rng.exponential(1.0 / rate, size=len(reqs) - 1))])
+        loads[f"poisson_{f:g}"] = dict(arrival_rate=round(rate, 4), **serve(arr, f"poisson {rate:.3f}/s"))
+    plain_wall = sum(p["seconds"] for p in plain)
+    return dict(workload=dict(recipe="serving: cubic edge log-uniform [48,512], Boolean spheres r U[6,20], porosity "
+                                     "U[0.30,0.90], random direction and method; Stokes limit; mixed with FP64 "
+                                     "fallback on non-convergence",
+                              requests=len(reqs), max_edge=args.serve_max_edge, skipped_larger=skipped,
+                              fluid_cells=int(sum(d["fluid"] for _, d in reqs)),
+                              images=[{k: d[k] for k in ("recipe_index", "shape", "porosity", "direction", "method")}
+                                      for _, d in reqs]),
+                plain_fp64=dict(solves_per_s=round(len(reqs) / plain_wall, 4), seconds=round(plain_wall, 2),
+                                converged=sum(p["converged"] for p in plain)),
+                loads=loads, speedup_burst_vs_plain_fp64=round(loads["burst"]["solves_per_s"] * plain_wall / len(reqs), 3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -647,8 +747,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5, help="untimed steps of the stream before the timed ones")
     ap.add_argument("--per-step", type=int, default=2, help="completed requests per step")
     ap.add_argument("--backlog", type=int, default=2, help="stream: queued requests kept beyond the running ones")
-    ap.add_argument("--section", default="all", choices=["all", "mix", "single", "cpu", "precision"],
-                    help="precision: the 384^3 precision configuration run to convergence (not in the default run)")
+    ap.add_argument("--section", default="all", choices=["all", "mix", "single", "cpu", "precision", "serving"],
+                    help="precision: the 384^3 precision configuration run to convergence; serving: a bounded "
+                         "sample of the serving workload (neither in the default run)")
+    ap.add_argument("--serve-requests", type=int, default=20, help="serving: requests in the sample")
+    ap.add_argument("--serve-max-edge", type=int, default=128, help="serving: largest edge in the sample")
+    ap.add_argument("--serve-loads", default="0.5", help="serving: Poisson rates as fractions of the burst rate")
     ap.add_argument("--prec-parts", default="stokes,forch",
                     help="precision: stokes (run 1), stokes_horizon (its FP64 / mixed comparison only), forch (run 2)")
     ap.add_argument("--prec-horizon", type=int, default=30,
@@ -720,6 +824,9 @@ def main():
     copy_gbs = gpu.copy_bandwidth(1 << 30, 20)
     log(f"device copy (1 GiB read + written): {copy_gbs:.0f} GB/s")
 
+    if args.section == "serving":
+        print(json.dumps({"serving_config": serving_runs(args), "device_copy_GBs": round(copy_gbs, 1)}), flush=True)
+        return
     if args.section == "precision":
         print(json.dumps({"precision_config": precision_runs(args), "device_copy_GBs": round(copy_gbs, 1)}),
               flush=True)

@@ -1,5 +1,6 @@
 """Seeded request mixes for the batched and served forms.
 
+request_mix: the batching workload.
 A mix follows the batching workload's recipe: per image, edges drawn
 independently per axis from a set (anisotropic boxes allowed), porosity
 uniform in a range, and the microstructure generator split evenly among
@@ -10,6 +11,8 @@ with the recipe's radii whatever the box.  Image q uses seed base_seed + q.  An 
 does not connect the inlet to the outlet along the flow axis has no
 permeability to compute, so it is drawn again with the next seed (the seed
 used is recorded).
+
+serving_mix: the serving workload (see its docstring).
 """
 from __future__ import annotations
 
@@ -69,4 +72,62 @@ def request_mix(count: int, base_seed: int = 20261203, edges=(32, 48, 64, 96), p
             seed += 100003  # redraw
         else:
             raise RuntimeError(f"no percolating image for request {q} after {max_tries} draws")
+    return out
+
+
+METHODS = ("SIMPLEC", "SIMPLER", "SIMPLERR", "IDEAL")
+
+
+def serving_draw(q: int, base_seed: int = 20261203, edge_range=(48, 512)):
+    """Edge, flow direction and method of the serving recipe's request q
+    (drawn before its image, so a bounded sample can skip large requests
+    without generating them)."""
+    lo, hi = float(edge_range[0]), float(edge_range[1])
+    if not (0 < lo <= hi):
+        raise ValueError("edge_range must be 0 < lo <= hi")
+    rng = SplitMix64((base_seed + q) ^ 0x5DEECE66D)
+    edge = int(round(np.exp(np.log(lo) + (np.log(hi) - np.log(lo)) * rng.uniform())))
+    edge = min(max(edge, int(lo)), int(hi))
+    direction = 1 + int(rng.uniform() * 3) % 3
+    method = METHODS[int(rng.uniform() * 4) % 4]
+    return edge, direction, method
+
+
+def serving_mix(count: int, base_seed: int = 20261203, edge_range=(48, 512), porosity=(0.30, 0.90),
+                radii=(6.0, 20.0), max_edge: int | None = None, max_tries: int = 20):
+    """The serving workload's requests: request q (seed base_seed + q) is a
+    cubic image whose edge is log-uniform in edge_range, drawn from the
+    precision workload's generator (Boolean model of overlapping spheres,
+    radii uniform in radii) at a porosity uniform in the range, with a flow
+    direction uniform in {1, 2, 3} (x, y, z) and a method uniform among
+    SIMPLEC, SIMPLER, SIMPLERR and IDEAL.  An image whose fluid does not
+    connect inlet and outlet along its flow direction is drawn again with
+    the next seed, as in request_mix.
+
+    max_edge bounds a sample: requests of the recipe whose edge exceeds it
+    are skipped (their count is recorded as ``skipped_before``), the others
+    kept in the recipe's order.  Returns count (image, description) pairs;
+    the description's ``recipe_index`` is q."""
+    out, skipped, q = [], 0, 0
+    while len(out) < count:
+        seed = base_seed + q
+        edge, direction, method = serving_draw(q, base_seed, edge_range)
+        if max_edge is not None and edge > max_edge:
+            skipped += 1
+            q += 1
+            continue
+        for attempt in range(max_tries):
+            r2 = SplitMix64(seed)
+            phi = porosity[0] + (porosity[1] - porosity[0]) * r2.uniform()
+            img, got, count_s = generators.boolean_spheres((edge, edge, edge), radii[0], radii[1], phi, seed)
+            if percolates(img, direction - 1):
+                out.append((img, dict(recipe_index=q, seed=seed, shape=[edge] * 3, generator="boolean",
+                                      rmin=radii[0], rmax=radii[1], spheres=count_s, porosity_target=round(phi, 4),
+                                      porosity=round(float(got), 4), direction=direction, method=method,
+                                      fluid=int((img == 0).sum()), skipped_before=skipped)))
+                break
+            seed += 100003  # redraw
+        else:
+            raise RuntimeError(f"no percolating image for serving request {q} after {max_tries} draws")
+        q += 1
     return out
