@@ -36,6 +36,12 @@ FP64 and mixed, with per-kernel timings from dispatch events -- the roofline
 of the dominant FP64 kernel.  The CPU baseline is the oracle on the box's
 host cores over a bounded sample of the mix.
 
+Sections outside the default run: "precision" (the 384^3 Boolean image run
+to convergence in the mixed form, Stokes limit and Forchheimer points, each
+against plain FP64 at a horizon), "serving" (a bounded sample of the serving
+workload's recipe at saturating and half load) and "cpu" (the CPU baseline's
+whole runs).
+
 Output: one JSON line; progress on stderr.
 """
 from __future__ import annotations
@@ -237,14 +243,19 @@ def mix_section(args):
     gate = dk_gate(smx["all"], plain["results"])
     dks = [abs(m["K"] / plain["results"][r % len(reqs)]["K"] - 1) for r, m, _ in smx["all"]
            if m["converged"] and plain["results"][r % len(reqs)]["converged"]]
+    # fluid-cell updates: pressure rows x outer iterations of the completed solves, per second
+    fcu_plain = sum(float(r.get("np", 0)) * r["iterations"] for r in plain["results"]) / plain["wall"]
+    fcu_served = sum(float(m.get("np", 0)) * m["iterations"] for _, m, _ in smx["timed"]) / smx["wall"]
     forms = {
         "plain_fp64": dict(solves_per_s=round(len(reqs) / plain["wall"], 4), ms_per_pass=round(1e3 * plain["wall"], 1),
+                           fluid_cell_updates_per_s=round(fcu_plain, 1),
                            solves_per_s_on_timed_requests=round(plain_rate, 4),
                            converged=int(sum(r["converged"] for r in plain["results"])),
                            outer_iterations=int(sum(r["iterations"] for r in plain["results"])),
                            latency_s=dict(p50=round(pct(plain["seconds"], 50), 3),
                                           p99=round(pct(plain["seconds"], 99), 3))),
         "served_mixed": dict(solves_per_s=round(value, 4), timed_requests=n,
+                             fluid_cell_updates_per_s=round(fcu_served, 1),
                              latency_s=dict(p50=round(pct(smx["latency"], 50), 3), p99=round(pct(smx["latency"], 99), 3)),
                              max_dK_over_K=max(dks) if dks else None,
                              median_dK_over_K=float(np.median(dks)) if dks else None,
@@ -429,7 +440,9 @@ def single_section(args):
                mixed=dict(solves_per_s=round(mx["value"], 5), ms_per_solve=round(1e3 * mx["wall"] / len(mx["lat"]), 1),
                           K=mx["summs"][0]["K"], dK_over_K_vs_fp64=abs(mx["summs"][0]["K"] / s0["K"] - 1),
                           outer_iterations=mx["summs"][0]["iterations"]),
-               fluid_cell_updates_per_s_fp64=round(f64["counts"][0] * s0["iterations"] / (f64["wall"] / len(f64["lat"])), 1))
+               fluid_cell_updates_per_s_fp64=round(f64["counts"][0] * s0["iterations"] / (f64["wall"] / len(f64["lat"])), 1),
+               fluid_cell_updates_per_s_mixed=round(mx["counts"][0] * mx["summs"][0]["iterations"] /
+                                                    (mx["wall"] / len(mx["lat"])), 1))
     return dict(out=out, roof=roof, kernels=kernels, img=img, params=params, work=s0["solver_work"])
 
 
