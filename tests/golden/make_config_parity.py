@@ -68,6 +68,8 @@ CASES = {
     "c2_045_simpler_x": ("c2_045", "SIMPLER", 1),
     "c2_045_simplerr_x": ("c2_045", "SIMPLERR", 1),
     "c2_045_ideal_x": ("c2_045", "IDEAL", 1),
+    "c2_045_simplec_y": ("c2_045", "SIMPLEC", 2),
+    "c2_045_simplec_z": ("c2_045", "SIMPLEC", 3),
 }
 
 SIMPLETOL = 1e-6
