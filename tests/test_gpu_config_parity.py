@@ -85,7 +85,14 @@ def test_config_case_matches_oracle(case):
     assert summ["iterations"] == meta["final_it"] and not summ["converged"]
     dk = abs(summ["K"] / meta["K"] - 1.0)
     orows = fx["rows"]
-    assert rows.shape == orows.shape, (rows.shape, orows.shape)
+    if meta["method"] == "IDEAL":
+        # the reference's IDEAL writes no RES rows (IDEAL.cxx:138), and the
+        # oracle keeps none; the product keeps them in memory only (its RES
+        # file is checked by the output tests)
+        assert len(orows) == 0 and len(rows) == meta["final_it"]
+        orows = orows[:0]
+    else:
+        assert rows.shape == orows.shape, (rows.shape, orows.shape)
     if len(orows):
         assert np.array_equal(rows[:, 0], orows[:, 0])
         dks = np.abs(rows[:, 4] / orows[:, 4] - 1.0)
